@@ -1,0 +1,879 @@
+// brkga_decoder.hip -- MI355X (gfx950) BRKGA population decoder.
+//
+// One persistent launch decodes a whole population.  Work item = context =
+// (chromosome, printer machine).  Each 256-thread workgroup pulls contexts
+// from a device-side queue and runs the sequential placement procedure of
+// the reference (placement.py:29-91, binClassNew.py:45-174) for it:
+//
+//   * sequence decode        placement.py:109-133, full_native_decoder.py:961-1002
+//   * per-part feasibility   placement.py:46-53
+//   * first fit over bins    placement.py:55-64 (f64 area check :57-59)
+//   * per rotation: vacancy filter (numba_utils.py:131-157), then the exact
+//     collision test: an int8 MFMA correlation of the part mask against the
+//     bin's occupancy grid replaces rfft2/irfft2 (collision_backend.py:109-128)
+//   * best rotation by density -> row -> col   binClassNew.py:101-117
+//   * grid stamp + free-run ("vacancy") update binClassNew.py:147-174,
+//     numba_utils.py:161-188, cuda_batch_update.py:90-194
+//   * new bin at (0, H-1) with best rotation   placement.py:66-83
+//   * makespan = sum over bins (pt + pth + setup), max over machines
+//     placement.py:85-89,147-156
+//
+// Wave r of the workgroup owns rotation r of the part being placed, so the
+// up-to-four rotations of a bin are tested concurrently.
+//
+// Correlation as a GEMM.  For a group of 32 candidate top rows y_m and an
+// x-tile of 32 placements starting at x0 = 32t:
+//     count[y_m][x0+n] = sum_a sum_c sum_k A_{a,c}[m][k] * B_{a,c}[k][n]
+//     A_{a,c}[m][k] = G[y_m + a][x0 + 32c + k]        (grid bytes, k < 32)
+//     B_{a,c}[k][n] = P[a][32c + k - n]               (banded Toeplitz of mask row a)
+// issued as V_MFMA_I32_32X32X32_I8 with int32 accumulation, so counts are
+// exact integers.  B is built once per rotation at create time.
+//
+// Row pruning: a placement with top row y can only be collision-free if every
+// grid row y+a has a free run >= the mask row's longest 1-run (the vacancy
+// filter's condition, necessary, so results-neutral).  Candidate rows are
+// listed bottom-up and evaluated 32 at a time; the first group holding a zero
+// count contains the bottom-most valid row.  Dense mode evaluates every row
+// of the valid region and never exits early (same results).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/brkga_mi355x.h"
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+#define WG_THREADS 256
+#define MAX_ROT 4
+#define GROUP_ROWS 32
+#define XKEY 8192           // key = m * XKEY + x ; plates must have W < XKEY
+#define KEY_NONE 0x7fffffff
+
+enum {
+    ST_CALLS = 0, ST_CHROM, ST_CTX, ST_PLACE, ST_BINS, ST_TESTS, ST_GROUPS, ST_MFMA, ST_ALGO,
+    ST_OVERFLOW, ST_COUNT
+};
+
+struct DevProblem {
+    int n_parts, n_machines, n_types;
+    const double *thresholds;
+    const int *slot_type, *type_id;
+    const double *area;
+    const int *nrot, *best_rot, *rot_base;
+    const int *rot_h, *rot_w, *rot_nc;
+    const long long *mask_off, *dens_off, *bfrag_off;
+    const unsigned char *masks;
+    const int *dens;
+    const v4i *bfrag;
+    const int *plate_h, *plate_w;
+    const double *setup, *proc_time, *pth;
+    const unsigned char *fits;   // [M][n_types] rotation 0 fits the plate in some orientation
+};
+
+struct DevWork {
+    unsigned char *grids;    // [n_wg][max_bins][hmax][pitch]
+    int *vac;                // [n_wg][max_bins][hmax]
+    double *bin_area, *bin_pt, *bin_pth;  // [n_wg][max_bins]
+    int *bin_minrow;         // [n_wg][max_bins]
+    unsigned int *queue;
+    unsigned long long *counters;
+    int pitch, hmax, max_bins;
+};
+
+struct LaunchArgs {
+    const void *keys;
+    int key_f64;
+    long long n_chrom;
+    double *ctx_ms;          // [n_chrom * M]
+    int *log;                // nullable [n_chrom][n_parts][6]
+    int dense;
+    int tiles_per_pass;
+};
+
+__device__ __forceinline__ double load_key(const void *keys, int f64, long long idx) {
+    return f64 ? ((const double *)keys)[idx] : (double)((const float *)keys)[idx];
+}
+
+__device__ __forceinline__ int machine_of(double mv, const double *thr, int M) {
+    if (M <= 1) return 0;
+    if (mv <= thr[0]) return 0;
+    for (int m = 1; m < M - 1; m++)
+        if (mv > thr[m - 1] && mv <= thr[m]) return m;
+    return M - 1;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        int o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+// Correlation of one rotation against one bin over a list of candidate top
+// rows (descending).  Executed by one full wave.  Returns the key m*XKEY+x of
+// the bottom-most, then left-most, zero-overlap placement in the list, as
+// (list index, x), or KEY_NONE.
+template <int TP>
+__device__ int correlate_rows(const unsigned char *grid, int pitch, int W, int h, int w, int nc,
+                              const v4i *bfr, const short *cand, int ncand, int dense,
+                              unsigned long long &groups, unsigned long long &mfma) {
+    const int lane = threadIdx.x & 63;
+    const int T = (W - w + 1 + 31) >> 5;
+    int found = KEY_NONE;
+    for (int g0 = 0; g0 < ncand; g0 += GROUP_ROWS) {
+        const int nrows = min(GROUP_ROWS, ncand - g0);
+        const int msel = min(lane & 31, nrows - 1);
+        const int ysel = cand[g0 + msel];
+        const unsigned char *base = grid + (long long)ysel * pitch + ((lane >> 5) << 4);
+        int gkey = KEY_NONE;
+        groups++;
+        for (int t0 = 0; t0 < T; t0 += TP) {
+            v16i acc[TP];
+#pragma unroll
+            for (int t = 0; t < TP; t++) acc[t] = (v16i){};
+            const int tl = min(TP, T - t0);
+            for (int a = 0; a < h; a++) {
+                const unsigned char *rowp = base + (long long)a * pitch + (t0 << 5);
+                const v4i *bb = bfr + (long long)a * nc * 64 + lane;
+                for (int c = 0; c < nc; c++) {
+                    const v4i bf = bb[c * 64];
+#pragma unroll
+                    for (int t = 0; t < TP; t++) {
+                        if (t < tl) {
+                            const v4i av = *(const v4i *)(rowp + ((t + c) << 5));
+                            acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bf, acc[t], 0, 0, 0);
+                        }
+                    }
+                }
+            }
+            mfma += (unsigned long long)tl * h * nc;
+            // epilogue: lane holds column n = lane&31, rows m = 8*(i/4) + 4*(lane/32) + i%4
+            const int n = lane & 31;
+            const int mhi = (lane >> 5) << 2;
+#pragma unroll
+            for (int t = 0; t < TP; t++) {
+                if (t < tl) {
+                    const int x = ((t0 + t) << 5) + n;
+#pragma unroll
+                    for (int i = 0; i < 16; i++) {
+                        const int m = ((i >> 2) << 3) + mhi + (i & 3);
+                        if (acc[t][i] == 0 && m < nrows && x <= W - w) {
+                            const int key = m * XKEY + x;
+                            gkey = key < gkey ? key : gkey;
+                        }
+                    }
+                }
+            }
+        }
+        gkey = wave_min(gkey);
+        if (gkey != KEY_NONE) {
+            if (found == KEY_NONE) found = (g0 + gkey / XKEY) * XKEY + (gkey % XKEY);
+            if (!dense) break;
+        }
+    }
+    return found;
+}
+
+struct __align__(8) PlaceRes { int has, y, x, pad; };
+
+__global__ void reset_counters_kernel(unsigned int *queue) { *queue = 0u; }
+
+template <int TP>
+__global__ __launch_bounds__(WG_THREADS) void decode_kernel(DevProblem P, DevWork Wk, LaunchArgs L) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n = P.n_parts;
+    const int M = P.n_machines;
+    const int hmax = Wk.hmax;
+    // LDS layout
+    double *s_sv = (double *)smem;                                  // [n]
+    int *s_mach = (int *)(s_sv + n);                                // [n]
+    int *s_seq = s_mach + n;                                        // [n]
+    short *s_cand = (short *)(s_seq + n);                           // [MAX_ROT][hmax]
+    __shared__ PlaceRes s_res[MAX_ROT];
+    __shared__ int s_ctx, s_cnt, s_off, s_infeasible, s_decision, s_place_b, s_place_y,
+        s_place_x, s_place_rot, s_nb;
+    __shared__ int s_ncand[MAX_ROT];
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    const long long wg = blockIdx.x;
+    unsigned char *grids = Wk.grids + wg * (long long)Wk.max_bins * hmax * Wk.pitch;
+    int *vacs = Wk.vac + wg * (long long)Wk.max_bins * hmax;
+    double *b_area = Wk.bin_area + wg * Wk.max_bins;
+    double *b_pt = Wk.bin_pt + wg * Wk.max_bins;
+    double *b_pth = Wk.bin_pth + wg * Wk.max_bins;
+    int *b_minrow = Wk.bin_minrow + wg * Wk.max_bins;
+
+    unsigned long long c_ctx = 0, c_place = 0, c_bins = 0, c_tests = 0, c_groups = 0, c_mfma = 0,
+                       c_algo = 0, c_over = 0;
+    const long long n_ctx = L.n_chrom * M;
+
+    for (;;) {
+        if (tid == 0) s_ctx = (int)atomicAdd(Wk.queue, 1u);
+        __syncthreads();
+        const long long ctx = s_ctx;
+        if (ctx >= n_ctx) break;
+        const long long chrom = ctx / M;
+        const int mach = (int)(ctx % M);
+        const int H = P.plate_h[mach], W = P.plate_w[mach];
+        const double plate_area = (double)H * (double)W;
+        if (tid == 0) { s_cnt = 0; s_off = 0; s_infeasible = 0; s_nb = 0; }
+        // ---- sequence decode (placement.py:109-133) ----
+        for (int j = tid; j < n; j += WG_THREADS) {
+            const long long kb = chrom * 2LL * n;
+            s_sv[j] = load_key(L.keys, L.key_f64, kb + j);
+            s_mach[j] = machine_of(load_key(L.keys, L.key_f64, kb + n + j), P.thresholds, M);
+        }
+        __syncthreads();
+        for (int j = tid; j < n; j += WG_THREADS) {
+            const int mj = s_mach[j];
+            if (mj < mach) atomicAdd(&s_off, 1);
+            if (mj != mach) continue;
+            atomicAdd(&s_cnt, 1);
+            const double kj = s_sv[j];
+            int rank = 0;
+            for (int k = 0; k < n; k++)
+                if (s_mach[k] == mach && (s_sv[k] < kj || (s_sv[k] == kj && k < j))) rank++;
+            s_seq[rank] = j;
+            if (!P.fits[(long long)mach * P.n_types + P.slot_type[j]]) s_infeasible = 1;
+        }
+        __syncthreads();
+        const int cnt = s_cnt;
+        const int log_off = s_off;
+        int *logp = L.log ? L.log + (chrom * n + log_off) * 6 : nullptr;
+        if (s_infeasible) {   // placement.py:47-53
+            if (tid == 0) L.ctx_ms[ctx] = -1.0;
+            if (logp)
+                for (int k = tid; k < cnt * 6; k += WG_THREADS) logp[k] = -1;
+            __syncthreads();
+            continue;
+        }
+        c_ctx++;
+        bool overflow = false;
+        for (int i = 0; i < cnt; i++) {
+            const int ty = P.slot_type[s_seq[i]];
+            const double parea = P.area[ty];
+            const int nr = P.nrot[ty];
+            const int rb = P.rot_base[ty];
+            const int nb = s_nb;
+            bool placed = false;
+            for (int b = 0; b < nb && !placed; b++) {
+                if (b_area[b] + parea > plate_area) continue;   // placement.py:57-59
+                const unsigned char *grid = grids + (long long)b * hmax * Wk.pitch;
+                const int *vac = vacs + (long long)b * hmax;
+                // ---- wave `wave` handles rotation `wave` ----
+                if (wave < nr) {
+                    const int rg = rb + wave;
+                    const int h = P.rot_h[rg], w = P.rot_w[rg];
+                    short *cand = s_cand + wave * hmax;
+                    int nc_ = 0;
+                    if (h <= H && w <= W) {
+                        const int *den = P.dens + P.dens_off[rg];
+                        for (int yb = H - h; yb >= 0; yb -= 64) {
+                            const int y = yb - lane;
+                            bool ok = y >= 0;
+                            for (int a = 0; ok && a < h; a++) ok = vac[y + a] >= den[a];
+                            const unsigned long long bal = __ballot(ok);
+                            const int pos = __popcll(bal & ((1ull << lane) - 1ull));
+                            if (ok) cand[nc_ + pos] = (short)y;
+                            nc_ += __popcll(bal);
+                        }
+                        if (L.dense && nc_ > 0) {   // full valid region, bottom-up
+                            nc_ = H - h + 1;
+                            for (int k = lane; k < nc_; k += 64) cand[k] = (short)(H - h - k);
+                        }
+                    }
+                    int key = KEY_NONE;
+                    if (nc_ > 0) {
+                        c_tests++;
+                        c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
+                        const v4i *bfr = P.bfrag + P.bfrag_off[rg];
+                        key = correlate_rows<TP>(grid, Wk.pitch, W, h, w, P.rot_nc[rg], bfr, cand,
+                                                 nc_, L.dense, c_groups, c_mfma);
+                    }
+                    if (lane == 0) {
+                        s_res[wave].has = key != KEY_NONE;
+                        s_res[wave].y = key != KEY_NONE ? cand[key / XKEY] : 0;
+                        s_res[wave].x = key != KEY_NONE ? key % XKEY : 0;
+                    }
+                }
+                __syncthreads();
+                // ---- best rotation: density (new enclosure length) -> row -> col ----
+                if (tid == 0) {
+                    const int enc = H - b_minrow[b];
+                    int best = -1, bl = 0, brow = 0, bx = 0;
+                    for (int r = 0; r < nr; r++) {
+                        if (!s_res[r].has) continue;
+                        const int h = P.rot_h[rb + r];
+                        const int y = s_res[r].y, x = s_res[r].x, row = y + h - 1;
+                        const int nl = enc > H - y ? enc : H - y;
+                        if (best < 0 || nl < bl || (nl == bl && row > brow) ||
+                            (nl == bl && row == brow && x < bx)) {
+                            best = r; bl = nl; brow = row; bx = x;
+                        }
+                    }
+                    s_decision = best;
+                    if (best >= 0) {
+                        s_place_b = b; s_place_rot = best;
+                        s_place_y = brow - P.rot_h[rb + best] + 1; s_place_x = bx;
+                    }
+                }
+                __syncthreads();
+                placed = s_decision >= 0;
+            }
+            int pb, py, px, prot;
+            if (placed) {
+                pb = s_place_b; py = s_place_y; px = s_place_x; prot = s_place_rot;
+            } else {
+                if (nb >= Wk.max_bins) { overflow = true; break; }
+                pb = nb; prot = P.best_rot[ty];
+                py = H - P.rot_h[rb + prot]; px = 0;
+                // fresh bin: zero grid, full vacancy (full_native_decoder.py:2087-2089)
+                v4i *g4 = (v4i *)(grids + (long long)pb * hmax * Wk.pitch);
+                const int nvec = H * Wk.pitch / 16;
+                for (int k = tid; k < nvec; k += WG_THREADS) g4[k] = (v4i){0, 0, 0, 0};
+                int *vv = vacs + (long long)pb * hmax;
+                for (int k = tid; k < H; k += WG_THREADS) vv[k] = W;
+                if (tid == 0) {
+                    b_area[pb] = 0.0; b_pt[pb] = 0.0; b_pth[pb] = 0.0; b_minrow[pb] = H;
+                    s_nb = nb + 1;
+                }
+                c_bins++;
+                __syncthreads();
+            }
+            // ---- stamp the mask (binClassNew.py:155-160, cuda_batch_update.py:90-126) ----
+            {
+                const int rg = rb + prot;
+                const int h = P.rot_h[rg], w = P.rot_w[rg];
+                const unsigned char *mk = P.masks + P.mask_off[rg];
+                unsigned char *grid = grids + (long long)pb * hmax * Wk.pitch;
+                for (int k = tid; k < h * w; k += WG_THREADS) {
+                    const int a = k / w, bcol = k - a * w;
+                    if (mk[k]) grid[(long long)(py + a) * Wk.pitch + px + bcol] = 1;
+                }
+                __syncthreads();
+                // ---- free-run update of the stamped rows (numba_utils.py:161-188) ----
+                int *vv = vacs + (long long)pb * hmax;
+                for (int a = tid; a < h; a += WG_THREADS) {
+                    const unsigned char *row = grid + (long long)(py + a) * Wk.pitch;
+                    int run = 0, best = 0;
+                    for (int x0 = 0; x0 < W; x0 += 16) {
+                        const v4i q = *(const v4i *)(row + x0);
+                        const int lim = min(16, W - x0);
+#pragma unroll
+                        for (int j = 0; j < 16; j++) {
+                            if (j < lim) {
+                                const int byte = (q[j >> 2] >> ((j & 3) << 3)) & 0xff;
+                                run = byte ? 0 : run + 1;
+                                best = run > best ? run : best;
+                            }
+                        }
+                    }
+                    vv[py + a] = best;
+                }
+                if (tid == 0) {   // binClassNew.py:147-153,172-174 ; placement.py:81-82
+                    const long long mp = (long long)mach * P.n_types + ty;
+                    b_area[pb] = b_area[pb] + parea;
+                    b_pt[pb] = b_pt[pb] + P.proc_time[mp];
+                    const double pth = P.pth[mp];
+                    if (pth > b_pth[pb]) b_pth[pb] = pth;
+                    if (py < b_minrow[pb]) b_minrow[pb] = py;
+                    if (logp) {
+                        int *e = logp + i * 6;
+                        e[0] = mach; e[1] = P.type_id[ty]; e[2] = pb; e[3] = px;
+                        e[4] = py + h - 1; e[5] = prot;
+                    }
+                }
+                c_place++;
+                __syncthreads();
+            }
+        }
+        if (tid == 0) {
+            if (overflow) {
+                L.ctx_ms[ctx] = -2.0;
+                c_over++;
+            } else {
+                double total = 0.0;   // placement.py:86-89
+                const int nbf = s_nb;
+                for (int b = 0; b < nbf; b++) total = total + ((b_pt[b] + b_pth[b]) + P.setup[mach]);
+                L.ctx_ms[ctx] = total;
+            }
+        }
+        __syncthreads();
+    }
+    // ---- statistics ----
+    if (lane == 0) {
+        atomicAdd(&Wk.counters[ST_TESTS], c_tests);
+        atomicAdd(&Wk.counters[ST_GROUPS], c_groups);
+        atomicAdd(&Wk.counters[ST_MFMA], c_mfma * 32ull * 32ull * 32ull * 2ull);
+        atomicAdd(&Wk.counters[ST_ALGO], c_algo);
+    }
+    if (tid == 0) {
+        atomicAdd(&Wk.counters[ST_CTX], c_ctx);
+        atomicAdd(&Wk.counters[ST_PLACE], c_place);
+        atomicAdd(&Wk.counters[ST_BINS], c_bins);
+        atomicAdd(&Wk.counters[ST_OVERFLOW], c_over);
+    }
+}
+
+// fitness = max over machines, 1e16 if any machine infeasible (placement.py:147-156)
+__global__ void finalize_kernel(const double *ctx_ms, long long n_chrom, int M, double *out,
+                                unsigned long long *counters) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_chrom) return;
+    double worst = 0.0;
+    bool infeasible = false, overflow = false;
+    for (int m = 0; m < M; m++) {
+        const double v = ctx_ms[c * M + m];
+        if (v == -2.0) overflow = true;
+        else if (v < 0.0) infeasible = true;
+        else if (v > worst) worst = v;
+    }
+    out[c] = overflow ? NAN : (infeasible ? BRKGA_INFEASIBLE : worst);
+    if (c == 0) { atomicAdd(&counters[ST_CALLS], 1ull); atomicAdd(&counters[ST_CHROM], (unsigned long long)n_chrom); }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+static thread_local std::string g_create_error;
+
+struct brkga_handle {
+    int device = 0;
+    int n_cu = 0;
+    int n_wg = 0;
+    int dense = 0;
+    int tiles_per_pass = 8;
+    brkga_problem prob{};     // scalars only
+    DevProblem dp{};
+    DevWork dw{};
+    std::vector<void *> allocs;
+    // host-call staging
+    void *d_keys = nullptr; size_t d_keys_bytes = 0;
+    double *d_out = nullptr; size_t d_out_n = 0;
+    int *d_log = nullptr; size_t d_log_bytes = 0;
+    double *d_ctx = nullptr; size_t d_ctx_n = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ev_pending = false;
+    double kernel_ms = 0.0, last_ms = 0.0;
+    size_t smem = 0;
+    int max_bins_req = 0;
+    std::string err;
+};
+
+#define HIPCHK(h, call)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);                      \
+            return e_ == hipErrorOutOfMemory ? BRKGA_ENOMEM : BRKGA_EDEVICE;                   \
+        }                                                                                      \
+    } while (0)
+
+template <typename T>
+static int upload(brkga_handle *h, const T *src, size_t n, const T **dst) {
+    void *p = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    HIPCHK(h, hipMalloc(&p, bytes));
+    h->allocs.push_back(p);
+    if (n) HIPCHK(h, hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    *dst = (const T *)p;
+    return 0;
+}
+
+static int alloc_dev(brkga_handle *h, size_t bytes, void **dst) {
+    void *p = nullptr;
+    HIPCHK(h, hipMalloc(&p, std::max<size_t>(bytes, 16)));
+    h->allocs.push_back(p);
+    *dst = p;
+    return 0;
+}
+
+static int fail(brkga_handle *h, int code, const std::string &msg) {
+    if (h) h->err = msg; else g_create_error = msg;
+    return code;
+}
+
+static int validate(const brkga_problem *p, std::string &msg) {
+    char buf[256];
+    if (!p) { msg = "problem is NULL"; return BRKGA_EINVAL; }
+    if (p->n_parts < 1 || p->n_parts > 8192) { msg = "n_parts must be in [1, 8192]"; return BRKGA_EINVAL; }
+    if (p->n_machines < 1 || p->n_machines > 64) { msg = "n_machines must be in [1, 64]"; return BRKGA_EINVAL; }
+    if (p->n_types < 1 || p->n_rots < 1) { msg = "n_types and n_rots must be >= 1"; return BRKGA_EINVAL; }
+    if (!p->slot_type || !p->type_id || !p->area || !p->nrot || !p->best_rot || !p->rot_base ||
+        !p->rot_h || !p->rot_w || !p->mask_off || !p->masks || !p->plate_h || !p->plate_w ||
+        !p->setup || !p->proc_time || !p->proc_time_height || (p->n_machines > 1 && !p->thresholds)) {
+        msg = "a problem array is NULL"; return BRKGA_EINVAL;
+    }
+    for (int m = 0; m + 1 < p->n_machines; m++)
+        if (m > 0 && !(p->thresholds[m] >= p->thresholds[m - 1])) { msg = "thresholds must be ascending"; return BRKGA_EINVAL; }
+    for (int j = 0; j < p->n_parts; j++)
+        if (p->slot_type[j] < 0 || p->slot_type[j] >= p->n_types) {
+            snprintf(buf, sizeof buf, "slot_type[%d] out of range", j); msg = buf; return BRKGA_EINVAL;
+        }
+    int64_t mask_bytes = 0;
+    for (int r = 0; r < p->n_rots; r++) {
+        if (p->rot_h[r] < 1 || p->rot_w[r] < 1 || p->rot_h[r] > 4096 || p->rot_w[r] > 4096) {
+            snprintf(buf, sizeof buf, "rotation %d has bad shape %dx%d", r, p->rot_h[r], p->rot_w[r]); msg = buf; return BRKGA_EINVAL;
+        }
+        if (p->mask_off[r] < 0) { msg = "negative mask offset"; return BRKGA_EINVAL; }
+        mask_bytes = std::max<int64_t>(mask_bytes, p->mask_off[r] + (int64_t)p->rot_h[r] * p->rot_w[r]);
+    }
+    for (int64_t k = 0; k < mask_bytes; k++)
+        if (p->masks[k] > 1) { msg = "masks must be binary {0,1}"; return BRKGA_EINVAL; }
+    for (int t = 0; t < p->n_types; t++) {
+        if (p->nrot[t] < 1 || p->nrot[t] > MAX_ROT) { snprintf(buf, sizeof buf, "type %d: nrot must be 1..4", t); msg = buf; return BRKGA_EINVAL; }
+        if (p->rot_base[t] < 0 || p->rot_base[t] + p->nrot[t] > p->n_rots) { msg = "rot_base out of range"; return BRKGA_EINVAL; }
+        if (p->best_rot[t] < 0 || p->best_rot[t] >= p->nrot[t]) { msg = "best_rot out of range"; return BRKGA_EINVAL; }
+        if (!(p->area[t] > 0.0) || !std::isfinite(p->area[t])) { snprintf(buf, sizeof buf, "type %d: area must be finite and > 0", t); msg = buf; return BRKGA_EINVAL; }
+    }
+    for (int m = 0; m < p->n_machines; m++) {
+        const int H = p->plate_h[m], W = p->plate_w[m];
+        if (H < 1 || W < 1 || H > 4096 || W > 4096) { msg = "plate sizes must be in [1, 4096]"; return BRKGA_EINVAL; }
+        for (int t = 0; t < p->n_types; t++) {
+            const int r0 = p->rot_base[t];
+            const int h0 = p->rot_h[r0], w0 = p->rot_w[r0];
+            const bool fits0 = !((h0 > H || w0 > W) && (w0 > H || h0 > W));
+            const int rb = r0 + p->best_rot[t];
+            if (fits0 && (p->rot_h[rb] > H || p->rot_w[rb] > W)) {
+                snprintf(buf, sizeof buf,
+                         "type %d fits machine %d only in a rotation other than its best_rot; "
+                         "the placement procedure cannot open a bin for it", t, m);
+                msg = buf; return BRKGA_EINVAL;
+            }
+        }
+    }
+    return 0;
+}
+
+static int device_workspace(brkga_handle *h, int max_bins) {
+    DevWork &w = h->dw;
+    const size_t nwg = h->n_wg;
+    if (w.grids) {   // free the previous workspace (only grids/vac/bin arrays are resized)
+        for (void *p : {(void *)w.grids, (void *)w.vac, (void *)w.bin_area, (void *)w.bin_pt,
+                        (void *)w.bin_pth, (void *)w.bin_minrow}) {
+            auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+            if (it != h->allocs.end()) { (void)hipFree(p); h->allocs.erase(it); }
+        }
+    }
+    w.max_bins = max_bins;
+    void *p;
+    int rc;
+    if ((rc = alloc_dev(h, nwg * max_bins * (size_t)w.hmax * w.pitch, &p))) return rc;
+    w.grids = (unsigned char *)p;
+    if ((rc = alloc_dev(h, nwg * max_bins * (size_t)w.hmax * sizeof(int), &p))) return rc;
+    w.vac = (int *)p;
+    if ((rc = alloc_dev(h, nwg * max_bins * sizeof(double), &p))) return rc;
+    w.bin_area = (double *)p;
+    if ((rc = alloc_dev(h, nwg * max_bins * sizeof(double), &p))) return rc;
+    w.bin_pt = (double *)p;
+    if ((rc = alloc_dev(h, nwg * max_bins * sizeof(double), &p))) return rc;
+    w.bin_pth = (double *)p;
+    if ((rc = alloc_dev(h, nwg * max_bins * sizeof(int), &p))) return rc;
+    w.bin_minrow = (int *)p;
+    return 0;
+}
+
+extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkga_handle **out) {
+    if (!out) return fail(nullptr, BRKGA_EINVAL, "out is NULL");
+    *out = nullptr;
+    std::string msg;
+    int rc = validate(p, msg);
+    if (rc) return fail(nullptr, rc, msg);
+    brkga_handle *h = new (std::nothrow) brkga_handle();
+    if (!h) return fail(nullptr, BRKGA_ENOMEM, "host allocation failed");
+    brkga_options opt{};
+    if (o) opt = *o;
+    h->device = opt.device;
+    h->dense = opt.dense ? 1 : 0;
+    h->tiles_per_pass = opt.tiles_per_pass == 4 ? 4 : 8;
+    h->prob = *p;
+    auto bail = [&](int code) {
+        std::string e = h->err;
+        brkga_destroy(h);
+        return fail(nullptr, code, e);
+    };
+    if (hipSetDevice(h->device) != hipSuccess) { h->err = "hipSetDevice failed"; return bail(BRKGA_EDEVICE); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, h->device) != hipSuccess) { h->err = "hipGetDeviceProperties failed"; return bail(BRKGA_EDEVICE); }
+    h->n_cu = prop.multiProcessorCount;
+    const int wpc = opt.wgs_per_cu > 0 ? opt.wgs_per_cu : 2;
+    h->n_wg = std::max(1, h->n_cu * wpc);
+
+    const int M = p->n_machines, T = p->n_types, R = p->n_rots;
+    // ---- per-rotation tables: densities (BRKGA_alg3.py:351-360) and Toeplitz operands ----
+    std::vector<int> dens, nc(R);
+    std::vector<long long> dens_off(R), bfrag_off(R);
+    std::vector<v4i> bfrag;
+    int hmax = 0, wmax = 0, pitch = 64;
+    for (int m = 0; m < M; m++) { hmax = std::max(hmax, p->plate_h[m]); wmax = std::max(wmax, p->plate_w[m]); }
+    for (int r = 0; r < R; r++) {
+        const int rh = p->rot_h[r], rw = p->rot_w[r];
+        const uint8_t *mk = p->masks + p->mask_off[r];
+        dens_off[r] = (long long)dens.size();
+        for (int a = 0; a < rh; a++) {
+            int best = 0, cur = 0;
+            for (int b = 0; b < rw; b++) { cur = mk[a * rw + b] ? cur + 1 : 0; best = std::max(best, cur); }
+            dens.push_back(best);
+        }
+        nc[r] = (rw + 31 + 31) / 32;
+        bfrag_off[r] = (long long)bfrag.size();
+        for (int a = 0; a < rh; a++)
+            for (int c = 0; c < nc[r]; c++)
+                for (int lane = 0; lane < 64; lane++) {
+                    unsigned char bytes[16];
+                    for (int j = 0; j < 16; j++) {
+                        const int k = ((lane >> 5) << 4) + j, nn = lane & 31;
+                        const int col = 32 * c + k - nn;
+                        bytes[j] = (col >= 0 && col < rw) ? mk[a * rw + col] : 0;
+                    }
+                    v4i v;
+                    std::memcpy(&v, bytes, 16);
+                    bfrag.push_back(v);
+                }
+        for (int m = 0; m < M; m++) {
+            const int H = p->plate_h[m], W = p->plate_w[m];
+            if (rh > H || rw > W) continue;
+            const int Tt = (W - rw + 1 + 31) / 32;
+            pitch = std::max(pitch, 32 * (Tt + nc[r] - 1));
+        }
+    }
+    pitch = std::max(pitch, wmax);
+    pitch = (pitch + 63) & ~63;
+    std::vector<unsigned char> fits((size_t)M * T);
+    for (int m = 0; m < M; m++)
+        for (int t = 0; t < T; t++) {
+            const int r0 = p->rot_base[t];
+            const int h0 = p->rot_h[r0], w0 = p->rot_w[r0];
+            fits[(size_t)m * T + t] = !((h0 > p->plate_h[m] || w0 > p->plate_w[m]) &&
+                                        (w0 > p->plate_h[m] || h0 > p->plate_w[m]));
+        }
+    int64_t mask_bytes = 0;
+    for (int r = 0; r < R; r++) mask_bytes = std::max<int64_t>(mask_bytes, p->mask_off[r] + (int64_t)p->rot_h[r] * p->rot_w[r]);
+
+    DevProblem &d = h->dp;
+    d.n_parts = p->n_parts; d.n_machines = M; d.n_types = T;
+    std::vector<double> thr(std::max(M - 1, 0));
+    for (int m = 0; m + 1 < M; m++) thr[m] = p->thresholds[m];
+    if ((rc = upload(h, thr.data(), thr.size(), &d.thresholds)) ||
+        (rc = upload(h, p->slot_type, (size_t)p->n_parts, &d.slot_type)) ||
+        (rc = upload(h, p->type_id, (size_t)T, &d.type_id)) ||
+        (rc = upload(h, p->area, (size_t)T, &d.area)) ||
+        (rc = upload(h, p->nrot, (size_t)T, &d.nrot)) ||
+        (rc = upload(h, p->best_rot, (size_t)T, &d.best_rot)) ||
+        (rc = upload(h, p->rot_base, (size_t)T, &d.rot_base)) ||
+        (rc = upload(h, p->rot_h, (size_t)R, &d.rot_h)) ||
+        (rc = upload(h, p->rot_w, (size_t)R, &d.rot_w)) ||
+        (rc = upload(h, nc.data(), (size_t)R, &d.rot_nc)) ||
+        (rc = upload(h, (const long long *)p->mask_off, (size_t)R, &d.mask_off)) ||
+        (rc = upload(h, dens_off.data(), (size_t)R, &d.dens_off)) ||
+        (rc = upload(h, bfrag_off.data(), (size_t)R, &d.bfrag_off)) ||
+        (rc = upload(h, p->masks, (size_t)mask_bytes, &d.masks)) ||
+        (rc = upload(h, dens.data(), dens.size(), &d.dens)) ||
+        (rc = upload(h, bfrag.data(), bfrag.size(), &d.bfrag)) ||
+        (rc = upload(h, p->plate_h, (size_t)M, &d.plate_h)) ||
+        (rc = upload(h, p->plate_w, (size_t)M, &d.plate_w)) ||
+        (rc = upload(h, p->setup, (size_t)M, &d.setup)) ||
+        (rc = upload(h, p->proc_time, (size_t)M * T, &d.proc_time)) ||
+        (rc = upload(h, p->proc_time_height, (size_t)M * T, &d.pth)) ||
+        (rc = upload(h, fits.data(), fits.size(), &d.fits)))
+        return bail(rc);
+    h->prob.thresholds = nullptr;   // host pointers are not kept
+    DevWork &w = h->dw;
+    w.hmax = hmax;
+    w.pitch = pitch;
+    void *q;
+    if ((rc = alloc_dev(h, 64, &q))) return bail(rc);
+    w.queue = (unsigned int *)q;
+    if ((rc = alloc_dev(h, ST_COUNT * sizeof(unsigned long long), &q))) return bail(rc);
+    w.counters = (unsigned long long *)q;
+    if (hipMemset(w.counters, 0, ST_COUNT * sizeof(unsigned long long)) != hipSuccess) { h->err = "hipMemset failed"; return bail(BRKGA_EDEVICE); }
+    int mb = opt.max_bins > 0 ? opt.max_bins : std::min(p->n_parts, 32);
+    h->max_bins_req = mb;
+    if ((rc = device_workspace(h, mb))) return bail(rc);
+    h->smem = (size_t)p->n_parts * (8 + 4 + 4) + (size_t)MAX_ROT * hmax * 2 + 16;
+    if (h->smem > 60 * 1024) { h->err = "problem too large for the LDS layout (n_parts/plate rows)"; return bail(BRKGA_EINVAL); }
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) { h->err = "hipEventCreate failed"; return bail(BRKGA_EDEVICE); }
+    *out = h;
+    return BRKGA_OK;
+}
+
+static void settle_events(brkga_handle *h) {
+    if (!h->ev_pending) return;
+    if (hipEventSynchronize(h->ev1) == hipSuccess) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, h->ev0, h->ev1) == hipSuccess) {
+            h->last_ms = ms;
+            h->kernel_ms += ms;
+        }
+    }
+    h->ev_pending = false;
+}
+
+static int launch(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_chrom, double *d_out,
+                  int32_t *d_log, hipStream_t s) {
+    const int M = h->prob.n_machines;
+    if ((size_t)(n_chrom * M) > h->d_ctx_n) {
+        if (h->d_ctx) { (void)hipFree(h->d_ctx); h->d_ctx = nullptr; }
+        HIPCHK(h, hipMalloc(&h->d_ctx, (size_t)n_chrom * M * sizeof(double)));
+        h->d_ctx_n = (size_t)n_chrom * M;
+    }
+    settle_events(h);
+    LaunchArgs L;
+    L.keys = d_keys; L.key_f64 = key_dtype == BRKGA_KEYS_F64; L.n_chrom = n_chrom;
+    L.ctx_ms = h->d_ctx; L.log = d_log; L.dense = h->dense; L.tiles_per_pass = h->tiles_per_pass;
+    const long long n_ctx = n_chrom * M;
+    const int grid = (int)std::min<long long>(h->n_wg, std::max<long long>(n_ctx, 1));
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, s, h->dw.queue);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipEventRecord(h->ev0, s));
+    if (h->tiles_per_pass == 4)
+        hipLaunchKernelGGL(decode_kernel<4>, dim3(grid), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, L);
+    else
+        hipLaunchKernelGGL(decode_kernel<8>, dim3(grid), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, L);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipEventRecord(h->ev1, s));
+    h->ev_pending = true;
+    const int fb = 256;
+    hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)((n_chrom + fb - 1) / fb)), dim3(fb), 0, s,
+                       h->d_ctx, (long long)n_chrom, M, d_out, h->dw.counters);
+    HIPCHK(h, hipGetLastError());
+    return 0;
+}
+
+static int check_args(brkga_handle *h, const void *keys, int key_dtype, int64_t n_chrom, int64_t n_genes,
+                      const void *out) {
+    if (!h) return BRKGA_EINVAL;
+    if (!keys || !out) return fail(h, BRKGA_EINVAL, "keys and out must not be NULL");
+    if (key_dtype != BRKGA_KEYS_F32 && key_dtype != BRKGA_KEYS_F64) return fail(h, BRKGA_EINVAL, "key_dtype must be BRKGA_KEYS_F32 or BRKGA_KEYS_F64");
+    if (n_chrom < 1) return fail(h, BRKGA_EINVAL, "n_chrom must be >= 1");
+    if (n_genes != 2LL * h->prob.n_parts) return fail(h, BRKGA_EINVAL, "n_genes must equal 2*n_parts");
+    if (n_chrom > (1LL << 31) / std::max(1, h->prob.n_machines)) return fail(h, BRKGA_EINVAL, "n_chrom too large");
+    return 0;
+}
+
+static int read_overflow(brkga_handle *h, hipStream_t s, unsigned long long *ov) {
+    HIPCHK(h, hipMemcpyAsync(ov, h->dw.counters + ST_OVERFLOW, sizeof *ov, hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int brkga_evaluate_device(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_chrom,
+                                     int64_t n_genes, double *d_out, int32_t *d_log, void *stream,
+                                     int check_overflow) {
+    int rc = check_args(h, d_keys, key_dtype, n_chrom, n_genes, d_out);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->device));
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long ov0 = 0, ov1 = 0;
+    if (check_overflow && (rc = read_overflow(h, s, &ov0))) return rc;
+    if ((rc = launch(h, d_keys, key_dtype, n_chrom, d_out, d_log, s))) return rc;
+    if (check_overflow) {
+        if ((rc = read_overflow(h, s, &ov1))) return rc;
+        settle_events(h);
+        if (ov1 != ov0) return fail(h, BRKGA_EOVERFLOW, "a context needed more bins than max_bins");
+    }
+    return 0;
+}
+
+extern "C" int brkga_evaluate(brkga_handle *h, const void *keys, int key_dtype, int64_t n_chrom,
+                              int64_t n_genes, double *out, int32_t *log, void *stream) {
+    int rc = check_args(h, keys, key_dtype, n_chrom, n_genes, out);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->device));
+    hipStream_t s = (hipStream_t)stream;
+    const size_t kb = (size_t)n_chrom * n_genes * (key_dtype == BRKGA_KEYS_F64 ? 8 : 4);
+    if (kb > h->d_keys_bytes) {
+        if (h->d_keys) { (void)hipFree(h->d_keys); h->d_keys = nullptr; h->d_keys_bytes = 0; }
+        HIPCHK(h, hipMalloc(&h->d_keys, kb));
+        h->d_keys_bytes = kb;
+    }
+    if ((size_t)n_chrom > h->d_out_n) {
+        if (h->d_out) { (void)hipFree(h->d_out); h->d_out = nullptr; h->d_out_n = 0; }
+        HIPCHK(h, hipMalloc(&h->d_out, (size_t)n_chrom * sizeof(double)));
+        h->d_out_n = (size_t)n_chrom;
+    }
+    const size_t lb = log ? (size_t)n_chrom * h->prob.n_parts * 6 * sizeof(int32_t) : 0;
+    if (lb > h->d_log_bytes) {
+        if (h->d_log) { (void)hipFree(h->d_log); h->d_log = nullptr; h->d_log_bytes = 0; }
+        HIPCHK(h, hipMalloc(&h->d_log, lb));
+        h->d_log_bytes = lb;
+    }
+    HIPCHK(h, hipMemcpyAsync(h->d_keys, keys, kb, hipMemcpyHostToDevice, s));
+    for (int attempt = 0; attempt < 8; attempt++) {
+        rc = brkga_evaluate_device(h, h->d_keys, key_dtype, n_chrom, n_genes, h->d_out, log ? h->d_log : nullptr, s, 1);
+        if (rc != BRKGA_EOVERFLOW) break;
+        // grow the bin workspace and decode again (full_native_decoder.py:2102-2153 grows likewise)
+        const int nb = std::min(h->prob.n_parts, h->dw.max_bins * 2);
+        if (nb == h->dw.max_bins) break;
+        HIPCHK(h, hipStreamSynchronize(s));
+        if ((rc = device_workspace(h, nb))) return rc;
+    }
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpyAsync(out, h->d_out, (size_t)n_chrom * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (log) HIPCHK(h, hipMemcpyAsync(log, h->d_log, lb, hipMemcpyDeviceToHost, s));
+    HIPCHK(h, hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int brkga_get_stats(const brkga_handle *hc, brkga_stats *out) {
+    if (!hc || !out) return BRKGA_EINVAL;
+    brkga_handle *h = const_cast<brkga_handle *>(hc);
+    HIPCHK(h, hipSetDevice(h->device));
+    settle_events(h);
+    unsigned long long c[ST_COUNT];
+    HIPCHK(h, hipMemcpy(c, h->dw.counters, sizeof c, hipMemcpyDeviceToHost));
+    out->calls = (int64_t)c[ST_CALLS];
+    out->chromosomes = (int64_t)c[ST_CHROM];
+    out->contexts = (int64_t)c[ST_CTX];
+    out->placements = (int64_t)c[ST_PLACE];
+    out->bins_opened = (int64_t)c[ST_BINS];
+    out->tests = (int64_t)c[ST_TESTS];
+    out->row_groups = (int64_t)c[ST_GROUPS];
+    out->mfma_ops = (int64_t)c[ST_MFMA];
+    out->algo_ops = (int64_t)c[ST_ALGO];
+    out->overflows = (int64_t)c[ST_OVERFLOW];
+    out->kernel_ms = h->kernel_ms;
+    out->last_kernel_ms = h->last_ms;
+    return 0;
+}
+
+extern "C" int brkga_reset_stats(brkga_handle *h) {
+    if (!h) return BRKGA_EINVAL;
+    HIPCHK(h, hipSetDevice(h->device));
+    settle_events(h);
+    HIPCHK(h, hipMemset(h->dw.counters, 0, ST_COUNT * sizeof(unsigned long long)));
+    h->kernel_ms = 0.0;
+    h->last_ms = 0.0;
+    return 0;
+}
+
+extern "C" const char *brkga_last_error(const brkga_handle *h) {
+    return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" void brkga_destroy(brkga_handle *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    (void)hipDeviceSynchronize();
+    for (void *p : h->allocs) (void)hipFree(p);
+    if (h->d_keys) (void)hipFree(h->d_keys);
+    if (h->d_out) (void)hipFree(h->d_out);
+    if (h->d_log) (void)hipFree(h->d_log);
+    if (h->d_ctx) (void)hipFree(h->d_ctx);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    delete h;
+}

@@ -1,0 +1,184 @@
+"""ctypes binding of the C ABI in ``include/brkga_mi355x.h``.
+
+The library is loaded from ``brkga_fft2_amd/_lib/libbrkga_mi355x.so``.  There
+is no fallback: if the library is missing or the device is unavailable the
+calls raise, naming what is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .build import LIB_PATH
+
+BRKGA_KEYS_F32 = 0
+BRKGA_KEYS_F64 = 1
+INFEASIBLE = 1e16
+_ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "EOVERFLOW", -5: "ESTATE"}
+
+_P = ctypes.c_void_p
+
+
+class BrkgaProblem(ctypes.Structure):
+    _fields_ = [
+        ("n_parts", ctypes.c_int32), ("n_machines", ctypes.c_int32),
+        ("n_types", ctypes.c_int32), ("n_rots", ctypes.c_int32),
+        ("thresholds", _P), ("slot_type", _P), ("type_id", _P), ("area", _P),
+        ("nrot", _P), ("best_rot", _P), ("rot_base", _P), ("rot_h", _P), ("rot_w", _P),
+        ("mask_off", _P), ("masks", _P), ("plate_h", _P), ("plate_w", _P), ("setup", _P),
+        ("proc_time", _P), ("proc_time_height", _P),
+    ]
+
+
+class BrkgaOptions(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("wgs_per_cu", ctypes.c_int32),
+                ("max_bins", ctypes.c_int32), ("dense", ctypes.c_int32),
+                ("tiles_per_pass", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+
+
+class BrkgaStats(ctypes.Structure):
+    _fields_ = [("calls", ctypes.c_int64), ("chromosomes", ctypes.c_int64),
+                ("contexts", ctypes.c_int64), ("placements", ctypes.c_int64),
+                ("bins_opened", ctypes.c_int64), ("tests", ctypes.c_int64),
+                ("row_groups", ctypes.c_int64), ("mfma_ops", ctypes.c_int64),
+                ("algo_ops", ctypes.c_int64), ("overflows", ctypes.c_int64),
+                ("kernel_ms", ctypes.c_double), ("last_kernel_ms", ctypes.c_double)]
+
+
+EXPORTED = ["brkga_create", "brkga_evaluate", "brkga_evaluate_device", "brkga_get_stats",
+            "brkga_reset_stats", "brkga_last_error", "brkga_destroy"]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load the decoder library (cached).  Raises OSError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError("decoder library %s is not built; run __graft_entry__.build() "
+                      "or python -m brkga_fft2_amd.build" % path)
+    lib = ctypes.CDLL(path)
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    lib.brkga_create.argtypes = [ctypes.POINTER(BrkgaProblem), ctypes.POINTER(BrkgaOptions), PP]
+    lib.brkga_create.restype = ctypes.c_int
+    lib.brkga_evaluate.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _P, _P, _P]
+    lib.brkga_evaluate.restype = ctypes.c_int
+    lib.brkga_evaluate_device.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                          _P, _P, _P, ctypes.c_int]
+    lib.brkga_evaluate_device.restype = ctypes.c_int
+    lib.brkga_get_stats.argtypes = [_P, ctypes.POINTER(BrkgaStats)]
+    lib.brkga_get_stats.restype = ctypes.c_int
+    lib.brkga_reset_stats.argtypes = [_P]
+    lib.brkga_reset_stats.restype = ctypes.c_int
+    lib.brkga_last_error.argtypes = [_P]
+    lib.brkga_last_error.restype = ctypes.c_char_p
+    lib.brkga_destroy.argtypes = [_P]
+    lib.brkga_destroy.restype = None
+    _lib = lib
+    return lib
+
+
+_ARRAYS = [("thresholds", np.float64), ("slot_type", np.int32), ("type_id", np.int32),
+           ("area", np.float64), ("nrot", np.int32), ("best_rot", np.int32),
+           ("rot_base", np.int32), ("rot_h", np.int32), ("rot_w", np.int32),
+           ("mask_off", np.int64), ("masks", np.uint8), ("plate_h", np.int32),
+           ("plate_w", np.int32), ("setup", np.float64), ("proc_time", np.float64),
+           ("proc_time_height", np.float64)]
+
+
+class DecoderError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%s)" % (msg, _ERRORS.get(code, code)))
+        self.code = code
+
+
+class Decoder:
+    """One decoder handle: a problem uploaded to one GPU."""
+
+    def __init__(self, packed, device=0, dense=False, tiles_per_pass=0, wgs_per_cu=0,
+                 max_bins=0):
+        self._lib = load_library()
+        self.n_parts = int(packed["n_parts"])
+        self.n_machines = int(packed["n_machines"])
+        keep = []
+        prob = BrkgaProblem(self.n_parts, self.n_machines, int(packed["n_types"]),
+                            int(packed["n_rots"]))
+        for name, dt in _ARRAYS:
+            a = np.ascontiguousarray(packed[name], dtype=dt)
+            keep.append(a)
+            setattr(prob, name, a.ctypes.data if a.size else None)
+        opt = BrkgaOptions(int(device), int(wgs_per_cu), int(max_bins), int(bool(dense)),
+                           int(tiles_per_pass))
+        h = ctypes.c_void_p()
+        rc = self._lib.brkga_create(ctypes.byref(prob), ctypes.byref(opt), ctypes.byref(h))
+        if rc != 0:
+            raise DecoderError(rc, "brkga_create: " + self._lib.brkga_last_error(None).decode())
+        self._h = h
+        self.device = int(device)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise DecoderError(rc, "%s: %s" % (what, self._lib.brkga_last_error(self._h).decode()))
+
+    def evaluate(self, keys, with_log=False):
+        """Decode host keys ``[N, 2n]`` (float32 or float64).
+
+        Returns ``(makespans float64[N], log int32[N, n, 6] or None)``.
+        """
+        keys = np.asarray(keys)
+        if keys.dtype not in (np.float32, np.float64):
+            keys = keys.astype(np.float32)
+        keys = np.ascontiguousarray(keys)
+        if keys.ndim != 2:
+            raise ValueError("keys must be a 2-D array [N, 2*n_parts]")
+        n = keys.shape[0]
+        out = np.empty(n, np.float64)
+        log = np.empty((n, self.n_parts, 6), np.int32) if with_log else None
+        if n == 0:
+            return out, log
+        dt = BRKGA_KEYS_F64 if keys.dtype == np.float64 else BRKGA_KEYS_F32
+        rc = self._lib.brkga_evaluate(self._h, keys.ctypes.data, dt, n, keys.shape[1],
+                                      out.ctypes.data, log.ctypes.data if with_log else None,
+                                      None)
+        self._check(rc, "brkga_evaluate")
+        return out, log
+
+    def evaluate_device(self, keys, out, log=None, stream=None, check_overflow=True):
+        """Decode device-resident keys (torch tensors on this GPU) into ``out`` (float64)."""
+        import torch
+        if keys.device.type != "cuda" or out.device.type != "cuda":
+            raise ValueError("evaluate_device needs GPU tensors")
+        if not keys.is_contiguous() or out.dtype != torch.float64 or not out.is_contiguous():
+            raise ValueError("keys must be contiguous; out must be contiguous float64")
+        if keys.dtype not in (torch.float32, torch.float64):
+            raise ValueError("keys must be float32 or float64")
+        dt = BRKGA_KEYS_F64 if keys.dtype == torch.float64 else BRKGA_KEYS_F32
+        if stream is None:
+            stream = torch.cuda.current_stream(keys.device)
+        rc = self._lib.brkga_evaluate_device(
+            self._h, keys.data_ptr(), dt, keys.shape[0], keys.shape[1], out.data_ptr(),
+            log.data_ptr() if log is not None else None, ctypes.c_void_p(stream.cuda_stream),
+            int(bool(check_overflow)))
+        self._check(rc, "brkga_evaluate_device")
+        return out
+
+    def stats(self):
+        st = BrkgaStats()
+        self._check(self._lib.brkga_get_stats(self._h, ctypes.byref(st)), "brkga_get_stats")
+        return {k: getattr(st, k) for k, _ in BrkgaStats._fields_}
+
+    def reset_stats(self):
+        self._check(self._lib.brkga_reset_stats(self._h), "brkga_reset_stats")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.brkga_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

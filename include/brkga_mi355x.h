@@ -1,0 +1,142 @@
+/*
+ * brkga_mi355x.h -- C ABI of the MI355X BRKGA population decoder.
+ *
+ * The library decodes a population of random-key chromosomes into makespans
+ * (and optionally per-part placements) on one gfx950 GPU.  It is the drop-in
+ * replacement for the reference's native decoder extension:
+ *
+ *   reference                                       here
+ *   ---------                                       ----
+ *   NativeFullDecoder(packed, nb_parts, nb_machines, device)
+ *     full_native_decoder.py:408-533 (ctor), packed schema :2884-2925
+ *                                                   brkga_create()
+ *   NativeFullDecoder::evaluate_batch(float[N,2n]) -> double[N]
+ *     full_native_decoder.py:563-641                brkga_evaluate() /
+ *                                                   brkga_evaluate_device()
+ *   get_bin_stats / reset_bin_stats, get_profile_summary / reset_profile
+ *     full_native_decoder.py:2290-2327              brkga_get_stats() /
+ *                                                   brkga_reset_stats()
+ *   RuntimeError("...") on bad input               brkga_last_error()
+ *     full_native_decoder.py:440-533,567-573
+ *   (destructor)                                    brkga_destroy()
+ *
+ * Conventions: every call returns 0 on success or a negative BRKGA_E* code;
+ * no C++ exception crosses the ABI; host buffers belong to the caller; device
+ * workspaces belong to the handle.  A handle is not re-entrant: calls on one
+ * handle must not overlap (one handle per thread).
+ */
+#ifndef BRKGA_MI355X_H
+#define BRKGA_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BRKGA_OK 0
+#define BRKGA_EINVAL (-1)      /* bad argument or problem data */
+#define BRKGA_EDEVICE (-2)     /* HIP runtime error */
+#define BRKGA_ENOMEM (-3)      /* device allocation failed */
+#define BRKGA_EOVERFLOW (-4)   /* a context needed more bins than the workspace holds */
+#define BRKGA_ESTATE (-5)      /* call not valid in the handle's state */
+
+#define BRKGA_KEYS_F32 0
+#define BRKGA_KEYS_F64 1
+
+/* Infeasible chromosomes decode to this makespan (placement.py:9). */
+#define BRKGA_INFEASIBLE 1e16
+
+/*
+ * Problem data, structure of arrays.  Same content as the reference's packed
+ * problem (full_native_decoder.py:2788-2926 _pack_problem_data) minus the
+ * part spectra: part types are numbered 0..n_types-1, rotation r of type t is
+ * row rot_base[t]+r of the rotation table, its mask is
+ * masks[mask_off[row] .. + rot_h*rot_w] row-major {0,1} bytes.
+ * Plates: plate_h rows (the reference's bin_length, L) by plate_w columns
+ * (bin_width, W).  proc_time / proc_time_height are [n_machines][n_types].
+ */
+typedef struct brkga_problem {
+    int32_t n_parts;      /* instance slots n; chromosomes have 2n keys */
+    int32_t n_machines;   /* M >= 1; M == 1 puts every part on machine 0 */
+    int32_t n_types;
+    int32_t n_rots;
+    const double *thresholds;        /* [M-1], ascending; t/M in the reference */
+    const int32_t *slot_type;        /* [n_parts] type of instance slot j */
+    const int32_t *type_id;          /* [n_types] external part id (for logs) */
+    const double *area;              /* [n_types] part area, > 0 */
+    const int32_t *nrot;             /* [n_types] 1..4 */
+    const int32_t *best_rot;         /* [n_types] rotation used to open a bin */
+    const int32_t *rot_base;         /* [n_types] */
+    const int32_t *rot_h, *rot_w;    /* [n_rots] */
+    const int64_t *mask_off;         /* [n_rots] */
+    const uint8_t *masks;
+    const int32_t *plate_h, *plate_w;              /* [M] */
+    const double *setup;                           /* [M] setup time per bin */
+    const double *proc_time, *proc_time_height;    /* [M * n_types] */
+} brkga_problem;
+
+typedef struct brkga_options {
+    int32_t device;          /* HIP device ordinal */
+    int32_t wgs_per_cu;      /* persistent workgroups per compute unit (0 = default 2) */
+    int32_t max_bins;        /* bins per context in the workspace (0 = default) */
+    int32_t dense;           /* 1: correlation over every placement row, no early exit
+                                (same results; used to measure the kernel at full shape) */
+    int32_t tiles_per_pass;  /* correlation x-tiles per pass, 4 or 8 (0 = autotuned) */
+    int32_t reserved[3];
+} brkga_options;
+
+typedef struct brkga_stats {
+    int64_t calls;           /* evaluate calls */
+    int64_t chromosomes;     /* chromosomes decoded */
+    int64_t contexts;        /* (chromosome, machine) contexts decoded */
+    int64_t placements;      /* parts placed */
+    int64_t bins_opened;
+    int64_t tests;           /* collision tests (bin, rotation) run */
+    int64_t row_groups;      /* 32-row correlation groups evaluated */
+    int64_t mfma_ops;        /* int8 multiply-adds issued x 2 (MFMA work done) */
+    int64_t algo_ops;        /* sum over tests of 2*h*w*(H-h+1)*(W-w+1) */
+    int64_t overflows;       /* contexts that exceeded max_bins */
+    double kernel_ms;        /* device time of the decode kernels (HIP events) */
+    double last_kernel_ms;   /* decode kernel time of the last call */
+} brkga_stats;
+
+typedef struct brkga_handle brkga_handle;
+
+/* Build a decoder for one problem on one device.  Validates the problem
+ * (shapes, binary masks, positive areas, plate sizes, rotation/plate fit) and
+ * uploads the masks and the per-rotation correlation operands. */
+int brkga_create(const brkga_problem *problem, const brkga_options *options,
+                 brkga_handle **out);
+
+/* Decode n_chrom chromosomes held in HOST memory.  keys: [n_chrom][n_genes]
+ * with n_genes == 2*n_parts, float32 or float64 (key_dtype).  out_makespan:
+ * [n_chrom] host doubles.  opt_log (nullable): [n_chrom][n_parts][6] int32
+ * rows (machine, part id, bin, col, row, rotation) in machine order then
+ * placement order; rows of infeasible contexts are -1.  stream: hipStream_t or
+ * NULL for the null stream.  Synchronous: returns after the results are in
+ * host memory. */
+int brkga_evaluate(brkga_handle *h, const void *keys, int key_dtype, int64_t n_chrom,
+                   int64_t n_genes, double *out_makespan, int32_t *opt_log, void *stream);
+
+/* Same with DEVICE pointers (e.g. from torch tensors).  Enqueued on `stream`;
+ * returns after enqueueing unless check_overflow is nonzero, in which case it
+ * synchronises the stream and returns BRKGA_EOVERFLOW if a context ran out of
+ * bins (then no output may be used). */
+int brkga_evaluate_device(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_chrom,
+                          int64_t n_genes, double *d_out_makespan, int32_t *d_opt_log,
+                          void *stream, int check_overflow);
+
+int brkga_get_stats(const brkga_handle *h, brkga_stats *out);
+int brkga_reset_stats(brkga_handle *h);
+
+/* Message of the last failed call on h (or of the last failed brkga_create
+ * when h is NULL).  Valid until the next call on the same handle. */
+const char *brkga_last_error(const brkga_handle *h);
+
+void brkga_destroy(brkga_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BRKGA_MI355X_H */

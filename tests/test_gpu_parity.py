@@ -1,0 +1,147 @@
+"""GPU decoder vs the CPU oracle: bit-exact makespans and placement logs."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from tests.helpers import (GOLDEN, ref_instance, syn_instance, syn_keys, load_fingerprints,
+                           fingerprint_population, fingerprint_of, feasible_logs_equal)
+
+pytestmark = pytest.mark.gpu
+
+
+def _decoder(packed, **kw):
+    from brkga_fft2_amd.native import Decoder
+    return Decoder(packed, device=0, **kw)
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("cfg,n_chrom", [("C1", 64), ("C2", 48), ("C3", 16)])
+def test_synthetic_parity(cfg, n_chrom):
+    _say("synthetic parity %s N=%d" % (cfg, n_chrom))
+    prob, n, M, packed = syn_instance(cfg)
+    keys = syn_keys(cfg, n_chrom, n)
+    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=os.cpu_count() or 4, with_log=True)
+    dec = _decoder(packed)
+    ms, log = dec.evaluate(keys, with_log=True)
+    np.testing.assert_array_equal(ms, ref_ms)
+    assert feasible_logs_equal(ref_ms, log, ref_log) is None
+    st = dec.stats()
+    assert st["chromosomes"] == n_chrom and st["overflows"] == 0
+    dec.close()
+
+
+@pytest.mark.parametrize("tiles", [4, 8])
+def test_dense_mode_and_tile_shapes_agree(tiles):
+    """Full-shape correlation (no row pruning, no early exit) gives identical results."""
+    _say("dense tiles=%d" % tiles)
+    prob, n, M, packed = syn_instance("C2")
+    keys = syn_keys("C2", 16, n, seed=11)
+    ref_ms, ref_log, _ = oracle.decode(packed, keys, with_log=True)
+    dec = _decoder(packed, dense=True, tiles_per_pass=tiles)
+    ms, log = dec.evaluate(keys, with_log=True)
+    np.testing.assert_array_equal(ms, ref_ms)
+    assert feasible_logs_equal(ref_ms, log, ref_log) is None
+    st = dec.stats()
+    assert st["mfma_ops"] > 0 and st["algo_ops"] > 0
+    dec.close()
+
+
+def test_reference_golden_p75m2_0():
+    _say("golden P75M2-0")
+    prob, n, M, packed = ref_instance("P75M2-0")
+    with open(os.path.join(GOLDEN, "golden_P75M2-0.json")) as f:
+        g = json.load(f)
+    dec = _decoder(packed)
+    ms, log = dec.evaluate(np.array(g["chromosome"], dtype=np.float32)[None], with_log=True)
+    assert round(float(ms[0]), 6) == g["makespan"]
+    for m in range(M):
+        want = [(p["part_id"], p["bin_idx"], p["col"], p["row"], p["rot"])
+                for p in g["machines"][str(m)]["placements"]]
+        got = [tuple(int(v) for v in e[1:]) for e in log[0] if e[0] == m]
+        assert got == want
+    dec.close()
+
+
+@pytest.mark.parametrize("name", ["P25M2-2", "P50M2-0", "P75M2-4", "P100M4-0"])
+def test_reference_fingerprints(name):
+    _say("fingerprint %s" % name)
+    fp = load_fingerprints()[name]
+    prob, n, M, packed = ref_instance(name)
+    pop = fingerprint_population(fp, n)
+    dec = _decoder(packed)
+    ms, _ = dec.evaluate(pop)
+    got, n_feas = fingerprint_of(ms)
+    assert n_feas == fp["n_feasible"]
+    assert got == fp["fingerprint"]
+    dec.close()
+
+
+def test_full_population_vs_oracle_p50m2_0():
+    """verify_full_population_vs_pp.py:88-194: seed-42 population, element-wise."""
+    _say("full population P50M2-0")
+    prob, n, M, packed = ref_instance("P50M2-0")
+    np.random.seed(42)
+    pop = np.random.uniform(0.0, 1.0, size=(500, 100)).astype(np.float32)
+    ref_ms, ref_log, _ = oracle.decode(packed, pop, n_threads=os.cpu_count() or 4, with_log=True)
+    dec = _decoder(packed)
+    ms, log = dec.evaluate(pop, with_log=True)
+    np.testing.assert_array_equal(ms, ref_ms)
+    assert feasible_logs_equal(ref_ms, log, ref_log) is None
+    dec.close()
+
+
+def test_determinism_and_batch_invariance():
+    """Same batch twice; permuted batch; single rows (CROSS_ROW_LEAKAGE_FINDINGS.md:12-47)."""
+    _say("determinism")
+    prob, n, M, packed = syn_instance("C2", seed=3)
+    keys = syn_keys("C2", 40, n, seed=5)
+    dec = _decoder(packed)
+    a, la = dec.evaluate(keys, with_log=True)
+    b, lb = dec.evaluate(keys, with_log=True)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(la, lb)
+    perm = np.random.default_rng(0).permutation(len(keys))
+    c, _ = dec.evaluate(keys[perm])
+    np.testing.assert_array_equal(c, a[perm])
+    for i in (0, 17, 39):
+        d, _ = dec.evaluate(keys[i:i + 1])
+        assert d[0] == a[i]
+    dec.close()
+
+
+def test_bin_capacity_growth():
+    """A workspace of one bin per context overflows and is grown transparently."""
+    _say("capacity growth")
+    prob, n, M, packed = ref_instance("P25M2-4")
+    fp = load_fingerprints()["P25M2-4"]
+    pop = fingerprint_population(fp, n)[:64]
+    ref_ms, _, _ = oracle.decode(packed, pop)
+    dec = _decoder(packed, max_bins=1)
+    ms, _ = dec.evaluate(pop)
+    np.testing.assert_array_equal(ms, ref_ms)
+    dec.close()
+
+
+def test_float64_keys_and_device_entry_point():
+    _say("f64 + device entry")
+    import torch
+    prob, n, M, packed = syn_instance("C1", seed=2)
+    keys = syn_keys("C1", 32, n, seed=9).astype(np.float64)
+    ref_ms, _, _ = oracle.decode(packed, keys)
+    dec = _decoder(packed)
+    ms, _ = dec.evaluate(keys)
+    np.testing.assert_array_equal(ms, ref_ms)
+    dk = torch.from_numpy(keys).cuda()
+    out = torch.empty(len(keys), dtype=torch.float64, device="cuda")
+    dec.evaluate_device(dk, out)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), ref_ms)
+    dec.close()
