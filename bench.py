@@ -1,0 +1,185 @@
+"""Benchmark: decoded chromosomes/s of the MI355X population decoder.
+
+Workload (default): 50 parts, 2 identical printer machines with 300x300-cell
+plates, seeded polygon masks of 20-100 cells per side, 4 rotations, population
+256 (one BRKGA generation's decode = one step).  Keys are float32 and resident
+in HBM before the timed region; every step decodes a different population.
+
+Prints one JSON line with the throughput, the roofline of the dominant
+kernel (the decode kernel run at full shape: every placement row, no early
+exit -- the work the reference's FFT path does), and the CPU oracle timed on
+this host's cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA peak, 5.0 POPS
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3", "C5"])
+    ap.add_argument("--pop", type=int, default=0, help="population per step (default: config's)")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--roofline-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--single", action="store_true",
+                    help="also time the plain single-chromosome path (population 1 per launch)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(the decoder shards by chromosome; see the design notes)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from brkga_fft2_amd.native import Decoder
+    from brkga_fft2_amd.problem import (SYNTHETIC_CONFIGS, pack_problem, synthetic_population,
+                                        synthetic_problem, thresholds_for)
+
+    cfg = SYNTHETIC_CONFIGS[args.config]
+    pop = args.pop or cfg["pop"]
+    key_dt = np.float64 if cfg["key_dtype"] == "float64" else np.float32
+    prob, n, M = synthetic_problem(args.config, seed=args.seed)
+    packed = pack_problem(prob, n, M, thresholds_for(M), prob.instance_parts)
+    total = args.warmup + args.steps
+    print("bench: config %s n=%d M=%d pop=%d steps=%d warmup=%d" %
+          (args.config, n, M, pop, args.steps, args.warmup), file=sys.stderr, flush=True)
+    host_pops = [synthetic_population(pop, n, seed=1000 + s, dtype=key_dt) for s in range(total)]
+    dev = torch.device("cuda:0")
+    keys = [torch.from_numpy(p).to(dev) for p in host_pops]
+    outs = [torch.empty(pop, dtype=torch.float64, device=dev) for _ in range(total)]
+    stream = torch.cuda.current_stream(dev)
+    dec = Decoder(packed, device=0)
+    torch.cuda.synchronize()
+
+    for s in range(args.warmup):
+        dec.evaluate_device(keys[s], outs[s], stream=stream, check_overflow=False)
+    torch.cuda.synchronize()
+    dec.reset_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.warmup, total):
+        dec.evaluate_device(keys[s], outs[s], stream=stream, check_overflow=False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = dec.stats()
+    if st["overflows"]:
+        raise RuntimeError("bin workspace overflowed during the timed steps")
+    elapsed = t1 - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = pop * args.steps / elapsed
+    print("bench: %.3f ms/step, %.1f chrom/s, decode kernel %.3f ms/launch, tests/chrom %.1f" %
+          (ms_per_step, value, st["kernel_ms"] / args.steps, st["tests"] / (pop * args.steps)),
+          file=sys.stderr, flush=True)
+    # spot-check one step against a fresh decode of the same keys (full check is in tests)
+    chk = torch.empty(pop, dtype=torch.float64, device=dev)
+    dec.evaluate_device(keys[-1], chk, stream=stream)
+    torch.cuda.synchronize()
+    if not torch.equal(chk, outs[-1]):
+        raise RuntimeError("decode is not deterministic")
+    pruned = dict(tests=st["tests"], algo_ops=st["algo_ops"], mfma_ops=st["mfma_ops"],
+                  kernel_ms=st["kernel_ms"])
+    dec.close()
+
+    # ---- roofline of the decode kernel at full shape (no pruning, no early exit) ----
+    dd = Decoder(packed, device=0, dense=True)
+    dd.evaluate_device(keys[0], outs[0], stream=stream, check_overflow=False)
+    torch.cuda.synchronize()
+    dd.reset_stats()
+    rs = max(1, args.roofline_steps)
+    for s in range(rs):
+        dd.evaluate_device(keys[args.warmup + s % args.steps], outs[0], stream=stream,
+                           check_overflow=False)
+    torch.cuda.synchronize()
+    ds = dd.stats()
+    dd.close()
+    dense_ms = ds["kernel_ms"] / rs
+    achieved = ds["algo_ops"] / rs / (dense_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None,
+                "kernel": "decode_kernel<8,true>", "kernel_ms": round(dense_ms, 4),
+                "algo_ops_per_launch": ds["algo_ops"] / rs,
+                "mfma_ops_per_launch": ds["mfma_ops"] / rs}
+    print("bench: dense decode %.3f ms/launch, %.1f TOPS algorithmic (%.1f%% of int8 peak), "
+          "issued MFMA %.1f TOPS" % (dense_ms, achieved, 100 * achieved / INT8_PEAK_TOPS,
+                                      ds["mfma_ops"] / rs / (dense_ms * 1e-3) / 1e12),
+          file=sys.stderr, flush=True)
+
+    single = None
+    if args.single:
+        ds1 = Decoder(packed, device=0)
+        one = [k[:1].contiguous() for k in keys[:8]]
+        o1 = torch.empty(1, dtype=torch.float64, device=dev)
+        ds1.evaluate_device(one[0], o1, stream=stream, check_overflow=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in one:
+            ds1.evaluate_device(k, o1, stream=stream, check_overflow=False)
+        torch.cuda.synchronize()
+        single = len(one) / (time.perf_counter() - t0)
+        ds1.close()
+
+    # ---- CPU baseline: the C oracle on this host's cores, bounded sample ----
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        cores = min(16, len(os.sched_getaffinity(0)))
+        sample = host_pops[args.warmup]
+        t0 = time.perf_counter()
+        done = 0
+        chunk = max(cores, 8)
+        while time.perf_counter() - t0 < args.cpu_seconds and done < len(sample):
+            oracle.decode(packed, sample[done:done + chunk], n_threads=cores)
+            done += min(chunk, len(sample) - done)
+        cpu_t = time.perf_counter() - t0
+        want, _, _ = oracle.decode(packed, sample[:8], n_threads=cores)
+        if not np.array_equal(want, outs[args.warmup][:8].cpu().numpy()):
+            raise RuntimeError("GPU makespans differ from the CPU oracle")
+        cpu = {"value": round(done / cpu_t, 2), "unit": "chromosomes/s", "cores": cores,
+               "kind": "port",
+               "sample": "%d of the %d chromosomes of the first timed step, same instance, "
+                         "C oracle with %d OpenMP threads, %.1f s" % (done, pop, cores, cpu_t)}
+
+    line = {
+        "metric": "decoded chromosomes/s (exact placements)",
+        "value": round(value, 2),
+        "unit": "chromosomes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8",
+        "data": "synthetic",
+        "config": {"workload": "%s: %d parts, %d machines, plates %s cells, masks %d-%d cells, "
+                               "%d rotations, population %d per step, %s keys" %
+                               (args.config, n, M, "/".join("%dx%d" % (p, p) for p in cfg["plates"]),
+                                cfg["bbox"][0], cfg["bbox"][1], cfg["nrot"], pop, cfg["key_dtype"]),
+                   "population": pop, "parts": n, "machines": M},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "decode_kernel_ms": round(pruned["kernel_ms"] / args.steps, 4),
+        "tests_per_chromosome": round(pruned["tests"] / (pop * args.steps), 2),
+    }
+    if single is not None:
+        line["single_chromosome_per_s"] = round(single, 2)
+        line["batched_over_single"] = round(value / single, 2)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -94,8 +94,6 @@ struct LaunchArgs {
     long long n_chrom;
     double *ctx_ms;          // [n_chrom * M]
     int *log;                // nullable [n_chrom][n_parts][6]
-    int dense;
-    int tiles_per_pass;
 };
 
 __device__ __forceinline__ double load_key(const void *keys, int f64, long long idx) {
@@ -123,9 +121,9 @@ __device__ __forceinline__ int wave_min(int v) {
 // rows (descending).  Executed by one full wave.  Returns the key m*XKEY+x of
 // the bottom-most, then left-most, zero-overlap placement in the list, as
 // (list index, x), or KEY_NONE.
-template <int TP>
+template <int TP, bool DENSE>
 __device__ int correlate_rows(const unsigned char *grid, int pitch, int W, int h, int w, int nc,
-                              const v4i *bfr, const short *cand, int ncand, int dense,
+                              const v4i *bfr, const short *cand, int ncand,
                               unsigned long long &groups, unsigned long long &mfma) {
     const int lane = threadIdx.x & 63;
     const int T = (W - w + 1 + 31) >> 5;
@@ -178,7 +176,7 @@ __device__ int correlate_rows(const unsigned char *grid, int pitch, int W, int h
         gkey = wave_min(gkey);
         if (gkey != KEY_NONE) {
             if (found == KEY_NONE) found = (g0 + gkey / XKEY) * XKEY + (gkey % XKEY);
-            if (!dense) break;
+            if (!DENSE) break;
         }
     }
     return found;
@@ -188,7 +186,7 @@ struct __align__(8) PlaceRes { int has, y, x, pad; };
 
 __global__ void reset_counters_kernel(unsigned int *queue) { *queue = 0u; }
 
-template <int TP>
+template <int TP, bool DENSE>
 __global__ __launch_bounds__(WG_THREADS) void decode_kernel(DevProblem P, DevWork Wk, LaunchArgs L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = P.n_parts;
@@ -289,7 +287,7 @@ __global__ __launch_bounds__(WG_THREADS) void decode_kernel(DevProblem P, DevWor
                             if (ok) cand[nc_ + pos] = (short)y;
                             nc_ += __popcll(bal);
                         }
-                        if (L.dense && nc_ > 0) {   // full valid region, bottom-up
+                        if (DENSE && nc_ > 0) {   // full valid region, bottom-up
                             nc_ = H - h + 1;
                             for (int k = lane; k < nc_; k += 64) cand[k] = (short)(H - h - k);
                         }
@@ -299,8 +297,8 @@ __global__ __launch_bounds__(WG_THREADS) void decode_kernel(DevProblem P, DevWor
                         c_tests++;
                         c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
                         const v4i *bfr = P.bfrag + P.bfrag_off[rg];
-                        key = correlate_rows<TP>(grid, Wk.pitch, W, h, w, P.rot_nc[rg], bfr, cand,
-                                                 nc_, L.dense, c_groups, c_mfma);
+                        key = correlate_rows<TP, DENSE>(grid, Wk.pitch, W, h, w, P.rot_nc[rg], bfr,
+                                                        cand, nc_, c_groups, c_mfma);
                     }
                     if (lane == 0) {
                         s_res[wave].has = key != KEY_NONE;
@@ -733,16 +731,15 @@ static int launch(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_
     settle_events(h);
     LaunchArgs L;
     L.keys = d_keys; L.key_f64 = key_dtype == BRKGA_KEYS_F64; L.n_chrom = n_chrom;
-    L.ctx_ms = h->d_ctx; L.log = d_log; L.dense = h->dense; L.tiles_per_pass = h->tiles_per_pass;
+    L.ctx_ms = h->d_ctx; L.log = d_log;
     const long long n_ctx = n_chrom * M;
     const int grid = (int)std::min<long long>(h->n_wg, std::max<long long>(n_ctx, 1));
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, s, h->dw.queue);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipEventRecord(h->ev0, s));
-    if (h->tiles_per_pass == 4)
-        hipLaunchKernelGGL(decode_kernel<4>, dim3(grid), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, L);
-    else
-        hipLaunchKernelGGL(decode_kernel<8>, dim3(grid), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, L);
+    auto kern = h->tiles_per_pass == 4 ? (h->dense ? decode_kernel<4, true> : decode_kernel<4, false>)
+                                       : (h->dense ? decode_kernel<8, true> : decode_kernel<8, false>);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, L);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipEventRecord(h->ev1, s));
     h->ev_pending = true;

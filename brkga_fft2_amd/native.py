@@ -59,6 +59,13 @@ def load_library(path=LIB_PATH):
     if not os.path.exists(path):
         raise OSError("decoder library %s is not built; run __graft_entry__.build() "
                       "or python -m brkga_fft2_amd.build" % path)
+    # torch ships its own HIP runtime.  Loading torch first makes the library
+    # bind to that same runtime (one runtime per process), so device pointers
+    # and streams of torch tensors are valid inside the library.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     PP = ctypes.POINTER(ctypes.c_void_p)
     lib.brkga_create.argtypes = [ctypes.POINTER(BrkgaProblem), ctypes.POINTER(BrkgaOptions), PP]
