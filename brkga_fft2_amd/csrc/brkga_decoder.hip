@@ -54,6 +54,7 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 #define WG_THREADS 256
 #define MAX_ROT 4
 #define GROUP_ROWS 32
+#define LDS_RESERVE 1024    // static LDS of the kernel + slack
 #define XKEY 8192           // key = m * XKEY + x ; plates must have W < XKEY
 #define KEY_NONE 0x7fffffff
 
@@ -86,6 +87,8 @@ struct DevWork {
     unsigned int *queue;
     unsigned long long *counters;
     int pitch, hmax, max_bins;
+    int hrot;                // max rotation height (LDS density rows)
+    int band_off, band_bytes;  // LDS band for staged grid rows
 };
 
 struct LaunchArgs {
@@ -117,69 +120,241 @@ __device__ __forceinline__ int wave_min(int v) {
     return v;
 }
 
-// Correlation of one rotation against one bin over a list of candidate top
-// rows (descending).  Executed by one full wave.  Returns the key m*XKEY+x of
-// the bottom-most, then left-most, zero-overlap placement in the list, as
-// (list index, x), or KEY_NONE.
-template <int TP, bool DENSE>
-__device__ int correlate_rows(const unsigned char *grid, int pitch, int W, int h, int w, int nc,
-                              const v4i *bfr, const short *cand, int ncand,
-                              unsigned long long &groups, unsigned long long &mfma) {
+// One pass of the correlation for one wave: TP x-tiles of 32 placements, 32
+// candidate rows, operands read from the LDS band.  NC = number of 32-wide
+// Toeplitz chunks of the rotation; with NC known at compile time each A
+// fragment (32 rows x 32 grid bytes) loaded from LDS feeds up to NC MFMAs
+// (fragment j serves tile t with chunk c = j - t).  Returns the wave-wide
+// minimum key m*XKEY + x over zero counts (m < nrows, x <= xmax), or KEY_NONE.
+template <int TP, int NC>
+__device__ __forceinline__ int corr_pass(const unsigned char *band, int stride, int rowoff, int h,
+                                         const v4i *bfr, int tloc, int tcnt, int xbase, int xmax,
+                                         int nrows) {
     const int lane = threadIdx.x & 63;
-    const int T = (W - w + 1 + 31) >> 5;
-    int found = KEY_NONE;
-    for (int g0 = 0; g0 < ncand; g0 += GROUP_ROWS) {
-        const int nrows = min(GROUP_ROWS, ncand - g0);
-        const int msel = min(lane & 31, nrows - 1);
-        const int ysel = cand[g0 + msel];
-        const unsigned char *base = grid + (long long)ysel * pitch + ((lane >> 5) << 4);
-        int gkey = KEY_NONE;
-        groups++;
-        for (int t0 = 0; t0 < T; t0 += TP) {
-            v16i acc[TP];
+    v16i acc[TP];
 #pragma unroll
-            for (int t = 0; t < TP; t++) acc[t] = (v16i){};
-            const int tl = min(TP, T - t0);
-            for (int a = 0; a < h; a++) {
-                const unsigned char *rowp = base + (long long)a * pitch + (t0 << 5);
-                const v4i *bb = bfr + (long long)a * nc * 64 + lane;
-                for (int c = 0; c < nc; c++) {
-                    const v4i bf = bb[c * 64];
+    for (int t = 0; t < TP; t++) acc[t] = (v16i){};
+    const unsigned char *base = band + rowoff * stride + (tloc << 5) + ((lane >> 5) << 4);
+    const v4i *bl = bfr + lane;
+    for (int a = 0; a < h; a++) {
+        v4i B[NC];
 #pragma unroll
-                    for (int t = 0; t < TP; t++) {
-                        if (t < tl) {
-                            const v4i av = *(const v4i *)(rowp + ((t + c) << 5));
-                            acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bf, acc[t], 0, 0, 0);
-                        }
-                    }
-                }
+        for (int c = 0; c < NC; c++) B[c] = bl[(a * NC + c) * 64];
+        const unsigned char *rp = base + a * stride;
+        v4i A[TP + NC - 1];
+#pragma unroll
+        for (int j = 0; j < TP + NC - 1; j++)
+            A[j] = (j < tcnt + NC - 1) ? *(const v4i *)(rp + (j << 5)) : (v4i){0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < TP; t++) {
+            if (t < tcnt) {
+#pragma unroll
+                for (int c = 0; c < NC; c++)
+                    acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[t + c], B[c], acc[t], 0, 0, 0);
             }
-            mfma += (unsigned long long)tl * h * nc;
-            // epilogue: lane holds column n = lane&31, rows m = 8*(i/4) + 4*(lane/32) + i%4
-            const int n = lane & 31;
-            const int mhi = (lane >> 5) << 2;
-#pragma unroll
-            for (int t = 0; t < TP; t++) {
-                if (t < tl) {
-                    const int x = ((t0 + t) << 5) + n;
-#pragma unroll
-                    for (int i = 0; i < 16; i++) {
-                        const int m = ((i >> 2) << 3) + mhi + (i & 3);
-                        if (acc[t][i] == 0 && m < nrows && x <= W - w) {
-                            const int key = m * XKEY + x;
-                            gkey = key < gkey ? key : gkey;
-                        }
-                    }
-                }
-            }
-        }
-        gkey = wave_min(gkey);
-        if (gkey != KEY_NONE) {
-            if (found == KEY_NONE) found = (g0 + gkey / XKEY) * XKEY + (gkey % XKEY);
-            if (!DENSE) break;
         }
     }
+    // C layout of the 32x32 MFMA: lane holds column n = lane&31 and rows
+    // m = 8*(i/4) + 4*(lane/32) + i%4 for its 16 accumulators i.
+    int key = KEY_NONE;
+    const int n = lane & 31;
+    const int mhi = (lane >> 5) << 2;
+#pragma unroll
+    for (int t = 0; t < TP; t++) {
+        if (t < tcnt) {
+            const int x = xbase + (t << 5) + n;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int m = ((i >> 2) << 3) + mhi + (i & 3);
+                if (acc[t][i] == 0 && m < nrows && x <= xmax) {
+                    const int k = m * XKEY + x;
+                    key = k < key ? k : key;
+                }
+            }
+        }
+    }
+    return wave_min(key);
+}
+
+// Same for chunk counts above the compile-time set: runtime loop over the
+// chunks, one LDS fragment per MFMA.
+template <int TP>
+__device__ int corr_pass_generic(const unsigned char *band, int stride, int rowoff, int h, int nc,
+                                 const v4i *bfr, int tloc, int tcnt, int xbase, int xmax, int nrows) {
+    const int lane = threadIdx.x & 63;
+    v16i acc[TP];
+#pragma unroll
+    for (int t = 0; t < TP; t++) acc[t] = (v16i){};
+    const unsigned char *base = band + rowoff * stride + (tloc << 5) + ((lane >> 5) << 4);
+    const v4i *bl = bfr + lane;
+    for (int a = 0; a < h; a++) {
+        const unsigned char *rp = base + a * stride;
+        for (int c = 0; c < nc; c++) {
+            const v4i bf = bl[(a * nc + c) * 64];
+#pragma unroll
+            for (int t = 0; t < TP; t++)
+                if (t < tcnt)
+                    acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(*(const v4i *)(rp + ((t + c) << 5)), bf,
+                                                                   acc[t], 0, 0, 0);
+        }
+    }
+    int key = KEY_NONE;
+    const int n = lane & 31;
+    const int mhi = (lane >> 5) << 2;
+#pragma unroll
+    for (int t = 0; t < TP; t++) {
+        if (t < tcnt) {
+            const int x = xbase + (t << 5) + n;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int m = ((i >> 2) << 3) + mhi + (i & 3);
+                if (acc[t][i] == 0 && m < nrows && x <= xmax) {
+                    const int k = m * XKEY + x;
+                    key = k < key ? k : key;
+                }
+            }
+        }
+    }
+    return wave_min(key);
+}
+
+#define MAX_NC 12
+
+template <int TP>
+__device__ int corr_dispatch(int nc, const unsigned char *band, int stride, int rowoff, int h,
+                             const v4i *bfr, int tloc, int tcnt, int xbase, int xmax, int nrows) {
+#define CASE(K) case K: return corr_pass<TP, K>(band, stride, rowoff, h, bfr, tloc, tcnt, xbase, xmax, nrows);
+    switch (nc) {
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
+        default:
+            return corr_pass_generic<TP>(band, stride, rowoff, h, nc, bfr, tloc, tcnt, xbase, xmax, nrows);
+    }
+#undef CASE
+}
+
+// LDS row stride of a window of tw x-tiles: the staged width rounded to 128
+// bytes plus 16, so the 32 rows of an A fragment fall into distinct banks.
+__host__ __device__ __forceinline__ int band_stride(int tw, int nc) {
+    return ((((tw + nc - 1) << 5) + 127) & ~127) + 16;
+}
+
+// Largest x-window (in tiles, <= cap) whose 32-row band fits; shrinks the
+// window rather than the row group.
+__host__ __device__ __forceinline__ int band_window(int T, int nc, int h, int band_bytes, int cap,
+                                                    int *stride_out, int *rcap_out) {
+    int tw = T < cap ? T : cap;
+    int st, rc;
+    for (;;) {
+        st = band_stride(tw, nc);
+        rc = band_bytes / st;
+        if (rc >= h + GROUP_ROWS - 1 || tw == 1) break;
+        tw = (tw + 1) >> 1;
+    }
+    *stride_out = st;
+    *rcap_out = rc;
+    return tw;
+}
+
+// Collision test of one rotation against one bin, run by the whole workgroup.
+// cand: candidate top rows, descending.  Groups of up to 32 candidates are
+// staged into the LDS band (rows ylo..ytop+h-1, one x-window at a time) and
+// correlated; the 4 waves split the window's x-tiles.  Returns
+// (candidate index)*XKEY + x of the bottom-most, then left-most, placement
+// with zero overlap, or KEY_NONE.  Uniform across the workgroup.
+template <int TP, bool DENSE>
+__device__ int test_rotation(const unsigned char *grid, int gpitch, int W, int h, int w, int nc,
+                             const v4i *bfr, const short *cand, int ncand, unsigned char *band,
+                             int band_bytes, int *s_key, unsigned long long &groups,
+                             unsigned long long &mfma) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int T = (W - w + 1 + 31) >> 5;
+    int stride, rcap;
+    const int TW = band_window(T, nc, h, band_bytes, 4 * TP, &stride, &rcap);
+    const int maxrows = min(GROUP_ROWS, rcap - h + 1);
+    int found = KEY_NONE;
+    for (int g0 = 0; g0 < ncand;) {
+        const int ytop = cand[g0];
+        int g1 = g0 + 1;
+        while (g1 < ncand && g1 - g0 < maxrows && ytop + h - cand[g1] <= rcap) g1++;
+        const int ylo = cand[g1 - 1];
+        const int rows = ytop + h - ylo;
+        const int nrows = g1 - g0;
+        const int rowoff = cand[g0 + min(lane & 31, nrows - 1)] - ylo;
+        int gkey = KEY_NONE;
+        groups++;
+        for (int tw0 = 0; tw0 < T; tw0 += TW) {
+            const int tl = min(TW, T - tw0);
+            const int q16 = (tl + nc - 1) * 2;
+            const unsigned char *src = grid + (long long)ylo * gpitch + (tw0 << 5);
+            for (int k = tid; k < rows * q16; k += WG_THREADS) {
+                const int r = k / q16, q = k - r * q16;
+                *(v4i *)(band + r * stride + (q << 4)) =
+                    *(const v4i *)(src + (long long)r * gpitch + (q << 4));
+            }
+            __syncthreads();
+            const int per = (tl + 3) >> 2;
+            const int tb = wave * per;
+            const int tc = max(0, min(per, tl - tb));
+            for (int tp = 0; tp < tc; tp += TP) {
+                const int cnt = min(TP, tc - tp);
+                const int k = corr_dispatch<TP>(nc, band, stride, rowoff, h, bfr, tb + tp, cnt,
+                                                (tw0 + tb + tp) << 5, W - w, nrows);
+                gkey = k < gkey ? k : gkey;
+                mfma += (unsigned long long)cnt * h * nc;
+            }
+            __syncthreads();
+        }
+        if (lane == 0) s_key[wave] = gkey;
+        __syncthreads();
+        const int kmin = min(min(s_key[0], s_key[1]), min(s_key[2], s_key[3]));
+        if (kmin != KEY_NONE) {
+            if (found == KEY_NONE) found = (g0 + kmin / XKEY) * XKEY + kmin % XKEY;
+            if (!DENSE) break;
+        }
+        g0 = g1;
+    }
     return found;
+}
+
+// Longest free run of one grid row, computed by one wave: each lane scans a
+// segment, then an ordered tree combine of (prefix run, suffix run, max run,
+// all-free, length) summaries (numba_utils.py:161-188 computes the same
+// sequentially).
+__device__ __forceinline__ int row_free_run(const unsigned char *rp, int W) {
+    const int lane = threadIdx.x & 63;
+    const int seg = (W + 63) >> 6;
+    const int c0 = min(W, lane * seg), c1 = min(W, c0 + seg);
+    int p = 0, mx = 0, run = 0, len = c1 - c0;
+    bool full = true, seen = false;
+    for (int c = c0; c < c1; c++) {
+        if (rp[c]) {
+            if (!seen) { p = run; seen = true; }
+            run = 0;
+            full = false;
+        } else {
+            run++;
+            mx = run > mx ? run : mx;
+        }
+    }
+    if (!seen) p = run;
+    int s = run;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int op = __shfl_down(p, off, 64), os = __shfl_down(s, off, 64);
+        const int om = __shfl_down(mx, off, 64), ol = __shfl_down(len, off, 64);
+        const int of = __shfl_down((int)full, off, 64);
+        if ((lane & (2 * off - 1)) == 0) {
+            const int nm = max(max(mx, om), s + op);
+            const int np = full ? len + op : p;
+            const int ns = of ? ol + s : os;
+            full = full && of;
+            len += ol;
+            p = np;
+            s = ns;
+            mx = nm;
+        }
+    }
+    return __shfl(mx, 0, 64);
 }
 
 struct __align__(8) PlaceRes { int has, y, x, pad; };
@@ -187,20 +362,22 @@ struct __align__(8) PlaceRes { int has, y, x, pad; };
 __global__ void reset_counters_kernel(unsigned int *queue) { *queue = 0u; }
 
 template <int TP, bool DENSE>
-__global__ __launch_bounds__(WG_THREADS) void decode_kernel(DevProblem P, DevWork Wk, LaunchArgs L) {
+__global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, DevWork Wk, LaunchArgs L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = P.n_parts;
     const int M = P.n_machines;
     const int hmax = Wk.hmax;
-    // LDS layout
-    double *s_sv = (double *)smem;                                  // [n]
-    int *s_mach = (int *)(s_sv + n);                                // [n]
-    int *s_seq = s_mach + n;                                        // [n]
-    short *s_cand = (short *)(s_seq + n);                           // [MAX_ROT][hmax]
-    __shared__ PlaceRes s_res[MAX_ROT];
-    __shared__ int s_ctx, s_cnt, s_off, s_infeasible, s_decision, s_place_b, s_place_y,
-        s_place_x, s_place_rot, s_nb;
+    // LDS layout (sizes computed on the host, brkga_create)
+    double *s_sv = (double *)smem;                          // [n]
+    int *s_mach = (int *)(s_sv + n);                        // [n]
+    int *s_seq = s_mach + n;                                // [n]
+    int *s_vac = s_seq + n;                                 // [hmax]
+    int *s_den = s_vac + hmax;                              // [MAX_ROT][hrot]
+    short *s_cand = (short *)(s_den + MAX_ROT * Wk.hrot);   // [MAX_ROT][hmax]
+    unsigned char *band = smem + Wk.band_off;               // [band_bytes]
+    __shared__ int s_ctx, s_cnt, s_off, s_infeasible, s_nb;
     __shared__ int s_ncand[MAX_ROT];
+    __shared__ int s_key[4];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
@@ -265,23 +442,32 @@ __global__ __launch_bounds__(WG_THREADS) void decode_kernel(DevProblem P, DevWor
             const int nr = P.nrot[ty];
             const int rb = P.rot_base[ty];
             const int nb = s_nb;
+            // densities of this part's rotations -> LDS
+            for (int r = 0; r < nr; r++) {
+                const int h = P.rot_h[rb + r];
+                const int *den = P.dens + P.dens_off[rb + r];
+                for (int a = tid; a < h; a += WG_THREADS) s_den[r * Wk.hrot + a] = den[a];
+            }
             bool placed = false;
+            int pb = 0, py = 0, px = 0, prot = 0;
             for (int b = 0; b < nb && !placed; b++) {
                 if (b_area[b] + parea > plate_area) continue;   // placement.py:57-59
                 const unsigned char *grid = grids + (long long)b * hmax * Wk.pitch;
                 const int *vac = vacs + (long long)b * hmax;
-                // ---- wave `wave` handles rotation `wave` ----
+                for (int y = tid; y < H; y += WG_THREADS) s_vac[y] = vac[y];
+                __syncthreads();
+                // ---- vacancy filter, per candidate top row (wave r: rotation r) ----
                 if (wave < nr) {
                     const int rg = rb + wave;
                     const int h = P.rot_h[rg], w = P.rot_w[rg];
                     short *cand = s_cand + wave * hmax;
+                    const int *den = s_den + wave * Wk.hrot;
                     int nc_ = 0;
                     if (h <= H && w <= W) {
-                        const int *den = P.dens + P.dens_off[rg];
                         for (int yb = H - h; yb >= 0; yb -= 64) {
                             const int y = yb - lane;
                             bool ok = y >= 0;
-                            for (int a = 0; ok && a < h; a++) ok = vac[y + a] >= den[a];
+                            for (int a = 0; ok && a < h; a++) ok = s_vac[y + a] >= den[a];
                             const unsigned long long bal = __ballot(ok);
                             const int pos = __popcll(bal & ((1ull << lane) - 1ull));
                             if (ok) cand[nc_ + pos] = (short)y;
@@ -292,48 +478,38 @@ __global__ __launch_bounds__(WG_THREADS) void decode_kernel(DevProblem P, DevWor
                             for (int k = lane; k < nc_; k += 64) cand[k] = (short)(H - h - k);
                         }
                     }
-                    int key = KEY_NONE;
-                    if (nc_ > 0) {
-                        c_tests++;
-                        c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
-                        const v4i *bfr = P.bfrag + P.bfrag_off[rg];
-                        key = correlate_rows<TP, DENSE>(grid, Wk.pitch, W, h, w, P.rot_nc[rg], bfr,
-                                                        cand, nc_, c_groups, c_mfma);
-                    }
-                    if (lane == 0) {
-                        s_res[wave].has = key != KEY_NONE;
-                        s_res[wave].y = key != KEY_NONE ? cand[key / XKEY] : 0;
-                        s_res[wave].x = key != KEY_NONE ? key % XKEY : 0;
-                    }
+                    if (lane == 0) s_ncand[wave] = nc_;
                 }
                 __syncthreads();
-                // ---- best rotation: density (new enclosure length) -> row -> col ----
-                if (tid == 0) {
-                    const int enc = H - b_minrow[b];
-                    int best = -1, bl = 0, brow = 0, bx = 0;
-                    for (int r = 0; r < nr; r++) {
-                        if (!s_res[r].has) continue;
-                        const int h = P.rot_h[rb + r];
-                        const int y = s_res[r].y, x = s_res[r].x, row = y + h - 1;
-                        const int nl = enc > H - y ? enc : H - y;
-                        if (best < 0 || nl < bl || (nl == bl && row > brow) ||
-                            (nl == bl && row == brow && x < bx)) {
-                            best = r; bl = nl; brow = row; bx = x;
-                        }
-                    }
-                    s_decision = best;
-                    if (best >= 0) {
-                        s_place_b = b; s_place_rot = best;
-                        s_place_y = brow - P.rot_h[rb + best] + 1; s_place_x = bx;
+                // ---- collision tests, one rotation at a time (whole workgroup) ----
+                int best = -1, bl = 0, brow = 0, bx = 0, by = 0;
+                const int enc = H - b_minrow[b];
+                for (int r = 0; r < nr; r++) {
+                    const int ncand = s_ncand[r];
+                    if (ncand == 0) continue;
+                    const int rg = rb + r;
+                    const int h = P.rot_h[rg], w = P.rot_w[rg];
+                    c_tests++;
+                    c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
+                    const short *cand = s_cand + r * hmax;
+                    const int key = test_rotation<TP, DENSE>(grid, Wk.pitch, W, h, w, P.rot_nc[rg],
+                                                             P.bfrag + P.bfrag_off[rg], cand, ncand,
+                                                             band, Wk.band_bytes, s_key, c_groups, c_mfma);
+                    if (key == KEY_NONE) continue;
+                    // density (new enclosure length) -> row -> col (binClassNew.py:101-117)
+                    const int y = cand[key / XKEY], x = key % XKEY, row = y + h - 1;
+                    const int nl = enc > H - y ? enc : H - y;
+                    if (best < 0 || nl < bl || (nl == bl && row > brow) || (nl == bl && row == brow && x < bx)) {
+                        best = r; bl = nl; brow = row; bx = x; by = y;
                     }
                 }
+                if (best >= 0) {
+                    placed = true;
+                    pb = b; prot = best; py = by; px = bx;
+                }
                 __syncthreads();
-                placed = s_decision >= 0;
             }
-            int pb, py, px, prot;
-            if (placed) {
-                pb = s_place_b; py = s_place_y; px = s_place_x; prot = s_place_rot;
-            } else {
+            if (!placed) {
                 if (nb >= Wk.max_bins) { overflow = true; break; }
                 pb = nb; prot = P.best_rot[ty];
                 py = H - P.rot_h[rb + prot]; px = 0;
@@ -363,22 +539,9 @@ __global__ __launch_bounds__(WG_THREADS) void decode_kernel(DevProblem P, DevWor
                 __syncthreads();
                 // ---- free-run update of the stamped rows (numba_utils.py:161-188) ----
                 int *vv = vacs + (long long)pb * hmax;
-                for (int a = tid; a < h; a += WG_THREADS) {
-                    const unsigned char *row = grid + (long long)(py + a) * Wk.pitch;
-                    int run = 0, best = 0;
-                    for (int x0 = 0; x0 < W; x0 += 16) {
-                        const v4i q = *(const v4i *)(row + x0);
-                        const int lim = min(16, W - x0);
-#pragma unroll
-                        for (int j = 0; j < 16; j++) {
-                            if (j < lim) {
-                                const int byte = (q[j >> 2] >> ((j & 3) << 3)) & 0xff;
-                                run = byte ? 0 : run + 1;
-                                best = run > best ? run : best;
-                            }
-                        }
-                    }
-                    vv[py + a] = best;
+                for (int a = wave; a < h; a += 4) {
+                    const int v = row_free_run(grid + (long long)(py + a) * Wk.pitch, W);
+                    if (lane == 0) vv[py + a] = v;
                 }
                 if (tid == 0) {   // binClassNew.py:147-153,172-174 ; placement.py:81-82
                     const long long mp = (long long)mach * P.n_types + ty;
@@ -411,13 +574,11 @@ __global__ __launch_bounds__(WG_THREADS) void decode_kernel(DevProblem P, DevWor
         __syncthreads();
     }
     // ---- statistics ----
-    if (lane == 0) {
+    if (lane == 0) atomicAdd(&Wk.counters[ST_MFMA], c_mfma * 32ull * 32ull * 32ull * 2ull);
+    if (tid == 0) {
         atomicAdd(&Wk.counters[ST_TESTS], c_tests);
         atomicAdd(&Wk.counters[ST_GROUPS], c_groups);
-        atomicAdd(&Wk.counters[ST_MFMA], c_mfma * 32ull * 32ull * 32ull * 2ull);
         atomicAdd(&Wk.counters[ST_ALGO], c_algo);
-    }
-    if (tid == 0) {
         atomicAdd(&Wk.counters[ST_CTX], c_ctx);
         atomicAdd(&Wk.counters[ST_PLACE], c_place);
         atomicAdd(&Wk.counters[ST_BINS], c_bins);
@@ -453,7 +614,7 @@ struct brkga_handle {
     int n_cu = 0;
     int n_wg = 0;
     int dense = 0;
-    int tiles_per_pass = 8;
+    int tiles_per_pass = 4;
     brkga_problem prob{};     // scalars only
     DevProblem dp{};
     DevWork dw{};
@@ -596,7 +757,7 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     if (o) opt = *o;
     h->device = opt.device;
     h->dense = opt.dense ? 1 : 0;
-    h->tiles_per_pass = opt.tiles_per_pass == 4 ? 4 : 8;
+    h->tiles_per_pass = opt.tiles_per_pass == 2 ? 2 : 4;
     h->prob = *p;
     auto bail = [&](int code) {
         std::string e = h->err;
@@ -607,8 +768,6 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, h->device) != hipSuccess) { h->err = "hipGetDeviceProperties failed"; return bail(BRKGA_EDEVICE); }
     h->n_cu = prop.multiProcessorCount;
-    const int wpc = opt.wgs_per_cu > 0 ? opt.wgs_per_cu : 2;
-    h->n_wg = std::max(1, h->n_cu * wpc);
 
     const int M = p->n_machines, T = p->n_types, R = p->n_rots;
     // ---- per-rotation tables: densities (BRKGA_alg3.py:351-360) and Toeplitz operands ----
@@ -700,9 +859,43 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     if (hipMemset(w.counters, 0, ST_COUNT * sizeof(unsigned long long)) != hipSuccess) { h->err = "hipMemset failed"; return bail(BRKGA_EDEVICE); }
     int mb = opt.max_bins > 0 ? opt.max_bins : std::min(p->n_parts, 32);
     h->max_bins_req = mb;
+    // ---- LDS layout: keys/sequence, vacancy, densities, candidates, then the band ----
+    int hrot = 1;
+    for (int r = 0; r < R; r++) hrot = std::max(hrot, p->rot_h[r]);
+    w.hrot = hrot;
+    size_t fixed = (size_t)p->n_parts * (8 + 4 + 4) + (size_t)hmax * 4 + (size_t)MAX_ROT * hrot * 4 +
+                   (size_t)MAX_ROT * hmax * 2;
+    fixed = (fixed + 15) & ~(size_t)15;
+    size_t need_full = 0, need_min = 0;
+    for (int m = 0; m < M; m++)
+        for (int r = 0; r < R; r++) {
+            const int H = p->plate_h[m], W = p->plate_w[m], rh = p->rot_h[r], rw = p->rot_w[r];
+            if (rh > H || rw > W) continue;
+            const int Tt = (W - rw + 1 + 31) / 32;
+            need_full = std::max(need_full, (size_t)(rh + GROUP_ROWS - 1) *
+                                                band_stride(std::min(Tt, 4 * h->tiles_per_pass), nc[r]));
+            need_min = std::max(need_min, (size_t)rh * band_stride(1, nc[r]));
+        }
+    size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (size_t)prop.maxSharedMemoryPerMultiProcessor : 65536;
+    int wpc2 = opt.wgs_per_cu > 0 ? opt.wgs_per_cu : (fixed + need_full + LDS_RESERVE <= lds_cu / 2 ? 2 : 1);
+    size_t per_wg = lds_cu / wpc2;
+    if (per_wg < fixed + need_min + LDS_RESERVE) {
+        char b2[200];
+        snprintf(b2, sizeof b2, "a part rotation needs %zu bytes of LDS band, %zu available per workgroup",
+                 need_min, per_wg > fixed + LDS_RESERVE ? per_wg - fixed - LDS_RESERVE : (size_t)0);
+        h->err = b2;
+        return bail(BRKGA_EINVAL);
+    }
+    w.band_off = (int)fixed;
+    w.band_bytes = (int)((per_wg - fixed - LDS_RESERVE) & ~(size_t)15);
+    h->smem = fixed + w.band_bytes;
+    h->n_wg = std::max(1, h->n_cu * wpc2);   // persistent grid at the occupancy the LDS allows
     if ((rc = device_workspace(h, mb))) return bail(rc);
-    h->smem = (size_t)p->n_parts * (8 + 4 + 4) + (size_t)MAX_ROT * hmax * 2 + 16;
-    if (h->smem > 60 * 1024) { h->err = "problem too large for the LDS layout (n_parts/plate rows)"; return bail(BRKGA_EINVAL); }
+    for (auto kern : {decode_kernel<2, false>, decode_kernel<2, true>, decode_kernel<4, false>, decode_kernel<4, true>})
+        if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->smem) != hipSuccess) {
+            h->err = "hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed";
+            return bail(BRKGA_EDEVICE);
+        }
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) { h->err = "hipEventCreate failed"; return bail(BRKGA_EDEVICE); }
     *out = h;
     return BRKGA_OK;
@@ -737,8 +930,8 @@ static int launch(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, s, h->dw.queue);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipEventRecord(h->ev0, s));
-    auto kern = h->tiles_per_pass == 4 ? (h->dense ? decode_kernel<4, true> : decode_kernel<4, false>)
-                                       : (h->dense ? decode_kernel<8, true> : decode_kernel<8, false>);
+    auto kern = h->tiles_per_pass == 2 ? (h->dense ? decode_kernel<2, true> : decode_kernel<2, false>)
+                                       : (h->dense ? decode_kernel<4, true> : decode_kernel<4, false>);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, L);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipEventRecord(h->ev1, s));
