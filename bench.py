@@ -91,6 +91,13 @@ def main():
         raise RuntimeError("decode is not deterministic")
     pruned = dict(tests=st["tests"], algo_ops=st["algo_ops"], mfma_ops=st["mfma_ops"],
                   kernel_ms=st["kernel_ms"])
+    tot = sum(st["phase_us"].values()) or 1.0
+    print("bench: workgroup time by phase: " +
+          ", ".join("%s %.1f%%" % (k, 100 * v / tot) for k, v in st["phase_us"].items()) +
+          "; longest context %.0f us, mean context %.0f us, row groups/test %.2f, MFMA %.1f TOPS issued" %
+          (st["context_us_max"] / args.steps, st["context_us_sum"] / max(1, st["contexts"]),
+           st["row_groups"] / max(1, st["tests"]), st["mfma_ops"] / (st["kernel_ms"] * 1e-3) / 1e12),
+          file=sys.stderr, flush=True)
     dec.close()
 
     # ---- roofline of the decode kernel at full shape (no pruning, no early exit) ----
@@ -105,11 +112,15 @@ def main():
     torch.cuda.synchronize()
     ds = dd.stats()
     dd.close()
+    tot = sum(ds["phase_us"].values()) or 1.0
+    print("bench: dense workgroup time by phase: " +
+          ", ".join("%s %.1f%%" % (k, 100 * v / tot) for k, v in ds["phase_us"].items()),
+          file=sys.stderr, flush=True)
     dense_ms = ds["kernel_ms"] / rs
     achieved = ds["algo_ops"] / rs / (dense_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None,
-                "kernel": "decode_kernel<8,true>", "kernel_ms": round(dense_ms, 4),
+                "kernel": "decode_kernel<4,true>", "kernel_ms": round(dense_ms, 4),
                 "algo_ops_per_launch": ds["algo_ops"] / rs,
                 "mfma_ops_per_launch": ds["mfma_ops"] / rs}
     print("bench: dense decode %.3f ms/launch, %.1f TOPS algorithmic (%.1f%% of int8 peak), "
@@ -136,21 +147,20 @@ def main():
     if not args.no_cpu:
         from oracle import oracle
         cores = min(16, len(os.sched_getaffinity(0)))
-        sample = host_pops[args.warmup]
+        sample = np.concatenate(host_pops[args.warmup:])
         t0 = time.perf_counter()
         done = 0
-        chunk = max(cores, 8)
         while time.perf_counter() - t0 < args.cpu_seconds and done < len(sample):
-            oracle.decode(packed, sample[done:done + chunk], n_threads=cores)
-            done += min(chunk, len(sample) - done)
+            oracle.decode(packed, sample[done:done + pop], n_threads=cores)
+            done += min(pop, len(sample) - done)
         cpu_t = time.perf_counter() - t0
         want, _, _ = oracle.decode(packed, sample[:8], n_threads=cores)
         if not np.array_equal(want, outs[args.warmup][:8].cpu().numpy()):
             raise RuntimeError("GPU makespans differ from the CPU oracle")
         cpu = {"value": round(done / cpu_t, 2), "unit": "chromosomes/s", "cores": cores,
                "kind": "port",
-               "sample": "%d of the %d chromosomes of the first timed step, same instance, "
-                         "C oracle with %d OpenMP threads, %.1f s" % (done, pop, cores, cpu_t)}
+               "sample": "%d chromosomes (the first %d timed steps' populations), same instance, "
+                         "C oracle with %d OpenMP threads, %.1f s" % (done, done // pop, cores, cpu_t)}
 
     line = {
         "metric": "decoded chromosomes/s (exact placements)",

@@ -50,17 +50,28 @@
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+// Pointers into LDS carry the LDS address space explicitly, so loads through
+// them are ds_read instructions (with in-order completion the compiler can
+// wait for precisely the ones it needs) rather than flat loads.
+#define LDSP __attribute__((address_space(3)))
+typedef LDSP unsigned char lds_u8;
+typedef LDSP const unsigned char lds_cu8;
+typedef LDSP const unsigned lds_cu32;
+typedef LDSP unsigned lds_u32;
+typedef LDSP const v4i lds_cv4i;
+typedef LDSP v4i lds_v4i;
+typedef LDSP unsigned long long lds_u64;
 
 #define WG_THREADS 256
 #define MAX_ROT 4
 #define GROUP_ROWS 32
-#define LDS_RESERVE 1024    // static LDS of the kernel + slack
+#define LDS_RESERVE 2048    // static LDS of the kernel + slack
 #define XKEY 8192           // key = m * XKEY + x ; plates must have W < XKEY
 #define KEY_NONE 0x7fffffff
 
 enum {
     ST_CALLS = 0, ST_CHROM, ST_CTX, ST_PLACE, ST_BINS, ST_TESTS, ST_GROUPS, ST_MFMA, ST_ALGO,
-    ST_OVERFLOW, ST_COUNT
+    ST_OVERFLOW, ST_PH0, ST_CTXMAX = ST_PH0 + 8, ST_CTXSUM, ST_COUNT
 };
 
 struct DevProblem {
@@ -70,10 +81,10 @@ struct DevProblem {
     const double *area;
     const int *nrot, *best_rot, *rot_base;
     const int *rot_h, *rot_w, *rot_nc;
-    const long long *mask_off, *dens_off, *bfrag_off;
+    const long long *mask_off, *dens_off, *qbits_off;
     const unsigned char *masks;
     const int *dens;
-    const v4i *bfrag;
+    const unsigned *qbits;   // per rotation: h rows of (nc+1) dwords, mask bits after 32 zero bits
     const int *plate_h, *plate_w;
     const double *setup, *proc_time, *pth;
     const unsigned char *fits;   // [M][n_types] rotation 0 fits the plate in some orientation
@@ -88,6 +99,8 @@ struct DevWork {
     unsigned long long *counters;
     int pitch, hmax, max_bins;
     int hrot;                // max rotation height (LDS density rows)
+    int q_off;               // LDS offset of the staged mask bits
+    int vc_bins;             // bins whose vacancy vector is cached in LDS
     int band_off, band_bytes;  // LDS band for staged grid rows
 };
 
@@ -98,6 +111,14 @@ struct LaunchArgs {
     double *ctx_ms;          // [n_chrom * M]
     int *log;                // nullable [n_chrom][n_parts][6]
 };
+
+// Constant-rate (100 MHz on MI300-class parts) wall clock, read by thread 0
+// only, to attribute a workgroup's time to the decode phases.
+__device__ __forceinline__ unsigned long long tick() { return wall_clock64(); }
+
+// Values read from LDS or derived from them are the same in every lane;
+// saying so keeps them (and the loops they bound) in scalar registers.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ __forceinline__ double load_key(const void *keys, int f64, long long idx) {
     return f64 ? ((const double *)keys)[idx] : (double)((const float *)keys)[idx];
@@ -120,114 +141,209 @@ __device__ __forceinline__ int wave_min(int v) {
     return v;
 }
 
-// One pass of the correlation for one wave: TP x-tiles of 32 placements, 32
-// candidate rows, operands read from the LDS band.  NC = number of 32-wide
-// Toeplitz chunks of the rotation; with NC known at compile time each A
-// fragment (32 rows x 32 grid bytes) loaded from LDS feeds up to NC MFMAs
-// (fragment j serves tile t with chunk c = j - t).  Returns the wave-wide
-// minimum key m*XKEY + x over zero counts (m < nrows, x <= xmax), or KEY_NONE.
+// B fragment of the 32x32x32 MFMA for Toeplitz chunk c of one mask row,
+// built in registers from the row's bits.  Q is the mask row padded with 32
+// zero bits in front (Q[i] = P[a][i-32]); lane (n = lane&31, g = lane>>5)
+// needs the 16 bytes B[16g + j][n] = P[a][32c + 16g + j - n] = Q[32c + shift + j]
+// with shift = 32 + 16g - n in [1, 48], i.e. 16 bits of the 64-bit word
+// (Q dword c+1 : Q dword c) shifted right by `shift`.  Each bit becomes one
+// byte (0 or 1): a nibble times 0x00204081 spreads its bits to the bytes'
+// low bits.
+__device__ __forceinline__ v4i bfrag_from_bits(unsigned lo, unsigned hi, int shift) {
+    const unsigned long long w = ((unsigned long long)hi << 32) | lo;
+    const unsigned b = (unsigned)(w >> shift) & 0xffffu;
+    v4i r;
+    r[0] = (int)(((b & 0xfu) * 0x00204081u) & 0x01010101u);
+    r[1] = (int)((((b >> 4) & 0xfu) * 0x00204081u) & 0x01010101u);
+    r[2] = (int)((((b >> 8) & 0xfu) * 0x00204081u) & 0x01010101u);
+    r[3] = (int)((((b >> 12) & 0xfu) * 0x00204081u) & 0x01010101u);
+    return r;
+}
+
+// Correlation pass of one wave over mask rows [a0, a1) for TP x-tiles of 32
+// placements and the group's 32 candidate rows.  The four waves of the
+// workgroup take disjoint mask-row ranges of the same tiles: every partial
+// overlap count is a sum of non-negative products, so the total is zero iff
+// every wave's partial is zero, and the waves combine their results with an
+// AND of zero masks in LDS (s_zero[t*16 + i], bit = lane) instead of adding
+// counts.  A operands come from the LDS band, B operands are built from the
+// mask bits; with NC (Toeplitz chunks) known at compile time each A fragment
+// feeds up to NC MFMAs and each B fragment feeds all TP tiles, and the next
+// mask row's A fragments are loaded while the current row's MFMAs run.
 template <int TP, int NC>
-__device__ __forceinline__ int corr_pass(const unsigned char *band, int stride, int rowoff, int h,
-                                         const v4i *bfr, int tloc, int tcnt, int xbase, int xmax,
-                                         int nrows) {
+__device__ __forceinline__ void corr_pass(lds_cu8 *band, int stride, int rowoff, int a0,
+                                          int a1, lds_cu32 *qbits, int tloc, int tcnt,
+                                          lds_u64 *s_zero) {
+    constexpr int NA = TP + NC - 1;
     const int lane = threadIdx.x & 63;
+    const int shift = 32 + ((lane >> 5) << 4) - (lane & 31);
     v16i acc[TP];
 #pragma unroll
     for (int t = 0; t < TP; t++) acc[t] = (v16i){};
-    const unsigned char *base = band + rowoff * stride + (tloc << 5) + ((lane >> 5) << 4);
-    const v4i *bl = bfr + lane;
-    for (int a = 0; a < h; a++) {
-        v4i B[NC];
+    lds_cu8 *base = band + rowoff * stride + (tloc << 5) + ((lane >> 5) << 4);
+    if (a0 < a1) {
+        v4i A[NA];
+        unsigned qv[NC + 1];
+        lds_cu8 *r0 = base + a0 * stride;
 #pragma unroll
-        for (int c = 0; c < NC; c++) B[c] = bl[(a * NC + c) * 64];
-        const unsigned char *rp = base + a * stride;
-        v4i A[TP + NC - 1];
+        for (int j = 0; j < NA; j++) A[j] = *(lds_cv4i *)(r0 + (j << 5));
 #pragma unroll
-        for (int j = 0; j < TP + NC - 1; j++)
-            A[j] = (j < tcnt + NC - 1) ? *(const v4i *)(rp + (j << 5)) : (v4i){0, 0, 0, 0};
+        for (int c = 0; c <= NC; c++) qv[c] = qbits[a0 * (NC + 1) + c];
+        for (int a = a0; a < a1; a++) {
+            v4i B[NC];
 #pragma unroll
-        for (int t = 0; t < TP; t++) {
-            if (t < tcnt) {
+            for (int c = 0; c < NC; c++) B[c] = bfrag_from_bits(qv[c], qv[c + 1], shift);
+            const int an = a + 1 < a1 ? a + 1 : a;
+            v4i An[NA];
+            lds_cu8 *rn = base + an * stride;
 #pragma unroll
-                for (int c = 0; c < NC; c++)
+            for (int j = 0; j < NA; j++) An[j] = *(lds_cv4i *)(rn + (j << 5));
+#pragma unroll
+            for (int c = 0; c <= NC; c++) qv[c] = qbits[an * (NC + 1) + c];
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+#pragma unroll
+                for (int t = 0; t < TP; t++)
                     acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[t + c], B[c], acc[t], 0, 0, 0);
             }
+#pragma unroll
+            for (int j = 0; j < NA; j++) A[j] = An[j];
         }
     }
-    // C layout of the 32x32 MFMA: lane holds column n = lane&31 and rows
-    // m = 8*(i/4) + 4*(lane/32) + i%4 for its 16 accumulators i.
-    int key = KEY_NONE;
-    const int n = lane & 31;
-    const int mhi = (lane >> 5) << 2;
 #pragma unroll
     for (int t = 0; t < TP; t++) {
         if (t < tcnt) {
-            const int x = xbase + (t << 5) + n;
 #pragma unroll
             for (int i = 0; i < 16; i++) {
-                const int m = ((i >> 2) << 3) + mhi + (i & 3);
-                if (acc[t][i] == 0 && m < nrows && x <= xmax) {
-                    const int k = m * XKEY + x;
-                    key = k < key ? k : key;
-                }
+                const unsigned long long z = __ballot(acc[t][i] == 0);
+                if (lane == 0 && z != ~0ull) __hip_atomic_fetch_and(&s_zero[t * 16 + i], z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
     }
-    return wave_min(key);
 }
 
-// Same for chunk counts above the compile-time set: runtime loop over the
-// chunks, one LDS fragment per MFMA.
-template <int TP>
-__device__ int corr_pass_generic(const unsigned char *band, int stride, int rowoff, int h, int nc,
-                                 const v4i *bfr, int tloc, int tcnt, int xbase, int xmax, int nrows) {
+// Bottom-most then left-most zero in the combined masks of one pass, as
+// m*XKEY + x (m < nrows, x <= xmax), or KEY_NONE.  C layout of the 32x32
+// MFMA: lane holds column n = lane&31 and rows m = 8*(i/4) + 4*(lane/32) + i%4
+// of accumulator i, so row m lives in ballot word i = 4*(m/8) + m%4, half
+// (m/4)%2.  Lanes 0..31 each scan one row m; every wave computes the result.
+__device__ __forceinline__ int decode_zero(lds_u64 *s_zero, int tcnt, int xbase,
+                                           int xmax, int nrows) {
     const int lane = threadIdx.x & 63;
-    v16i acc[TP];
-#pragma unroll
-    for (int t = 0; t < TP; t++) acc[t] = (v16i){};
-    const unsigned char *base = band + rowoff * stride + (tloc << 5) + ((lane >> 5) << 4);
-    const v4i *bl = bfr + lane;
-    for (int a = 0; a < h; a++) {
-        const unsigned char *rp = base + a * stride;
-        for (int c = 0; c < nc; c++) {
-            const v4i bf = bl[(a * nc + c) * 64];
-#pragma unroll
-            for (int t = 0; t < TP; t++)
-                if (t < tcnt)
-                    acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(*(const v4i *)(rp + ((t + c) << 5)), bf,
-                                                                   acc[t], 0, 0, 0);
-        }
-    }
     int key = KEY_NONE;
-    const int n = lane & 31;
-    const int mhi = (lane >> 5) << 2;
-#pragma unroll
-    for (int t = 0; t < TP; t++) {
-        if (t < tcnt) {
-            const int x = xbase + (t << 5) + n;
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int m = ((i >> 2) << 3) + mhi + (i & 3);
-                if (acc[t][i] == 0 && m < nrows && x <= xmax) {
-                    const int k = m * XKEY + x;
-                    key = k < key ? k : key;
-                }
+    const int m = lane & 31;
+    if (lane < 32 && m < nrows) {
+        const int i = ((m >> 3) << 2) + (m & 3);
+        const int half = ((m >> 2) & 1) << 5;
+        for (int t = 0; t < tcnt; t++) {
+            unsigned bits = (unsigned)(s_zero[t * 16 + i] >> half);
+            const int lim = xmax - (xbase + (t << 5));   // last valid column offset in this tile
+            if (lim < 0) break;
+            if (lim < 31) bits &= (2u << lim) - 1u;
+            if (bits) {
+                key = m * XKEY + xbase + (t << 5) + __builtin_ctz(bits);
+                break;
             }
         }
     }
     return wave_min(key);
 }
 
+// Tiles per pass shrink as the chunk count grows so that the A double buffer,
+// the B fragments and the accumulators fit the register budget.
 #define MAX_NC 12
+__host__ __device__ constexpr int pass_tiles(int nc) { return nc <= 3 ? 8 : (nc <= 5 ? 4 : (nc <= MAX_NC ? 2 : 1)); }
 
-template <int TP>
-__device__ int corr_dispatch(int nc, const unsigned char *band, int stride, int rowoff, int h,
-                             const v4i *bfr, int tloc, int tcnt, int xbase, int xmax, int nrows) {
-#define CASE(K) case K: return corr_pass<TP, K>(band, stride, rowoff, h, bfr, tloc, tcnt, xbase, xmax, nrows);
+// One pass of TP tiles: correlate, combine the waves' zero masks, decode.
+template <int TP, int NC>
+__device__ __forceinline__ int one_pass(lds_cu8 *band, int stride, int rowoff, int a0, int a1,
+                                        lds_cu32 *qbits, int t0, int xbase, int xmax, int nrows,
+                                        lds_u64 *s_zero) {
+    corr_pass<TP, NC>(band, stride, rowoff, a0, a1, qbits, t0, TP, s_zero);
+    __syncthreads();
+    const int k = decode_zero(s_zero, TP, xbase + (t0 << 5), xmax, nrows);
+    __syncthreads();
+    if (threadIdx.x < TP * 16) s_zero[threadIdx.x] = ~0ull;
+    __syncthreads();
+    return k;
+}
+
+// All passes over one staged x-window (tiles [0, tl) of the window), run by
+// the whole workgroup.  The window is cut into passes of 8/4/2/1 tiles (no
+// MFMA is spent on tiles past the window).  Returns the window's minimum key.
+template <int NC>
+__device__ int window_passes(lds_cu8 *band, int stride, int rowoff, int h,
+                             lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
+                             lds_u64 *s_zero, unsigned long long &mfma) {
+    constexpr int TPM = pass_tiles(NC);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int a0 = (h * wave) >> 2, a1 = (h * (wave + 1)) >> 2;
+    int key = KEY_NONE;
+    for (int t0 = 0; t0 < tl;) {
+        const int rem = tl - t0;
+        int k, tp;
+        if (TPM >= 8 && rem >= 8) {
+            k = one_pass<(TPM >= 8 ? 8 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zero);
+            tp = 8;
+        } else if (TPM >= 4 && rem >= 4) {
+            k = one_pass<(TPM >= 4 ? 4 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zero);
+            tp = 4;
+        } else if (TPM >= 2 && rem >= 2) {
+            k = one_pass<(TPM >= 2 ? 2 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zero);
+            tp = 2;
+        } else {
+            k = one_pass<1, NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zero);
+            tp = 1;
+        }
+        mfma += (unsigned long long)tp * (a1 - a0) * NC;
+        key = k < key ? k : key;
+        t0 += tp;
+    }
+    return key;
+}
+
+// Generic chunk count (NC above the compiled set): one tile per pass, runtime
+// chunk loop, one LDS fragment per MFMA.
+__device__ int window_passes_generic(lds_cu8 *band, int stride, int rowoff, int h, int nc,
+                                     lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
+                                     lds_u64 *s_zero, unsigned long long &mfma) {
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int shift = 32 + ((lane >> 5) << 4) - (lane & 31);
+    const int a0 = (h * wave) >> 2, a1 = (h * (wave + 1)) >> 2;
+    int key = KEY_NONE;
+    for (int t0 = 0; t0 < tl; t0++) {
+        v16i acc = (v16i){};
+        lds_cu8 *base = band + rowoff * stride + (t0 << 5) + ((lane >> 5) << 4);
+        for (int a = a0; a < a1; a++) {
+            lds_cu8 *rp = base + a * stride;
+            lds_cu32 *q = qbits + a * (nc + 1);
+            for (int c = 0; c < nc; c++)
+                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(*(lds_cv4i *)(rp + (c << 5)),
+                                                            bfrag_from_bits(q[c], q[c + 1], shift), acc, 0, 0, 0);
+        }
+        mfma += (unsigned long long)(a1 - a0) * nc;
+        for (int i = 0; i < 16; i++) {
+            const unsigned long long z = __ballot(acc[i] == 0);
+            if (lane == 0 && z != ~0ull) __hip_atomic_fetch_and(&s_zero[i], z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        __syncthreads();
+        const int k = decode_zero(s_zero, 1, xbase + (t0 << 5), xmax, nrows);
+        key = k < key ? k : key;
+        __syncthreads();
+        if (threadIdx.x < 16) s_zero[threadIdx.x] = ~0ull;
+        __syncthreads();
+    }
+    return key;
+}
+
+__device__ __forceinline__ int window_dispatch(int nc, lds_cu8 *band, int stride, int rowoff, int h,
+                               lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
+                               lds_u64 *s_zero, unsigned long long &mfma) {
+#define CASE(K) case K: return window_passes<K>(band, stride, rowoff, h, qbits, tl, xbase, xmax, nrows, s_zero, mfma);
     switch (nc) {
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
         default:
-            return corr_pass_generic<TP>(band, stride, rowoff, h, nc, bfr, tloc, tcnt, xbase, xmax, nrows);
+            return window_passes_generic(band, stride, rowoff, h, nc, qbits, tl, xbase, xmax, nrows, s_zero, mfma);
     }
 #undef CASE
 }
@@ -255,60 +371,57 @@ __host__ __device__ __forceinline__ int band_window(int T, int nc, int h, int ba
     return tw;
 }
 
+#define WINDOW_CAP 64   // x-tiles per staged window at most
+
 // Collision test of one rotation against one bin, run by the whole workgroup.
 // cand: candidate top rows, descending.  Groups of up to 32 candidates are
 // staged into the LDS band (rows ylo..ytop+h-1, one x-window at a time) and
-// correlated; the 4 waves split the window's x-tiles.  Returns
-// (candidate index)*XKEY + x of the bottom-most, then left-most, placement
-// with zero overlap, or KEY_NONE.  Uniform across the workgroup.
-template <int TP, bool DENSE>
+// correlated.  Returns (candidate index)*XKEY + x of the bottom-most, then
+// left-most, placement with zero overlap, or KEY_NONE.  Uniform across the
+// workgroup.
+template <bool DENSE>
 __device__ int test_rotation(const unsigned char *grid, int gpitch, int W, int h, int w, int nc,
-                             const v4i *bfr, const short *cand, int ncand, unsigned char *band,
-                             int band_bytes, int *s_key, unsigned long long &groups,
-                             unsigned long long &mfma) {
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+                             const unsigned *qbits_g, const short *cand, int ncand, lds_u8 *band,
+                             int band_bytes, lds_u32 *s_q, lds_u64 *s_zero, unsigned long long &groups,
+                             unsigned long long &mfma, unsigned long long *ph) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    // mask bits of the rotation -> LDS (read by every wave for every group)
+    for (int k = tid; k < h * (nc + 1); k += WG_THREADS) s_q[k] = qbits_g[k];
     const int T = (W - w + 1 + 31) >> 5;
     int stride, rcap;
-    const int TW = band_window(T, nc, h, band_bytes, 4 * TP, &stride, &rcap);
+    const int TW = band_window(T, nc, h, band_bytes, WINDOW_CAP, &stride, &rcap);
     const int maxrows = min(GROUP_ROWS, rcap - h + 1);
     int found = KEY_NONE;
     for (int g0 = 0; g0 < ncand;) {
-        const int ytop = cand[g0];
+        const int ytop = uni(cand[g0]);
         int g1 = g0 + 1;
-        while (g1 < ncand && g1 - g0 < maxrows && ytop + h - cand[g1] <= rcap) g1++;
-        const int ylo = cand[g1 - 1];
+        while (g1 < ncand && g1 - g0 < maxrows && ytop + h - uni(cand[g1]) <= rcap) g1++;
+        const int ylo = uni(cand[g1 - 1]);
         const int rows = ytop + h - ylo;
         const int nrows = g1 - g0;
         const int rowoff = cand[g0 + min(lane & 31, nrows - 1)] - ylo;
         int gkey = KEY_NONE;
         groups++;
         for (int tw0 = 0; tw0 < T; tw0 += TW) {
+            unsigned long long t0 = tid == 0 ? tick() : 0;
             const int tl = min(TW, T - tw0);
             const int q16 = (tl + nc - 1) * 2;
             const unsigned char *src = grid + (long long)ylo * gpitch + (tw0 << 5);
             for (int k = tid; k < rows * q16; k += WG_THREADS) {
                 const int r = k / q16, q = k - r * q16;
-                *(v4i *)(band + r * stride + (q << 4)) =
+                *(lds_v4i *)(band + r * stride + (q << 4)) =
                     *(const v4i *)(src + (long long)r * gpitch + (q << 4));
             }
             __syncthreads();
-            const int per = (tl + 3) >> 2;
-            const int tb = wave * per;
-            const int tc = max(0, min(per, tl - tb));
-            for (int tp = 0; tp < tc; tp += TP) {
-                const int cnt = min(TP, tc - tp);
-                const int k = corr_dispatch<TP>(nc, band, stride, rowoff, h, bfr, tb + tp, cnt,
-                                                (tw0 + tb + tp) << 5, W - w, nrows);
-                gkey = k < gkey ? k : gkey;
-                mfma += (unsigned long long)cnt * h * nc;
-            }
-            __syncthreads();
+            unsigned long long t1 = tid == 0 ? tick() : 0;
+            if (tid == 0) ph[2] += t1 - t0;
+            const int k = window_dispatch(nc, band, stride, rowoff, h, s_q, tl, tw0 << 5, W - w,
+                                          nrows, s_zero, mfma);
+            gkey = k < gkey ? k : gkey;
+            if (tid == 0) ph[3] += tick() - t1;
         }
-        if (lane == 0) s_key[wave] = gkey;
-        __syncthreads();
-        const int kmin = min(min(s_key[0], s_key[1]), min(s_key[2], s_key[3]));
-        if (kmin != KEY_NONE) {
-            if (found == KEY_NONE) found = (g0 + kmin / XKEY) * XKEY + kmin % XKEY;
+        if (gkey != KEY_NONE) {
+            if (found == KEY_NONE) found = (g0 + gkey / XKEY) * XKEY + gkey % XKEY;
             if (!DENSE) break;
         }
         g0 = g1;
@@ -320,20 +433,28 @@ __device__ int test_rotation(const unsigned char *grid, int gpitch, int W, int h
 // segment, then an ordered tree combine of (prefix run, suffix run, max run,
 // all-free, length) summaries (numba_utils.py:161-188 computes the same
 // sequentially).
-__device__ __forceinline__ int row_free_run(const unsigned char *rp, int W) {
+__device__ __forceinline__ int row_free_run(lds_cu8 *rp, int W) {
     const int lane = threadIdx.x & 63;
-    const int seg = (W + 63) >> 6;
+    const int seg = (W + 63) >> 6;   // <= 8 for rows <= 512 cells; longer rows loop
     const int c0 = min(W, lane * seg), c1 = min(W, c0 + seg);
     int p = 0, mx = 0, run = 0, len = c1 - c0;
     bool full = true, seen = false;
-    for (int c = c0; c < c1; c++) {
-        if (rp[c]) {
-            if (!seen) { p = run; seen = true; }
-            run = 0;
-            full = false;
-        } else {
-            run++;
-            mx = run > mx ? run : mx;
+    for (int cb = c0; cb < c1; cb += 8) {
+        unsigned char cell[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) cell[j] = rp[min(cb + j, W - 1)];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (cb + j < c1) {
+                if (cell[j]) {
+                    if (!seen) { p = run; seen = true; }
+                    run = 0;
+                    full = false;
+                } else {
+                    run++;
+                    mx = run > mx ? run : mx;
+                }
+            }
         }
     }
     if (!seen) p = run;
@@ -361,7 +482,7 @@ struct __align__(8) PlaceRes { int has, y, x, pad; };
 
 __global__ void reset_counters_kernel(unsigned int *queue) { *queue = 0u; }
 
-template <int TP, bool DENSE>
+template <bool DENSE>
 __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, DevWork Wk, LaunchArgs L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = P.n_parts;
@@ -371,13 +492,15 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
     double *s_sv = (double *)smem;                          // [n]
     int *s_mach = (int *)(s_sv + n);                        // [n]
     int *s_seq = s_mach + n;                                // [n]
-    int *s_vac = s_seq + n;                                 // [hmax]
-    int *s_den = s_vac + hmax;                              // [MAX_ROT][hrot]
+    int *s_den = s_seq + n;                                 // [MAX_ROT][hrot]
     short *s_cand = (short *)(s_den + MAX_ROT * Wk.hrot);   // [MAX_ROT][hmax]
-    unsigned char *band = smem + Wk.band_off;               // [band_bytes]
+    short *s_vac = s_cand + MAX_ROT * hmax;                 // [hmax] vacancy of an uncached bin
+    short *s_vcache = s_vac + hmax;                         // [vc_bins][hmax] vacancy of bins < vc_bins
+    lds_u32 *s_q = (lds_u32 *)(smem + Wk.q_off);            // [hrot * (max nc + 1)]
+    lds_u8 *band = (lds_u8 *)(smem + Wk.band_off);          // [band_bytes]
     __shared__ int s_ctx, s_cnt, s_off, s_infeasible, s_nb;
     __shared__ int s_ncand[MAX_ROT];
-    __shared__ int s_key[4];
+    __shared__ unsigned long long s_zero[8 * 16];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
@@ -392,12 +515,15 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
 
     unsigned long long c_ctx = 0, c_place = 0, c_bins = 0, c_tests = 0, c_groups = 0, c_mfma = 0,
                        c_algo = 0, c_over = 0;
+    unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ctx_max = 0, ctx_sum = 0;
     const long long n_ctx = L.n_chrom * M;
+    if (tid < 8 * 16) s_zero[tid] = ~0ull;
 
     for (;;) {
+        unsigned long long tc0 = tid == 0 ? tick() : 0, tp = tc0;
         if (tid == 0) s_ctx = (int)atomicAdd(Wk.queue, 1u);
         __syncthreads();
-        const long long ctx = s_ctx;
+        const long long ctx = uni(s_ctx);
         if (ctx >= n_ctx) break;
         const long long chrom = ctx / M;
         const int mach = (int)(ctx % M);
@@ -424,8 +550,9 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
             if (!P.fits[(long long)mach * P.n_types + P.slot_type[j]]) s_infeasible = 1;
         }
         __syncthreads();
-        const int cnt = s_cnt;
-        const int log_off = s_off;
+        if (tid == 0) { const unsigned long long t = tick(); ph[0] += t - tp; tp = t; }
+        const int cnt = uni(s_cnt);
+        const int log_off = uni(s_off);
         int *logp = L.log ? L.log + (chrom * n + log_off) * 6 : nullptr;
         if (s_infeasible) {   // placement.py:47-53
             if (tid == 0) L.ctx_ms[ctx] = -1.0;
@@ -437,26 +564,36 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
         c_ctx++;
         bool overflow = false;
         for (int i = 0; i < cnt; i++) {
-            const int ty = P.slot_type[s_seq[i]];
+            const int ty = uni(P.slot_type[uni(s_seq[i])]);
             const double parea = P.area[ty];
             const int nr = P.nrot[ty];
             const int rb = P.rot_base[ty];
-            const int nb = s_nb;
+            const int nb = uni(s_nb);
             // densities of this part's rotations -> LDS
             for (int r = 0; r < nr; r++) {
                 const int h = P.rot_h[rb + r];
                 const int *den = P.dens + P.dens_off[rb + r];
                 for (int a = tid; a < h; a += WG_THREADS) s_den[r * Wk.hrot + a] = den[a];
             }
+            __syncthreads();   // s_den is read by every wave's vacancy filter
             bool placed = false;
             int pb = 0, py = 0, px = 0, prot = 0;
+            bool fresh = false;
             for (int b = 0; b < nb && !placed; b++) {
                 if (b_area[b] + parea > plate_area) continue;   // placement.py:57-59
                 const unsigned char *grid = grids + (long long)b * hmax * Wk.pitch;
-                const int *vac = vacs + (long long)b * hmax;
-                for (int y = tid; y < H; y += WG_THREADS) s_vac[y] = vac[y];
-                __syncthreads();
+                if (tid == 0) tp = tick();
+                const short *vrow = s_vcache + b * hmax;
+                if (b >= Wk.vc_bins) {   // vacancy of bins beyond the LDS cache comes from HBM
+                    const int *vac = vacs + (long long)b * hmax;
+                    for (int y = tid; y < H; y += WG_THREADS) s_vac[y] = (short)vac[y];
+                    vrow = s_vac;
+                    __syncthreads();
+                }
                 // ---- vacancy filter, per candidate top row (wave r: rotation r) ----
+                // A top row y can only hold the part if every grid row y+a has a
+                // free run >= the mask row's longest 1-run (numba_utils.py:131-157
+                // applies the same test to decide whether the bin is tried).
                 if (wave < nr) {
                     const int rg = rb + wave;
                     const int h = P.rot_h[rg], w = P.rot_w[rg];
@@ -464,10 +601,41 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                     const int *den = s_den + wave * Wk.hrot;
                     int nc_ = 0;
                     if (h <= H && w <= W) {
-                        for (int yb = H - h; yb >= 0; yb -= 64) {
-                            const int y = yb - lane;
-                            bool ok = y >= 0;
-                            for (int a = 0; ok && a < h; a++) ok = s_vac[y + a] >= den[a];
+                        // Bitset form: word k holds top rows y in [64k, 64k+64); for
+                        // mask row a, lane l tests grid row 64k + a + l, so the ballot
+                        // is already aligned with the candidate word.  Four mask rows
+                        // per step keep many LDS reads in flight.
+                        const int ny = H - h + 1;
+                        const int nk = (ny + 63) >> 6;   // <= 8 for plates <= 512 rows
+                        unsigned long long cw[8];
+#pragma unroll
+                        for (int k = 0; k < 8; k++) {
+                            const int left = ny - (k << 6);
+                            cw[k] = k >= nk ? 0ull : (left >= 64 ? ~0ull : ((1ull << left) - 1ull));
+                        }
+                        // loads are unconditional (clamped index) and the guards are
+                        // folded into the predicate, so all 4 x 8 reads of a step are
+                        // in flight together
+                        for (int a = 0; a < h; a += 4) {
+                            unsigned long long any = 0;
+#pragma unroll
+                            for (int j = 0; j < 4; j++) {
+                                const int aj = min(a + j, h - 1);
+                                const int d = den[aj];
+#pragma unroll
+                                for (int k = 0; k < 8; k++) {
+                                    const int r = (k << 6) + aj + lane;
+                                    const int v = vrow[min(r, H - 1)];
+                                    cw[k] &= __ballot(r >= H || v >= d);
+                                }
+                            }
+#pragma unroll
+                            for (int k = 0; k < 8; k++) any |= cw[k];
+                            if (!any) break;
+                        }
+                        for (int k = nk - 1; k >= 0; k--) {   // descending y
+                            const int y = (k << 6) + 63 - lane;
+                            const bool ok = (cw[k] >> (63 - lane)) & 1ull;
                             const unsigned long long bal = __ballot(ok);
                             const int pos = __popcll(bal & ((1ull << lane) - 1ull));
                             if (ok) cand[nc_ + pos] = (short)y;
@@ -481,23 +649,24 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                     if (lane == 0) s_ncand[wave] = nc_;
                 }
                 __syncthreads();
+                if (tid == 0) ph[1] += tick() - tp;
                 // ---- collision tests, one rotation at a time (whole workgroup) ----
                 int best = -1, bl = 0, brow = 0, bx = 0, by = 0;
                 const int enc = H - b_minrow[b];
                 for (int r = 0; r < nr; r++) {
-                    const int ncand = s_ncand[r];
+                    const int ncand = uni(s_ncand[r]);
                     if (ncand == 0) continue;
                     const int rg = rb + r;
                     const int h = P.rot_h[rg], w = P.rot_w[rg];
                     c_tests++;
                     c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
                     const short *cand = s_cand + r * hmax;
-                    const int key = test_rotation<TP, DENSE>(grid, Wk.pitch, W, h, w, P.rot_nc[rg],
-                                                             P.bfrag + P.bfrag_off[rg], cand, ncand,
-                                                             band, Wk.band_bytes, s_key, c_groups, c_mfma);
+                    const int key = test_rotation<DENSE>(grid, Wk.pitch, W, h, w, P.rot_nc[rg],
+                                                         P.qbits + P.qbits_off[rg], cand, ncand,
+                                                         band, Wk.band_bytes, s_q, (lds_u64 *)s_zero, c_groups, c_mfma, ph);
                     if (key == KEY_NONE) continue;
                     // density (new enclosure length) -> row -> col (binClassNew.py:101-117)
-                    const int y = cand[key / XKEY], x = key % XKEY, row = y + h - 1;
+                    const int y = uni(cand[key / XKEY]), x = key % XKEY, row = y + h - 1;
                     const int nl = enc > H - y ? enc : H - y;
                     if (best < 0 || nl < bl || (nl == bl && row > brow) || (nl == bl && row == brow && x < bx)) {
                         best = r; bl = nl; brow = row; bx = x; by = y;
@@ -509,39 +678,69 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                 }
                 __syncthreads();
             }
+            if (tid == 0) tp = tick();
             if (!placed) {
                 if (nb >= Wk.max_bins) { overflow = true; break; }
                 pb = nb; prot = P.best_rot[ty];
                 py = H - P.rot_h[rb + prot]; px = 0;
+                fresh = true;
                 // fresh bin: zero grid, full vacancy (full_native_decoder.py:2087-2089)
                 v4i *g4 = (v4i *)(grids + (long long)pb * hmax * Wk.pitch);
                 const int nvec = H * Wk.pitch / 16;
                 for (int k = tid; k < nvec; k += WG_THREADS) g4[k] = (v4i){0, 0, 0, 0};
                 int *vv = vacs + (long long)pb * hmax;
                 for (int k = tid; k < H; k += WG_THREADS) vv[k] = W;
+                if (pb < Wk.vc_bins)
+                    for (int k = tid; k < H; k += WG_THREADS) s_vcache[pb * hmax + k] = (short)W;
                 if (tid == 0) {
                     b_area[pb] = 0.0; b_pt[pb] = 0.0; b_pth[pb] = 0.0; b_minrow[pb] = H;
                     s_nb = nb + 1;
                 }
                 c_bins++;
                 __syncthreads();
+                if (tid == 0) { const unsigned long long t = tick(); ph[6] += t - tp; tp = t; }
             }
             // ---- stamp the mask (binClassNew.py:155-160, cuda_batch_update.py:90-126) ----
             {
                 const int rg = rb + prot;
                 const int h = P.rot_h[rg], w = P.rot_w[rg];
-                const unsigned char *mk = P.masks + P.mask_off[rg];
                 unsigned char *grid = grids + (long long)pb * hmax * Wk.pitch;
-                for (int k = tid; k < h * w; k += WG_THREADS) {
-                    const int a = k / w, bcol = k - a * w;
-                    if (mk[k]) grid[(long long)(py + a) * Wk.pitch + px + bcol] = 1;
-                }
-                __syncthreads();
-                // ---- free-run update of the stamped rows (numba_utils.py:161-188) ----
+                // rows [py, py+h) pass through LDS in chunks: load (a fresh bin's
+                // rows are zero), stamp the mask, write back, recompute free runs
+                const int wq = (W + 15) >> 4;               // 16-byte columns per row
+                const int chunk = min(h, Wk.band_bytes / (wq << 4));
                 int *vv = vacs + (long long)pb * hmax;
-                for (int a = wave; a < h; a += 4) {
-                    const int v = row_free_run(grid + (long long)(py + a) * Wk.pitch, W);
-                    if (lane == 0) vv[py + a] = v;
+                for (int a0 = 0; a0 < h; a0 += chunk) {
+                    const int rows = min(chunk, h - a0);
+                    for (int k = tid; k < rows * wq; k += WG_THREADS) {
+                        const int r = k / wq, q = k - r * wq;
+                        *(lds_v4i *)(band + ((r * wq + q) << 4)) =
+                            fresh ? (v4i){0, 0, 0, 0}
+                                  : *(const v4i *)(grid + (long long)(py + a0 + r) * Wk.pitch + (q << 4));
+                    }
+                    const int nq = P.rot_nc[rg] + 1;   // mask bits of these rows -> LDS
+                    const unsigned *qg = P.qbits + P.qbits_off[rg] + (long long)a0 * nq;
+                    for (int k = tid; k < rows * nq; k += WG_THREADS) s_q[k] = qg[k];
+                    __syncthreads();
+                    for (int k = tid; k < rows * w; k += WG_THREADS) {
+                        const int r = k / w, bcol = k - r * w;
+                        const unsigned bits = s_q[r * nq + ((bcol + 32) >> 5)];
+                        if ((bits >> ((bcol + 32) & 31)) & 1u) band[((r * wq) << 4) + px + bcol] = 1;
+                    }
+                    __syncthreads();
+                    for (int k = tid; k < rows * wq; k += WG_THREADS) {
+                        const int r = k / wq, q = k - r * wq;
+                        *(v4i *)(grid + (long long)(py + a0 + r) * Wk.pitch + (q << 4)) =
+                            *(lds_cv4i *)(band + ((r * wq + q) << 4));
+                    }
+                    for (int r = wave; r < rows; r += 4) {   // numba_utils.py:161-188
+                        const int v = row_free_run(band + ((r * wq) << 4), W);
+                        if (lane == 0) {
+                            vv[py + a0 + r] = v;
+                            if (pb < Wk.vc_bins) s_vcache[pb * hmax + py + a0 + r] = (short)v;
+                        }
+                    }
+                    __syncthreads();
                 }
                 if (tid == 0) {   // binClassNew.py:147-153,172-174 ; placement.py:81-82
                     const long long mp = (long long)mach * P.n_types + ty;
@@ -558,6 +757,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                 }
                 c_place++;
                 __syncthreads();
+                if (tid == 0) ph[5] += tick() - tp;
             }
         }
         if (tid == 0) {
@@ -566,10 +766,14 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                 c_over++;
             } else {
                 double total = 0.0;   // placement.py:86-89
-                const int nbf = s_nb;
+                const int nbf = uni(s_nb);
                 for (int b = 0; b < nbf; b++) total = total + ((b_pt[b] + b_pth[b]) + P.setup[mach]);
                 L.ctx_ms[ctx] = total;
             }
+            const unsigned long long t = tick();
+            ph[7] += t - tp;
+            ctx_sum += t - tc0;
+            ctx_max = t - tc0 > ctx_max ? t - tc0 : ctx_max;
         }
         __syncthreads();
     }
@@ -583,6 +787,9 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
         atomicAdd(&Wk.counters[ST_PLACE], c_place);
         atomicAdd(&Wk.counters[ST_BINS], c_bins);
         atomicAdd(&Wk.counters[ST_OVERFLOW], c_over);
+        for (int k = 0; k < 8; k++) atomicAdd(&Wk.counters[ST_PH0 + k], ph[k]);
+        atomicMax(&Wk.counters[ST_CTXMAX], ctx_max);
+        atomicAdd(&Wk.counters[ST_CTXSUM], ctx_sum);
     }
 }
 
@@ -629,6 +836,7 @@ struct brkga_handle {
     double kernel_ms = 0.0, last_ms = 0.0;
     size_t smem = 0;
     int max_bins_req = 0;
+    double wall_hz = 1e8;
     std::string err;
 };
 
@@ -768,12 +976,17 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, h->device) != hipSuccess) { h->err = "hipGetDeviceProperties failed"; return bail(BRKGA_EDEVICE); }
     h->n_cu = prop.multiProcessorCount;
+    {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, h->device) == hipSuccess && khz > 0)
+            h->wall_hz = khz * 1e3;
+    }
 
     const int M = p->n_machines, T = p->n_types, R = p->n_rots;
     // ---- per-rotation tables: densities (BRKGA_alg3.py:351-360) and Toeplitz operands ----
     std::vector<int> dens, nc(R);
-    std::vector<long long> dens_off(R), bfrag_off(R);
-    std::vector<v4i> bfrag;
+    std::vector<long long> dens_off(R), qbits_off(R);
+    std::vector<unsigned> qbits;
     int hmax = 0, wmax = 0, pitch = 64;
     for (int m = 0; m < M; m++) { hmax = std::max(hmax, p->plate_h[m]); wmax = std::max(wmax, p->plate_w[m]); }
     for (int r = 0; r < R; r++) {
@@ -786,20 +999,13 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
             dens.push_back(best);
         }
         nc[r] = (rw + 31 + 31) / 32;
-        bfrag_off[r] = (long long)bfrag.size();
-        for (int a = 0; a < rh; a++)
-            for (int c = 0; c < nc[r]; c++)
-                for (int lane = 0; lane < 64; lane++) {
-                    unsigned char bytes[16];
-                    for (int j = 0; j < 16; j++) {
-                        const int k = ((lane >> 5) << 4) + j, nn = lane & 31;
-                        const int col = 32 * c + k - nn;
-                        bytes[j] = (col >= 0 && col < rw) ? mk[a * rw + col] : 0;
-                    }
-                    v4i v;
-                    std::memcpy(&v, bytes, 16);
-                    bfrag.push_back(v);
-                }
+        qbits_off[r] = (long long)qbits.size();
+        for (int a = 0; a < rh; a++) {
+            const size_t row0 = qbits.size();
+            qbits.resize(row0 + nc[r] + 1, 0u);
+            for (int b = 0; b < rw; b++)
+                if (mk[a * rw + b]) qbits[row0 + (b + 32) / 32] |= 1u << ((b + 32) % 32);
+        }
         for (int m = 0; m < M; m++) {
             const int H = p->plate_h[m], W = p->plate_w[m];
             if (rh > H || rw > W) continue;
@@ -836,10 +1042,10 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
         (rc = upload(h, nc.data(), (size_t)R, &d.rot_nc)) ||
         (rc = upload(h, (const long long *)p->mask_off, (size_t)R, &d.mask_off)) ||
         (rc = upload(h, dens_off.data(), (size_t)R, &d.dens_off)) ||
-        (rc = upload(h, bfrag_off.data(), (size_t)R, &d.bfrag_off)) ||
+        (rc = upload(h, qbits_off.data(), (size_t)R, &d.qbits_off)) ||
         (rc = upload(h, p->masks, (size_t)mask_bytes, &d.masks)) ||
         (rc = upload(h, dens.data(), dens.size(), &d.dens)) ||
-        (rc = upload(h, bfrag.data(), bfrag.size(), &d.bfrag)) ||
+        (rc = upload(h, qbits.data(), qbits.size(), &d.qbits)) ||
         (rc = upload(h, p->plate_h, (size_t)M, &d.plate_h)) ||
         (rc = upload(h, p->plate_w, (size_t)M, &d.plate_w)) ||
         (rc = upload(h, p->setup, (size_t)M, &d.setup)) ||
@@ -863,9 +1069,14 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     int hrot = 1;
     for (int r = 0; r < R; r++) hrot = std::max(hrot, p->rot_h[r]);
     w.hrot = hrot;
-    size_t fixed = (size_t)p->n_parts * (8 + 4 + 4) + (size_t)hmax * 4 + (size_t)MAX_ROT * hrot * 4 +
-                   (size_t)MAX_ROT * hmax * 2;
+    w.vc_bins = std::min(mb, 16);
+    size_t fixed = (size_t)p->n_parts * (8 + 4 + 4) + (size_t)MAX_ROT * hrot * 4 +
+                   (size_t)MAX_ROT * hmax * 2 + (size_t)hmax * 2 + (size_t)w.vc_bins * hmax * 2;
     fixed = (fixed + 15) & ~(size_t)15;
+    size_t qwords = 1;
+    for (int r = 0; r < R; r++) qwords = std::max(qwords, (size_t)p->rot_h[r] * (nc[r] + 1));
+    w.q_off = (int)fixed;
+    fixed += (qwords * 4 + 15) & ~(size_t)15;
     size_t need_full = 0, need_min = 0;
     for (int m = 0; m < M; m++)
         for (int r = 0; r < R; r++) {
@@ -873,11 +1084,22 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
             if (rh > H || rw > W) continue;
             const int Tt = (W - rw + 1 + 31) / 32;
             need_full = std::max(need_full, (size_t)(rh + GROUP_ROWS - 1) *
-                                                band_stride(std::min(Tt, 4 * h->tiles_per_pass), nc[r]));
+                                                band_stride(std::min(Tt, WINDOW_CAP), nc[r]));
             need_min = std::max(need_min, (size_t)rh * band_stride(1, nc[r]));
         }
     size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (size_t)prop.maxSharedMemoryPerMultiProcessor : 65536;
-    int wpc2 = opt.wgs_per_cu > 0 ? opt.wgs_per_cu : (fixed + need_full + LDS_RESERVE <= lds_cu / 2 ? 2 : 1);
+    // Two workgroups per CU when a 32-row group of every rotation fits with at
+    // least a 4-tile window; windows shrink at run time when a row is wider.
+    size_t need_4 = 0;
+    for (int m = 0; m < M; m++)
+        for (int r = 0; r < R; r++) {
+            const int H = p->plate_h[m], W = p->plate_w[m], rh = p->rot_h[r], rw = p->rot_w[r];
+            if (rh > H || rw > W) continue;
+            const int Tt = (W - rw + 1 + 31) / 32;
+            need_4 = std::max(need_4, (size_t)(rh + GROUP_ROWS - 1) * band_stride(std::min(Tt, 4), nc[r]));
+        }
+    (void)need_full;
+    int wpc2 = opt.wgs_per_cu > 0 ? opt.wgs_per_cu : (fixed + need_4 + LDS_RESERVE <= lds_cu / 2 ? 2 : 1);
     size_t per_wg = lds_cu / wpc2;
     if (per_wg < fixed + need_min + LDS_RESERVE) {
         char b2[200];
@@ -891,7 +1113,7 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     h->smem = fixed + w.band_bytes;
     h->n_wg = std::max(1, h->n_cu * wpc2);   // persistent grid at the occupancy the LDS allows
     if ((rc = device_workspace(h, mb))) return bail(rc);
-    for (auto kern : {decode_kernel<2, false>, decode_kernel<2, true>, decode_kernel<4, false>, decode_kernel<4, true>})
+    for (auto kern : {decode_kernel<false>, decode_kernel<true>})
         if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->smem) != hipSuccess) {
             h->err = "hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed";
             return bail(BRKGA_EDEVICE);
@@ -930,8 +1152,7 @@ static int launch(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, s, h->dw.queue);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipEventRecord(h->ev0, s));
-    auto kern = h->tiles_per_pass == 2 ? (h->dense ? decode_kernel<2, true> : decode_kernel<2, false>)
-                                       : (h->dense ? decode_kernel<4, true> : decode_kernel<4, false>);
+    auto kern = h->dense ? decode_kernel<true> : decode_kernel<false>;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, L);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipEventRecord(h->ev1, s));
@@ -1037,6 +1258,10 @@ extern "C" int brkga_get_stats(const brkga_handle *hc, brkga_stats *out) {
     out->overflows = (int64_t)c[ST_OVERFLOW];
     out->kernel_ms = h->kernel_ms;
     out->last_kernel_ms = h->last_ms;
+    const double us_per_tick = 1e6 / h->wall_hz;
+    for (int k = 0; k < 8; k++) out->phase_us[k] = (double)c[ST_PH0 + k] * us_per_tick;
+    out->context_us_max = (double)c[ST_CTXMAX] * us_per_tick;
+    out->context_us_sum = (double)c[ST_CTXSUM] * us_per_tick;
     return 0;
 }
 

@@ -42,7 +42,12 @@ class BrkgaStats(ctypes.Structure):
                 ("bins_opened", ctypes.c_int64), ("tests", ctypes.c_int64),
                 ("row_groups", ctypes.c_int64), ("mfma_ops", ctypes.c_int64),
                 ("algo_ops", ctypes.c_int64), ("overflows", ctypes.c_int64),
-                ("kernel_ms", ctypes.c_double), ("last_kernel_ms", ctypes.c_double)]
+                ("kernel_ms", ctypes.c_double), ("last_kernel_ms", ctypes.c_double),
+                ("phase_us", ctypes.c_double * 8), ("context_us_max", ctypes.c_double),
+                ("context_us_sum", ctypes.c_double)]
+
+PHASES = ["sequence", "vacancy_filter", "band_staging", "correlation", "group_reduce",
+          "stamp_update", "new_bin", "makespan"]
 
 
 EXPORTED = ["brkga_create", "brkga_evaluate", "brkga_evaluate_device", "brkga_get_stats",
@@ -174,7 +179,9 @@ class Decoder:
     def stats(self):
         st = BrkgaStats()
         self._check(self._lib.brkga_get_stats(self._h, ctypes.byref(st)), "brkga_get_stats")
-        return {k: getattr(st, k) for k, _ in BrkgaStats._fields_}
+        d = {k: getattr(st, k) for k, _ in BrkgaStats._fields_}
+        d["phase_us"] = dict(zip(PHASES, list(st.phase_us)))
+        return d
 
     def reset_stats(self):
         self._check(self._lib.brkga_reset_stats(self._h), "brkga_reset_stats")

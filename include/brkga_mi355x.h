@@ -99,6 +99,13 @@ typedef struct brkga_stats {
     int64_t overflows;       /* contexts that exceeded max_bins */
     double kernel_ms;        /* device time of the decode kernels (HIP events) */
     double last_kernel_ms;   /* decode kernel time of the last call */
+    /* Where a workgroup's time goes, summed over workgroups, in microseconds of
+     * the GPU's constant-rate wall clock: [0] sequence decode, [1] vacancy
+     * filter, [2] band staging, [3] correlation, [4] group reduction,
+     * [5] stamp + free-run update, [6] new-bin clearing, [7] makespan. */
+    double phase_us[8];
+    double context_us_max;   /* longest single context */
+    double context_us_sum;   /* all contexts */
 } brkga_stats;
 
 typedef struct brkga_handle brkga_handle;
