@@ -1,0 +1,130 @@
+"""BRKGA generation loop: drop-in for the reference's ``BRKGA`` class.
+
+Host loop (`BRKGA.fit`) restates ``BRKGA_alg3.py:23-281`` with the same NumPy
+call order, so a run seeded with ``np.random.seed`` draws the same random
+numbers as the reference's:
+
+* initial population ``np.random.uniform(0, 1, (N, 2n)).astype(float32)``,
+  ``population[0] = initialSol`` (``:212-214``);
+* ``partition``: ``np.argpartition(fitness, num_elites)`` then the elites
+  sorted by fitness (``:178-186``);
+* ``mating``: ``randint`` elite parents, ``randint`` non-elite parents,
+  ``uniform`` crossover mask ``< eliteCProb`` (``:188-205``);
+* ``mutants``: ``uniform(0, 1, (num_mutants, 2n))`` (``:207-208``);
+* next population ``elites + mutants + offspring``, fitness of the elites
+  reused (``:244-250``).
+
+Fitness comes from the MI355X decoder through ``cal_fitness`` (the
+``native_full`` / ``wave_batch`` evaluators of ``BRKGA_alg3.py:119-123``).
+"""
+import math
+import time
+
+import numpy as np
+
+from .evaluator import FullNativeDecoderEvaluator
+
+
+class BRKGA:
+    """Biased random-key GA over the population decoder."""
+
+    def __init__(self, problem_data, nbParts, nbMachines, thresholds, instanceParts, initialSol,
+                 collision_backend=None, eval_mode="native_full", eval_workers=4, eval_chunksize=1,
+                 num_generations=200, num_individuals=100, num_elites=12, num_mutants=18,
+                 eliteCProb=0.7, evaluator=None):
+        self.problem_data = problem_data
+        self.nbMachines = nbMachines
+        self.thresholds = thresholds
+        self.N = nbParts
+        self.instanceParts = instanceParts
+        self.initialSol = initialSol
+        self.collision_backend = collision_backend
+        if eval_mode not in ("native_full", "wave_batch", "auto"):
+            raise ValueError("eval_mode %r: this build decodes on the GPU (native_full / wave_batch)"
+                             % eval_mode)
+        self.eval_mode = "native_full"
+        self.eval_workers = eval_workers
+        self.eval_chunksize = eval_chunksize
+        self.num_generations = num_generations
+        self.num_individuals = int(num_individuals)
+        self.num_gene = 2 * self.N
+        self.num_elites = int(num_elites)
+        self.num_mutants = int(num_mutants)
+        self.eliteCProb = eliteCProb
+        self.used_bins = -1
+        self.solution = None
+        self.best_fitness = -1
+        self.history = {"mean": [], "min": [], "time": []}
+        self._native_decoder = evaluator or FullNativeDecoderEvaluator(
+            problem_data, nbParts, nbMachines, thresholds, instanceParts, collision_backend)
+
+    def cal_fitness(self, population):
+        """``BRKGA_alg3.py:119-121``: a Python list of makespans."""
+        return self._native_decoder.evaluate_batch(np.array(population))
+
+    def partition(self, population, fitness_list):
+        fitness_arr = np.asarray(fitness_list)
+        partition_indices = np.argpartition(fitness_arr, self.num_elites)
+        elite_indices = partition_indices[:self.num_elites]
+        non_elite_indices = partition_indices[self.num_elites:]
+        elite_indices = elite_indices[np.argsort(fitness_arr[elite_indices])]
+        return population[elite_indices], population[non_elite_indices], fitness_arr[elite_indices]
+
+    def mating(self, elites, non_elites):
+        num_offspring = self.num_individuals - self.num_elites - self.num_mutants
+        elite_idx = np.random.randint(0, len(elites), size=num_offspring)
+        non_elite_idx = np.random.randint(0, len(non_elites), size=num_offspring)
+        elite_parents = elites[elite_idx]
+        non_elite_parents = non_elites[non_elite_idx]
+        mask = np.random.uniform(0.0, 1.0, size=(num_offspring, self.num_gene)) < self.eliteCProb
+        return np.where(mask, elite_parents, non_elite_parents).astype(np.float32)
+
+    def mutants(self):
+        return np.random.uniform(low=0.0, high=1.0,
+                                 size=(self.num_mutants, self.num_gene)).astype(np.float32)
+
+    def fit(self, verbose=False, target=None):
+        """Run the generations; stops early once ``target`` makespan is reached."""
+        startfit = time.time()
+        population = np.random.uniform(low=0.0, high=1.0,
+                                       size=(self.num_individuals, self.num_gene)).astype(np.float32)
+        if self.initialSol is not None:
+            population[0] = self.initialSol
+        fitness_list = self.cal_fitness(population)
+        best_fitness = np.min(fitness_list)
+        best_solution = population[np.argmin(fitness_list)]
+        self.history["min"].append(np.min(fitness_list))
+        self.history["mean"].append(np.mean(fitness_list))
+        self.history["time"].append(time.time() - startfit)
+        for g in range(self.num_generations):
+            if target is not None and best_fitness <= target:
+                break
+            t0 = time.time()
+            elites, non_elites, elite_fitness_list = self.partition(population, fitness_list)
+            offsprings = self.mating(elites, non_elites)
+            mutants = self.mutants()
+            offspring = np.concatenate((mutants, offsprings), axis=0)
+            offspring_fitness_list = self.cal_fitness(offspring)
+            population = np.concatenate((elites, mutants, offsprings), axis=0)
+            fitness_list = list(elite_fitness_list) + offspring_fitness_list
+            min_idx = np.argmin(fitness_list)
+            min_fitness = fitness_list[min_idx]
+            if min_fitness < best_fitness:
+                best_fitness = min_fitness
+                best_solution = population[min_idx]
+            self.history["min"].append(min_fitness)
+            self.history["mean"].append(np.mean(fitness_list))
+            self.history["time"].append(time.time() - startfit)
+            if verbose:
+                print("Generation %d: Best=%.4f, Time=%.2fs" % (g, best_fitness, time.time() - t0))
+        self.used_bins = math.floor(best_fitness)
+        self.best_fitness = best_fitness
+        self.solution = best_solution
+        return "feasible"
+
+
+def brkga_sizes(nb_parts, mult=10):
+    """Population sizes of the reference's entry point (``BRKGA_alg3.py:608-618``)."""
+    n = mult * nb_parts
+    return dict(num_individuals=n, num_elites=math.ceil(n * 0.1),
+                num_mutants=math.ceil(n * 0.15), eliteCProb=0.70)

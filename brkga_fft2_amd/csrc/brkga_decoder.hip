@@ -47,6 +47,7 @@
 #include <vector>
 
 #include "../../include/brkga_mi355x.h"
+#include "brkga_internal.h"
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
@@ -1135,15 +1136,22 @@ static void settle_events(brkga_handle *h) {
     h->ev_pending = false;
 }
 
-static int launch(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_chrom, double *d_out,
-                  int32_t *d_log, hipStream_t s) {
+int brkga_internal_reserve(brkga_handle *h, int64_t n_chrom) {
     const int M = h->prob.n_machines;
     if ((size_t)(n_chrom * M) > h->d_ctx_n) {
         if (h->d_ctx) { (void)hipFree(h->d_ctx); h->d_ctx = nullptr; }
         HIPCHK(h, hipMalloc(&h->d_ctx, (size_t)n_chrom * M * sizeof(double)));
         h->d_ctx_n = (size_t)n_chrom * M;
     }
-    settle_events(h);
+    return 0;
+}
+
+int brkga_internal_launch(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_chrom,
+                          double *d_out, int32_t *d_log, hipStream_t s, bool timed) {
+    const int M = h->prob.n_machines;
+    int rc = brkga_internal_reserve(h, n_chrom);
+    if (rc) return rc;
+    if (timed) settle_events(h);
     LaunchArgs L;
     L.keys = d_keys; L.key_f64 = key_dtype == BRKGA_KEYS_F64; L.n_chrom = n_chrom;
     L.ctx_ms = h->d_ctx; L.log = d_log;
@@ -1151,18 +1159,24 @@ static int launch(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_
     const int grid = (int)std::min<long long>(h->n_wg, std::max<long long>(n_ctx, 1));
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, s, h->dw.queue);
     HIPCHK(h, hipGetLastError());
-    HIPCHK(h, hipEventRecord(h->ev0, s));
+    if (timed) HIPCHK(h, hipEventRecord(h->ev0, s));
     auto kern = h->dense ? decode_kernel<true> : decode_kernel<false>;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, L);
     HIPCHK(h, hipGetLastError());
-    HIPCHK(h, hipEventRecord(h->ev1, s));
-    h->ev_pending = true;
+    if (timed) {
+        HIPCHK(h, hipEventRecord(h->ev1, s));
+        h->ev_pending = true;
+    }
     const int fb = 256;
     hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)((n_chrom + fb - 1) / fb)), dim3(fb), 0, s,
                        h->d_ctx, (long long)n_chrom, M, d_out, h->dw.counters);
     HIPCHK(h, hipGetLastError());
     return 0;
 }
+
+int brkga_internal_n_parts(const brkga_handle *h) { return h->prob.n_parts; }
+int brkga_internal_device(const brkga_handle *h) { return h->device; }
+void brkga_internal_error(brkga_handle *h, const char *msg) { h->err = msg; }
 
 static int check_args(brkga_handle *h, const void *keys, int key_dtype, int64_t n_chrom, int64_t n_genes,
                       const void *out) {
@@ -1190,7 +1204,7 @@ extern "C" int brkga_evaluate_device(brkga_handle *h, const void *d_keys, int ke
     hipStream_t s = (hipStream_t)stream;
     unsigned long long ov0 = 0, ov1 = 0;
     if (check_overflow && (rc = read_overflow(h, s, &ov0))) return rc;
-    if ((rc = launch(h, d_keys, key_dtype, n_chrom, d_out, d_log, s))) return rc;
+    if ((rc = brkga_internal_launch(h, d_keys, key_dtype, n_chrom, d_out, d_log, s, true))) return rc;
     if (check_overflow) {
         if ((rc = read_overflow(h, s, &ov1))) return rc;
         settle_events(h);

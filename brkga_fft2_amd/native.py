@@ -50,8 +50,16 @@ PHASES = ["sequence", "vacancy_filter", "band_staging", "correlation", "group_re
           "stamp_update", "new_bin", "makespan"]
 
 
+class BrkgaGaParams(ctypes.Structure):
+    _fields_ = [("num_individuals", ctypes.c_int32), ("num_elites", ctypes.c_int32),
+                ("num_mutants", ctypes.c_int32), ("history_capacity", ctypes.c_int32),
+                ("elite_prob", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
 EXPORTED = ["brkga_create", "brkga_evaluate", "brkga_evaluate_device", "brkga_get_stats",
-            "brkga_reset_stats", "brkga_last_error", "brkga_destroy"]
+            "brkga_reset_stats", "brkga_last_error", "brkga_destroy",
+            "brkga_ga_create", "brkga_ga_run", "brkga_ga_best", "brkga_ga_population",
+            "brkga_ga_last_error", "brkga_ga_destroy"]
 
 _lib = None
 
@@ -88,6 +96,18 @@ def load_library(path=LIB_PATH):
     lib.brkga_last_error.restype = ctypes.c_char_p
     lib.brkga_destroy.argtypes = [_P]
     lib.brkga_destroy.restype = None
+    lib.brkga_ga_create.argtypes = [_P, ctypes.POINTER(BrkgaGaParams), _P, PP]
+    lib.brkga_ga_create.restype = ctypes.c_int
+    lib.brkga_ga_run.argtypes = [_P, ctypes.c_int32, ctypes.c_int, _P]
+    lib.brkga_ga_run.restype = ctypes.c_int
+    lib.brkga_ga_best.argtypes = [_P, _P, _P, _P, _P]
+    lib.brkga_ga_best.restype = ctypes.c_int
+    lib.brkga_ga_population.argtypes = [_P, _P, _P]
+    lib.brkga_ga_population.restype = ctypes.c_int
+    lib.brkga_ga_last_error.argtypes = []
+    lib.brkga_ga_last_error.restype = ctypes.c_char_p
+    lib.brkga_ga_destroy.argtypes = [_P]
+    lib.brkga_ga_destroy.restype = None
     _lib = lib
     return lib
 
@@ -189,6 +209,78 @@ class Decoder:
     def close(self):
         if getattr(self, "_h", None):
             self._lib.brkga_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceGA:
+    """Device-resident BRKGA loop over a :class:`Decoder` (one hipGraph per generation)."""
+
+    def __init__(self, decoder, num_individuals, num_elites, num_mutants, elite_prob=0.7, seed=0,
+                 history_capacity=1024, init_population=None):
+        self._lib = load_library()
+        self.decoder = decoder
+        self.n_genes = 2 * decoder.n_parts
+        self.num_individuals = int(num_individuals)
+        p = BrkgaGaParams(int(num_individuals), int(num_elites), int(num_mutants),
+                          int(history_capacity), float(elite_prob), int(seed))
+        init = None
+        if init_population is not None:
+            init = np.ascontiguousarray(init_population, dtype=np.float32)
+            if init.shape != (self.num_individuals, self.n_genes):
+                raise ValueError("init_population must be [num_individuals, 2*n_parts]")
+        h = ctypes.c_void_p()
+        rc = self._lib.brkga_ga_create(decoder._h, ctypes.byref(p),
+                                       init.ctypes.data if init is not None else None,
+                                       ctypes.byref(h))
+        if rc != 0:
+            raise DecoderError(rc, "brkga_ga_create: " + self._lib.brkga_ga_last_error().decode())
+        self._h = h
+
+    def run(self, generations, use_graph=True, stream=None):
+        """Run generations on ``stream`` (a torch stream).  Graph replay needs a
+        non-null stream; by default the loop uses a stream of its own."""
+        import torch
+        if stream is None:
+            if getattr(self, "_stream", None) is None:
+                self._stream = torch.cuda.Stream(device=torch.device("cuda", self.decoder.device))
+            stream = self._stream
+        rc = self._lib.brkga_ga_run(self._h, int(generations), int(bool(use_graph)),
+                                    ctypes.c_void_p(stream.cuda_stream))
+        if rc != 0:
+            raise DecoderError(rc, "brkga_ga_run: " + self._lib.brkga_ga_last_error().decode())
+
+    def best(self):
+        """``(best_fitness, best_keys float32[2n], history float64[done+1], generations_done)``."""
+        bf = ctypes.c_double()
+        keys = np.empty(self.n_genes, np.float32)
+        done = ctypes.c_int32()
+        rc = self._lib.brkga_ga_best(self._h, None, None, None, ctypes.byref(done))
+        if rc != 0:
+            raise DecoderError(rc, "brkga_ga_best: " + self._lib.brkga_ga_last_error().decode())
+        hist = np.empty(done.value + 1, np.float64)
+        rc = self._lib.brkga_ga_best(self._h, ctypes.byref(bf), keys.ctypes.data, hist.ctypes.data,
+                                     ctypes.byref(done))
+        if rc != 0:
+            raise DecoderError(rc, "brkga_ga_best: " + self._lib.brkga_ga_last_error().decode())
+        return bf.value, keys, hist, done.value
+
+    def population(self):
+        keys = np.empty((self.num_individuals, self.n_genes), np.float32)
+        fit = np.empty(self.num_individuals, np.float64)
+        rc = self._lib.brkga_ga_population(self._h, keys.ctypes.data, fit.ctypes.data)
+        if rc != 0:
+            raise DecoderError(rc, "brkga_ga_population: " + self._lib.brkga_ga_last_error().decode())
+        return keys, fit
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.brkga_ga_destroy(self._h)
             self._h = None
 
     def __del__(self):

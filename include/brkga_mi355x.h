@@ -143,6 +143,43 @@ const char *brkga_last_error(const brkga_handle *h);
 
 void brkga_destroy(brkga_handle *h);
 
+/*
+ * Device-resident BRKGA generation loop.  The reference runs the loop on the
+ * host (BRKGA.fit, BRKGA_alg3.py:210-281: partition :178-186, mating
+ * :188-205, mutants :207-208); here the population and its fitness stay in
+ * HBM and each generation (sort, breed, decode, keep elites' fitness, best
+ * tracking) is one captured hipGraph replay.  Random numbers come from a
+ * counter-based hash of (seed, generation, row, gene), not NumPy's stream.
+ * The decoder handle must have max_bins large enough for every context
+ * (max_bins = n_parts is always enough): a graph replay cannot grow it.
+ */
+typedef struct brkga_ga_params {
+    int32_t num_individuals;   /* population N, 2..4096 */
+    int32_t num_elites;        /* kept unchanged each generation */
+    int32_t num_mutants;       /* fresh uniform chromosomes each generation */
+    int32_t history_capacity;  /* generations whose minimum fitness is recorded */
+    double elite_prob;         /* probability an offspring gene comes from the elite parent */
+    uint64_t seed;
+} brkga_ga_params;
+
+typedef struct brkga_ga brkga_ga;
+
+/* init_population: [N][2n] float32 host keys, or NULL to draw them.  Decodes
+ * the initial population before returning. */
+int brkga_ga_create(brkga_handle *decoder, const brkga_ga_params *params,
+                    const float *init_population, brkga_ga **out);
+/* Run `generations` generations on `stream` (graph replays when use_graph),
+ * then synchronise the stream. */
+int brkga_ga_run(brkga_ga *ga, int32_t generations, int use_graph, void *stream);
+/* Best fitness so far, its keys [2n], per-generation minima [done+1]
+ * (generation 0 = initial population); each pointer may be NULL. */
+int brkga_ga_best(const brkga_ga *ga, double *best_fitness, float *best_keys, double *history,
+                  int32_t *generations_done);
+/* Current population [N][2n] and fitness [N]; each pointer may be NULL. */
+int brkga_ga_population(const brkga_ga *ga, float *keys, double *fitness);
+const char *brkga_ga_last_error(void);
+void brkga_ga_destroy(brkga_ga *ga);
+
 #ifdef __cplusplus
 }
 #endif

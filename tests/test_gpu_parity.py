@@ -23,7 +23,7 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("cfg,n_chrom", [("C1", 64), ("C2", 48), ("C3", 16)])
+@pytest.mark.parametrize("cfg,n_chrom", [("C1", 64), ("C2", 48), ("C3", 16), ("C5", 8)])
 def test_synthetic_parity(cfg, n_chrom):
     _say("synthetic parity %s N=%d" % (cfg, n_chrom))
     prob, n, M, packed = syn_instance(cfg)
