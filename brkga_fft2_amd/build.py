@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libbrkga_mi355x.so")
-SOURCES = [os.path.join(HERE, "csrc", "brkga_decoder.hip"), os.path.join(HERE, "csrc", "brkga_ga.hip")]
+SOURCES = [os.path.join(HERE, "csrc", n) for n in ("brkga_decoder.hip", "brkga_ga.hip", "brkga_surrogate.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BRKGA_OFFLOAD_ARCH", "gfx950")
 

@@ -344,6 +344,50 @@ extern "C" int brkga_ga_population(const brkga_ga *ga, float *keys, double *fitn
     return 0;
 }
 
+// Offspring pool for a screened generation: pool_mutants fresh chromosomes,
+// then pool_offspring biased-crossover children of (elite, non-elite) parent
+// pairs drawn from the sorted population (same rules as ga_breed_kernel).
+__global__ void ga_pool_kernel(const float *pop, const int *order, int N, int G2, int ne, int pm, int po,
+                               float rho, unsigned long long seed, unsigned gen, float *out) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)(pm + po) * G2) return;
+    const int r = (int)(t / G2), g = (int)(t % G2);
+    float v;
+    if (r < pm) {
+        v = ga_uniform(ga_hash(seed, gen, r, g, 4));
+    } else {
+        const int e = order[(int)(ga_hash(seed, gen, r, 0, 5) % (unsigned long long)ne)];
+        const int o = order[ne + (int)(ga_hash(seed, gen, r, 0, 6) % (unsigned long long)(N - ne))];
+        const float u = ga_uniform(ga_hash(seed, gen, r, g, 7));
+        v = u < rho ? pop[(long long)e * G2 + g] : pop[(long long)o * G2 + g];
+    }
+    out[t] = v;
+}
+
+extern "C" int brkga_sort_fitness(const double *d_fitness, int32_t n, int32_t *d_order, void *stream) {
+    if (!d_fitness || !d_order || n < 1 || n > GA_SORT_MAX) { g_ga_error = "n must be in [1, 4096]"; return BRKGA_EINVAL; }
+    hipLaunchKernelGGL(ga_sort_kernel, dim3(1), dim3(GA_SORT_THREADS), 0, (hipStream_t)stream, d_fitness, n, d_order);
+    GACHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int brkga_breed_pool(const float *d_population, const int32_t *d_order, int32_t n_individuals,
+                                int32_t n_genes, int32_t num_elites, int32_t pool_mutants, int32_t pool_offspring,
+                                double elite_prob, uint64_t seed, uint32_t generation, float *d_out, void *stream) {
+    if (!d_population || !d_order || !d_out || num_elites < 1 || num_elites >= n_individuals || n_genes < 2 ||
+        pool_mutants < 0 || pool_offspring < 0) {
+        g_ga_error = "bad argument";
+        return BRKGA_EINVAL;
+    }
+    const long long nk = (long long)(pool_mutants + pool_offspring) * n_genes;
+    if (nk == 0) return 0;
+    hipLaunchKernelGGL(ga_pool_kernel, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       d_population, d_order, n_individuals, n_genes, num_elites, pool_mutants, pool_offspring,
+                       (float)elite_prob, seed, generation, d_out);
+    GACHK(hipGetLastError());
+    return 0;
+}
+
 extern "C" const char *brkga_ga_last_error(void) { return g_ga_error.c_str(); }
 
 extern "C" void brkga_ga_destroy(brkga_ga *ga) {

@@ -1,0 +1,466 @@
+// brkga_surrogate.hip -- BF16 MLP surrogate of the decoder's makespan.
+//
+// Network: x (F = 4n features) -> 512 ReLU -> 512 ReLU -> 1.  Weights and
+// activations are bfloat16, every product is accumulated in float32
+// (V_MFMA_F32_32X32X16_BF16), the optimiser keeps float32 master weights.
+// Features of a chromosome: its 2n keys, then the area and the height of the
+// parts in sequence order (ascending sequence key), as the surrogate-screened
+// configuration prescribes.  The reference has no surrogate; the network only
+// chooses WHICH offspring are decoded exactly, never a fitness value.
+//
+// Kernels: featurize, a tiled GEMM with transposes and fused epilogues
+// (bias, ReLU, ReLU-derivative mask, f32/bf16 outputs), the output layer and
+// loss gradient, column sums for bias gradients, and Adam.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/brkga_mi355x.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint16_t bf16_t;   // storage type
+
+__device__ __forceinline__ float bf2f(bf16_t v) { return __uint_as_float(((unsigned)v) << 16); }
+// round to nearest even (finite inputs)
+__device__ __forceinline__ bf16_t f2bf(float f) {
+    const unsigned u = __float_as_uint(f);
+    return (bf16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+#define GEMM_TILE 64
+#define GEMM_KC 32
+#define GEMM_LDS_K 40   // padded row length (bf16) of the LDS tiles
+
+enum { EPI_RELU = 1, EPI_MASK = 2 };
+
+// C[M][N] = op(A) op(B) (+ bias[n]) (ReLU) (* [mask[m][n] > 0]); f32 and/or bf16 out.
+// op(A)(m,k) = TA ? A[k*lda + m] : A[m*lda + k];  op(B)(k,n) = TB ? B[n*ldb + k] : B[k*ldb + n].
+// One workgroup = 4 waves = one 64x64 tile (2x2 of 32x32 MFMA tiles).
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, const bf16_t *A, int lda,
+                                                        const bf16_t *B, int ldb, const float *bias,
+                                                        const bf16_t *mask, int ldm, int flags,
+                                                        float *Cf, bf16_t *Cb, int ldc) {
+    __shared__ __align__(16) bf16_t As[GEMM_TILE][GEMM_LDS_K];
+    __shared__ __align__(16) bf16_t Bs[GEMM_TILE][GEMM_LDS_K];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m0 = blockIdx.y * GEMM_TILE, n0 = blockIdx.x * GEMM_TILE;
+    const int wm = (wave >> 1) << 5, wn = (wave & 1) << 5;
+    f32x16 acc = {};
+    for (int k0 = 0; k0 < K; k0 += GEMM_KC) {
+        // stage 64 x 32 of op(A) and 32 x 64 of op(B), both with k contiguous
+        for (int e = tid; e < GEMM_TILE * GEMM_KC; e += 256) {
+            int r, kk;
+            if (TA) { kk = e / GEMM_TILE; r = e % GEMM_TILE; } else { r = e / GEMM_KC; kk = e % GEMM_KC; }
+            const int m = m0 + r, k = k0 + kk;
+            As[r][kk] = (m < M && k < K) ? (TA ? A[(long long)k * lda + m] : A[(long long)m * lda + k]) : (bf16_t)0;
+            int c, kb;
+            if (TB) { c = e / GEMM_KC; kb = e % GEMM_KC; } else { kb = e / GEMM_TILE; c = e % GEMM_TILE; }
+            const int n = n0 + c, k2 = k0 + kb;
+            Bs[c][kb] = (n < N && k2 < K) ? (TB ? B[(long long)n * ldb + k2] : B[(long long)k2 * ldb + n]) : (bf16_t)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < GEMM_KC; ks += 16) {
+            const bf16x8 a = *(const bf16x8 *)&As[wm + (lane & 31)][ks + ((lane >> 5) << 3)];
+            const bf16x8 b = *(const bf16x8 *)&Bs[wn + (lane & 31)][ks + ((lane >> 5) << 3)];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int n = n0 + wn + (lane & 31);
+    if (n >= N) return;
+    const float bn = bias ? bias[n] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const int m = m0 + wm + ((i >> 2) << 3) + ((lane >> 5) << 2) + (i & 3);
+        if (m >= M) continue;
+        float v = acc[i] + bn;
+        if (flags & EPI_RELU) v = v > 0.0f ? v : 0.0f;
+        if ((flags & EPI_MASK) && !(bf2f(mask[(long long)m * ldm + n]) > 0.0f)) v = 0.0f;
+        if (Cf) Cf[(long long)m * ldc + n] = v;
+        if (Cb) Cb[(long long)m * ldc + n] = f2bf(v);
+    }
+}
+
+// Features of each chromosome (one workgroup per row): keys, then area and
+// height of the parts ordered by ascending sequence key (ties by slot).
+// Machine of a machine key: the first threshold bucket holding it (placement.py:121-127).
+__device__ __forceinline__ int machine_bucket(double mv, const double *thr, int M) {
+    if (M <= 1 || mv <= thr[0]) return 0;
+    for (int m = 1; m < M - 1; m++)
+        if (mv <= thr[m]) return m;
+    return M - 1;
+}
+
+__global__ __launch_bounds__(256) void featurize_kernel(const float *keys, int B, int n, int M,
+                                                        const double *thr, int Fp, const float *slot_area,
+                                                        const float *slot_height, bf16_t *x) {
+    extern __shared__ float sk[];   // [n]
+    const long long b = blockIdx.x;
+    const float *kr = keys + b * 2LL * n;
+    bf16_t *xr = x + b * Fp;
+    for (int j = threadIdx.x; j < n; j += 256) sk[j] = kr[j];
+    for (int j = 4 * n + threadIdx.x; j < Fp; j += 256) xr[j] = 0;
+    __syncthreads();
+    const float inv_den = M > 1 ? (float)(M - 1) : 1.0f;
+    for (int j = threadIdx.x; j < n; j += 256) {
+        const float kj = sk[j];
+        int rank = 0;
+        for (int i = 0; i < n; i++) rank += (sk[i] < kj) || (sk[i] == kj && i < j);
+        // the keys as the decoder reads them: sequence position and machine bucket
+        xr[j] = f2bf((float)rank / (float)n);
+        xr[n + j] = f2bf((float)machine_bucket((double)kr[n + j], thr, M) / inv_den);
+        xr[2 * n + rank] = f2bf(slot_area[j]);
+        xr[3 * n + rank] = f2bf(slot_height[j]);
+    }
+    (void)B;
+}
+
+// y[b] = H2[b] . w3 + b3 ; one wave per row.
+__global__ __launch_bounds__(256) void output_kernel(const bf16_t *H, int B, int Hd, const float *w3,
+                                                     const float *b3, float *y) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= B) return;
+    float s = 0.0f;
+    for (int j = lane; j < Hd; j += 64) s += bf2f(H[(long long)row * Hd + j]) * w3[j];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) y[row] = s + b3[0];
+}
+
+// dy = 2 (y - t) / B ; loss partial sums
+__global__ void loss_grad_kernel(const float *y, const float *t, int B, float *dy, float *loss) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const float d = y[b] - t[b];
+    dy[b] = 2.0f * d / (float)B;
+    if (loss) atomicAdd(loss, d * d / (float)B);
+}
+
+// dH2[b][j] = dy[b] * w3[j] * [H2[b][j] > 0]  (bf16)
+__global__ void dh2_kernel(const float *dy, const float *w3, const bf16_t *H2, int B, int Hd, bf16_t *dH2) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)B * Hd) return;
+    const int b = (int)(t / Hd), j = (int)(t % Hd);
+    dH2[t] = f2bf(bf2f(H2[t]) > 0.0f ? dy[b] * w3[j] : 0.0f);
+}
+
+// g[j] (+)= sum_b M[b][j] * (w ? w[b] : 1)
+__global__ void colsum_kernel(const bf16_t *Mx, int B, int Hd, const float *w, float *g) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= Hd) return;
+    float s = 0.0f;
+    for (int b = 0; b < B; b++) s += bf2f(Mx[(long long)b * Hd + j]) * (w ? w[b] : 1.0f);
+    g[j] = s;
+}
+
+__global__ void sum_kernel(const float *v, int B, float *out) {
+    __shared__ float sh[256];
+    float s = 0.0f;
+    for (int b = threadIdx.x; b < B; b += 256) s += v[b];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+__global__ void adam_kernel(float *p, const float *g, float *m, float *v, long long n, float lr, float b1,
+                            float b2, float eps, float c1, float c2, bf16_t *pb, const long long *bf_map,
+                            float wd) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i] + wd * p[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    (void)bf_map;
+    if (pb) pb[i] = f2bf(pi);
+}
+
+__global__ void to_bf16_kernel(const float *p, long long n, bf16_t *pb) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) pb[i] = f2bf(p[i]);
+}
+
+struct brkga_mlp {
+    int F = 0, Fp = 0, Hd = 512, n = 0, M = 1;
+    double *thr = nullptr;   // [max(M-1, 1)] machine-key thresholds
+    int device = 0;
+    // parameters: W1 [Fp][Hd], b1 [Hd], W2 [Hd][Hd], b2 [Hd], w3 [Hd], b3 [1] (f32 master)
+    long long oW1, ob1, oW2, ob2, ow3, ob3, np_;
+    float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    bf16_t *pb = nullptr;   // bf16 copy of every parameter (GEMMs read W1, W2)
+    float *slot_area = nullptr, *slot_height = nullptr;
+    // activations (grown on demand)
+    long long cap = 0;
+    bf16_t *x = nullptr, *H1 = nullptr, *H2 = nullptr, *dH2 = nullptr, *dH1 = nullptr;
+    float *y = nullptr, *dy = nullptr, *loss = nullptr;
+    long long step = 0;
+    float lr = 1e-3f, wd = 0.0f;
+    std::vector<void *> allocs;
+    std::string err;
+};
+
+static thread_local std::string g_mlp_error;
+
+#define MLPCHK(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            g_mlp_error = std::string(#call) + ": " + hipGetErrorString(e_);                  \
+            return e_ == hipErrorOutOfMemory ? BRKGA_ENOMEM : BRKGA_EDEVICE;                  \
+        }                                                                                     \
+    } while (0)
+
+static int mlp_alloc(brkga_mlp *net, size_t bytes, void **q) {
+    MLPCHK(hipMalloc(q, std::max<size_t>(bytes, 16)));
+    net->allocs.push_back(*q);
+    return 0;
+}
+
+static int mlp_reserve(brkga_mlp *net, long long B) {
+    if (B <= net->cap) return 0;
+    for (void *q : {(void *)net->x, (void *)net->H1, (void *)net->H2, (void *)net->dH2, (void *)net->dH1,
+                    (void *)net->y, (void *)net->dy}) {
+        if (!q) continue;
+        auto it = std::find(net->allocs.begin(), net->allocs.end(), q);
+        if (it != net->allocs.end()) { (void)hipFree(q); net->allocs.erase(it); }
+    }
+    void *q;
+    int rc;
+    if ((rc = mlp_alloc(net, (size_t)B * net->Fp * 2, &q))) return rc; net->x = (bf16_t *)q;
+    if ((rc = mlp_alloc(net, (size_t)B * net->Hd * 2, &q))) return rc; net->H1 = (bf16_t *)q;
+    if ((rc = mlp_alloc(net, (size_t)B * net->Hd * 2, &q))) return rc; net->H2 = (bf16_t *)q;
+    if ((rc = mlp_alloc(net, (size_t)B * net->Hd * 2, &q))) return rc; net->dH2 = (bf16_t *)q;
+    if ((rc = mlp_alloc(net, (size_t)B * net->Hd * 2, &q))) return rc; net->dH1 = (bf16_t *)q;
+    if ((rc = mlp_alloc(net, (size_t)B * 4, &q))) return rc; net->y = (float *)q;
+    if ((rc = mlp_alloc(net, (size_t)B * 4, &q))) return rc; net->dy = (float *)q;
+    net->cap = B;
+    return 0;
+}
+
+template <bool TA, bool TB>
+static void gemm(hipStream_t s, int M, int N, int K, const bf16_t *A, int lda, const bf16_t *B, int ldb,
+                 const float *bias, const bf16_t *mask, int ldm, int flags, float *Cf, bf16_t *Cb, int ldc) {
+    dim3 grid((N + GEMM_TILE - 1) / GEMM_TILE, (M + GEMM_TILE - 1) / GEMM_TILE);
+    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, bias, mask,
+                       ldm, flags, Cf, Cb, ldc);
+}
+
+static void forward(brkga_mlp *net, long long B, hipStream_t s) {
+    const int Hd = net->Hd;
+    gemm<false, false>(s, (int)B, Hd, net->Fp, net->x, net->Fp, net->pb + net->oW1, Hd, net->p + net->ob1,
+                       nullptr, 0, EPI_RELU, nullptr, net->H1, Hd);
+    gemm<false, false>(s, (int)B, Hd, Hd, net->H1, Hd, net->pb + net->oW2, Hd, net->p + net->ob2, nullptr, 0,
+                       EPI_RELU, nullptr, net->H2, Hd);
+    hipLaunchKernelGGL(output_kernel, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, s, net->H2, (int)B, Hd,
+                       net->p + net->ow3, net->p + net->ob3, net->y);
+}
+
+// gradients of the mean squared error into net->g (the inputs are already in net->x)
+static void backward(brkga_mlp *net, const float *t, long long B, hipStream_t s) {
+    const int Hd = net->Hd;
+    hipLaunchKernelGGL(loss_grad_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, net->y, t, (int)B,
+                       net->dy, net->loss);
+    hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, s, net->H2, (int)B, Hd, net->dy,
+                       net->g + net->ow3);
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, net->dy, (int)B, net->g + net->ob3);
+    hipLaunchKernelGGL(dh2_kernel, dim3((unsigned)((B * Hd + 255) / 256)), dim3(256), 0, s, net->dy,
+                       net->p + net->ow3, net->H2, (int)B, Hd, net->dH2);
+    hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, s, net->dH2, (int)B, Hd, nullptr,
+                       net->g + net->ob2);
+    // dW2 = H1^T dH2
+    gemm<true, false>(s, Hd, Hd, (int)B, net->H1, Hd, net->dH2, Hd, nullptr, nullptr, 0, 0, net->g + net->oW2,
+                      nullptr, Hd);
+    // dH1 = (dH2 W2^T) * [H1 > 0]
+    gemm<false, true>(s, (int)B, Hd, Hd, net->dH2, Hd, net->pb + net->oW2, Hd, nullptr, net->H1, Hd, EPI_MASK,
+                      nullptr, net->dH1, Hd);
+    hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, s, net->dH1, (int)B, Hd, nullptr,
+                       net->g + net->ob1);
+    // dW1 = X^T dH1
+    gemm<true, false>(s, net->Fp, Hd, (int)B, net->x, net->Fp, net->dH1, Hd, nullptr, nullptr, 0, 0,
+                      net->g + net->oW1, nullptr, Hd);
+}
+
+extern "C" int brkga_mlp_create(int32_t n_parts, int32_t n_machines, const double *thresholds,
+                                int32_t hidden, const float *slot_area, const float *slot_height,
+                                uint64_t seed, float lr, int32_t device, brkga_mlp **out) {
+    if (!out || n_parts < 1 || n_machines < 1 || (n_machines > 1 && !thresholds) || hidden < 32 ||
+        hidden % 32 || !slot_area || !slot_height) {
+        g_mlp_error = "bad argument (n_parts >= 1, n_machines >= 1 with n_machines-1 thresholds, "
+                      "hidden a positive multiple of 32, feature arrays)";
+        return BRKGA_EINVAL;
+    }
+    for (int m = 1; m + 1 < n_machines; m++)
+        if (!(thresholds[m] >= thresholds[m - 1])) { g_mlp_error = "thresholds must be ascending"; return BRKGA_EINVAL; }
+    *out = nullptr;
+    brkga_mlp *net = new (std::nothrow) brkga_mlp();
+    if (!net) { g_mlp_error = "host allocation failed"; return BRKGA_ENOMEM; }
+    auto bail = [&](int rc) { brkga_mlp_destroy(net); return rc; };
+    net->device = device;
+    net->n = n_parts;
+    net->M = n_machines;
+    net->F = 4 * n_parts;
+    net->Fp = (net->F + GEMM_KC - 1) / GEMM_KC * GEMM_KC;
+    net->Hd = hidden;
+    net->lr = lr > 0 ? lr : 1e-3f;
+    if (hipSetDevice(device) != hipSuccess) { g_mlp_error = "hipSetDevice failed"; return bail(BRKGA_EDEVICE); }
+    const long long Fp = net->Fp, Hd = hidden;
+    net->oW1 = 0; net->ob1 = Fp * Hd; net->oW2 = net->ob1 + Hd; net->ob2 = net->oW2 + Hd * Hd;
+    net->ow3 = net->ob2 + Hd; net->ob3 = net->ow3 + Hd; net->np_ = net->ob3 + 1;
+    // He-uniform initialisation (host, splitmix64 stream), zero biases
+    std::vector<float> h((size_t)net->np_, 0.0f);
+    unsigned long long z = seed * 0x9e3779b97f4a7c15ull + 12345;
+    auto uni = [&]() {
+        z += 0x9e3779b97f4a7c15ull;
+        unsigned long long x = z;
+        x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+        x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+        x ^= x >> 31;
+        return (float)((double)(x >> 11) * (1.0 / 9007199254740992.0)) * 2.0f - 1.0f;
+    };
+    const float s1 = std::sqrt(6.0f / (float)net->F), s2 = std::sqrt(6.0f / (float)Hd);
+    for (long long i = 0; i < Fp * Hd; i++) h[(size_t)(net->oW1 + i)] = (i / Hd) < net->F ? uni() * s1 : 0.0f;
+    for (long long i = 0; i < Hd * Hd; i++) h[(size_t)(net->oW2 + i)] = uni() * s2;
+    for (long long i = 0; i < Hd; i++) h[(size_t)(net->ow3 + i)] = uni() * s2;
+    void *q;
+    int rc;
+    if ((rc = mlp_alloc(net, (size_t)net->np_ * 4, &q))) return bail(rc); net->p = (float *)q;
+    if ((rc = mlp_alloc(net, (size_t)net->np_ * 4, &q))) return bail(rc); net->g = (float *)q;
+    if ((rc = mlp_alloc(net, (size_t)net->np_ * 4, &q))) return bail(rc); net->m = (float *)q;
+    if ((rc = mlp_alloc(net, (size_t)net->np_ * 4, &q))) return bail(rc); net->v = (float *)q;
+    if ((rc = mlp_alloc(net, (size_t)net->np_ * 2, &q))) return bail(rc); net->pb = (bf16_t *)q;
+    if ((rc = mlp_alloc(net, (size_t)n_parts * 4, &q))) return bail(rc); net->slot_area = (float *)q;
+    if ((rc = mlp_alloc(net, (size_t)n_parts * 4, &q))) return bail(rc); net->slot_height = (float *)q;
+    if ((rc = mlp_alloc(net, 16, &q))) return bail(rc); net->loss = (float *)q;
+    std::vector<double> thr((size_t)std::max(n_machines - 1, 1), 0.0);
+    for (int m = 0; m + 1 < n_machines; m++) thr[(size_t)m] = thresholds[m];
+    if ((rc = mlp_alloc(net, thr.size() * 8, &q))) return bail(rc); net->thr = (double *)q;
+    if (hipMemcpy(net->p, h.data(), (size_t)net->np_ * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(net->g, 0, (size_t)net->np_ * 4) != hipSuccess ||
+        hipMemset(net->m, 0, (size_t)net->np_ * 4) != hipSuccess ||
+        hipMemset(net->v, 0, (size_t)net->np_ * 4) != hipSuccess ||
+        hipMemcpy(net->slot_area, slot_area, (size_t)n_parts * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(net->slot_height, slot_height, (size_t)n_parts * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(net->thr, thr.data(), thr.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        g_mlp_error = "device initialisation failed";
+        return bail(BRKGA_EDEVICE);
+    }
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((net->np_ + 255) / 256)), dim3(256), 0, 0, net->p, net->np_,
+                       net->pb);
+    if (hipDeviceSynchronize() != hipSuccess) { g_mlp_error = "initialisation kernel failed"; return bail(BRKGA_EDEVICE); }
+    *out = net;
+    return BRKGA_OK;
+}
+
+static int mlp_features(brkga_mlp *net, const float *d_keys, long long B, hipStream_t s) {
+    int rc = mlp_reserve(net, B);
+    if (rc) return rc;
+    hipLaunchKernelGGL(featurize_kernel, dim3((unsigned)B), dim3(256), (size_t)net->n * 4, s, d_keys, (int)B, net->n,
+                       net->M, net->thr, net->Fp, net->slot_area, net->slot_height, net->x);
+    MLPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int brkga_mlp_predict(brkga_mlp *net, const float *d_keys, int64_t B, float *d_pred, void *stream) {
+    if (!net || !d_keys || !d_pred || B < 1) { g_mlp_error = "bad argument"; return BRKGA_EINVAL; }
+    MLPCHK(hipSetDevice(net->device));
+    hipStream_t s = (hipStream_t)stream;
+    int rc = mlp_features(net, d_keys, B, s);
+    if (rc) return rc;
+    forward(net, B, s);
+    MLPCHK(hipMemcpyAsync(d_pred, net->y, (size_t)B * 4, hipMemcpyDeviceToDevice, s));
+    MLPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int brkga_mlp_gradients(brkga_mlp *net, const float *d_keys, const float *d_target, int64_t B,
+                                   float *d_grad, void *stream) {
+    if (!net || !d_keys || !d_target || B < 1) { g_mlp_error = "bad argument"; return BRKGA_EINVAL; }
+    MLPCHK(hipSetDevice(net->device));
+    hipStream_t s = (hipStream_t)stream;
+    int rc = mlp_features(net, d_keys, B, s);
+    if (rc) return rc;
+    forward(net, B, s);
+    MLPCHK(hipMemsetAsync(net->loss, 0, 4, s));
+    backward(net, d_target, B, s);
+    if (d_grad) MLPCHK(hipMemcpyAsync(d_grad, net->g, (size_t)net->np_ * 4, hipMemcpyDeviceToDevice, s));
+    MLPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int brkga_mlp_train_step(brkga_mlp *net, const float *d_keys, const float *d_target, int64_t B,
+                                    float *h_loss, void *stream) {
+    int rc = brkga_mlp_gradients(net, d_keys, d_target, B, nullptr, stream);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    net->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - std::pow(b1, (float)net->step), c2 = 1.0f - std::pow(b2, (float)net->step);
+    hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((net->np_ + 255) / 256)), dim3(256), 0, s, net->p, net->g, net->m,
+                       net->v, net->np_, net->lr, b1, b2, 1e-8f, c1, c2, net->pb, nullptr, net->wd);
+    MLPCHK(hipGetLastError());
+    if (h_loss) {
+        MLPCHK(hipMemcpyAsync(h_loss, net->loss, 4, hipMemcpyDeviceToHost, s));
+        MLPCHK(hipStreamSynchronize(s));
+    }
+    return 0;
+}
+
+extern "C" int brkga_mlp_num_params(const brkga_mlp *net, int64_t *n_params, int32_t *n_features_padded) {
+    if (!net) return BRKGA_EINVAL;
+    if (n_params) *n_params = net->np_;
+    if (n_features_padded) *n_features_padded = net->Fp;
+    return 0;
+}
+
+extern "C" int brkga_mlp_get_params(const brkga_mlp *net, float *h_params) {
+    if (!net || !h_params) return BRKGA_EINVAL;
+    MLPCHK(hipSetDevice(net->device));
+    MLPCHK(hipDeviceSynchronize());
+    MLPCHK(hipMemcpy(h_params, net->p, (size_t)net->np_ * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int brkga_mlp_set_params(brkga_mlp *net, const float *h_params) {
+    if (!net || !h_params) return BRKGA_EINVAL;
+    MLPCHK(hipSetDevice(net->device));
+    MLPCHK(hipMemcpy(net->p, h_params, (size_t)net->np_ * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((net->np_ + 255) / 256)), dim3(256), 0, 0, net->p, net->np_,
+                       net->pb);
+    MLPCHK(hipDeviceSynchronize());
+    return 0;
+}
+
+extern "C" int brkga_mlp_features(brkga_mlp *net, const float *d_keys, int64_t B, uint16_t *d_x, void *stream) {
+    if (!net || !d_keys || !d_x || B < 1) return BRKGA_EINVAL;
+    MLPCHK(hipSetDevice(net->device));
+    hipStream_t s = (hipStream_t)stream;
+    int rc = mlp_features(net, d_keys, B, s);
+    if (rc) return rc;
+    MLPCHK(hipMemcpyAsync(d_x, net->x, (size_t)B * net->Fp * 2, hipMemcpyDeviceToDevice, s));
+    return 0;
+}
+
+extern "C" const char *brkga_mlp_last_error(void) { return g_mlp_error.c_str(); }
+
+extern "C" void brkga_mlp_destroy(brkga_mlp *net) {
+    if (!net) return;
+    (void)hipSetDevice(net->device);
+    (void)hipDeviceSynchronize();
+    for (void *q : net->allocs) (void)hipFree(q);
+    delete net;
+}

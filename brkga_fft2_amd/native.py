@@ -59,7 +59,10 @@ class BrkgaGaParams(ctypes.Structure):
 EXPORTED = ["brkga_create", "brkga_evaluate", "brkga_evaluate_device", "brkga_get_stats",
             "brkga_reset_stats", "brkga_last_error", "brkga_destroy",
             "brkga_ga_create", "brkga_ga_run", "brkga_ga_best", "brkga_ga_population",
-            "brkga_ga_last_error", "brkga_ga_destroy"]
+            "brkga_ga_last_error", "brkga_ga_destroy", "brkga_sort_fitness", "brkga_breed_pool",
+            "brkga_mlp_create", "brkga_mlp_predict", "brkga_mlp_gradients", "brkga_mlp_train_step",
+            "brkga_mlp_num_params", "brkga_mlp_get_params", "brkga_mlp_set_params",
+            "brkga_mlp_features", "brkga_mlp_last_error", "brkga_mlp_destroy"]
 
 _lib = None
 
@@ -134,6 +137,7 @@ class Decoder:
         self._lib = load_library()
         self.n_parts = int(packed["n_parts"])
         self.n_machines = int(packed["n_machines"])
+        self.thresholds = np.asarray(packed["thresholds"], np.float64).reshape(-1)[:max(self.n_machines - 1, 0)]
         keep = []
         prob = BrkgaProblem(self.n_parts, self.n_machines, int(packed["n_types"]),
                             int(packed["n_rots"]))

@@ -1,0 +1,334 @@
+"""Surrogate-screened BRKGA: a BF16 MLP ranks oversized offspring pools.
+
+The network (``brkga_surrogate.hip``: 4n -> 512 -> 512 -> 1, BF16 weights and
+activations, FP32 accumulation, Adam) is trained online on the decoder's
+exact makespans.  Each generation after a warm-up:
+
+1. the population is sorted by exact fitness (``brkga_sort_fitness``);
+2. a pool of ``pool`` candidates is bred -- mutants and biased-crossover
+   offspring in the plain method's proportion (``brkga_breed_pool``);
+3. the network predicts every candidate; the best-predicted ``num_mutants``
+   mutants and ``num_offspring`` offspring are decoded exactly;
+4. the next population is the elites (exact fitness, never predicted) plus
+   the decoded candidates; the decoded pairs feed the training buffer.
+
+During the warm-up, or when the network's rank correlation on the last
+decoded batch falls below ``fallback_rho``, step 3 takes the first
+candidates of each kind instead -- the plain BRKGA generation (the pool is
+i.i.d., so its first rows are exactly what plain BRKGA would breed).
+
+The reference has no surrogate; the exact decoder is the only source of
+fitness values, so every reported makespan is exact.
+"""
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from .native import Decoder, DecoderError, load_library
+
+_P = ctypes.c_void_p
+
+
+def _lib():
+    lib = load_library()
+    if not hasattr(lib, "_mlp_bound"):
+        lib.brkga_mlp_create.argtypes = [ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int32, _P, _P,
+                                         ctypes.c_uint64, ctypes.c_float, ctypes.c_int32, ctypes.POINTER(_P)]
+        lib.brkga_mlp_predict.argtypes = [_P, _P, ctypes.c_int64, _P, _P]
+        lib.brkga_mlp_gradients.argtypes = [_P, _P, _P, ctypes.c_int64, _P, _P]
+        lib.brkga_mlp_train_step.argtypes = [_P, _P, _P, ctypes.c_int64, _P, _P]
+        lib.brkga_mlp_num_params.argtypes = [_P, _P, _P]
+        lib.brkga_mlp_get_params.argtypes = [_P, _P]
+        lib.brkga_mlp_set_params.argtypes = [_P, _P]
+        lib.brkga_mlp_features.argtypes = [_P, _P, ctypes.c_int64, _P, _P]
+        lib.brkga_mlp_last_error.argtypes = []
+        lib.brkga_mlp_last_error.restype = ctypes.c_char_p
+        lib.brkga_mlp_destroy.argtypes = [_P]
+        lib.brkga_mlp_destroy.restype = None
+        lib.brkga_sort_fitness.argtypes = [_P, ctypes.c_int32, _P, _P]
+        lib.brkga_breed_pool.argtypes = [_P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                         ctypes.c_uint64, ctypes.c_uint32, _P, _P]
+        for f in ("brkga_mlp_create", "brkga_mlp_predict", "brkga_mlp_gradients",
+                  "brkga_mlp_train_step", "brkga_mlp_num_params", "brkga_mlp_get_params",
+                  "brkga_mlp_set_params", "brkga_mlp_features", "brkga_sort_fitness",
+                  "brkga_breed_pool"):
+            getattr(lib, f).restype = ctypes.c_int
+        lib._mlp_bound = True
+    return lib
+
+
+def slot_features(problem, instance_parts, machine=0):
+    """Per-slot (area, height) scaled to [0, 1]: part area, and the height-driven
+    processing time of machine 0 (proportional to the part height)."""
+    area = np.array([problem.parts[int(p)].area for p in instance_parts], np.float64)
+    height = np.array([problem.machines[machine].parts[int(p)].proc_time_height
+                       for p in instance_parts], np.float64)
+    return ((area / area.max()).astype(np.float32),
+            (height / max(height.max(), 1e-300)).astype(np.float32))
+
+
+def _stream_ptr(stream):
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class MLPSurrogate:
+    """ctypes handle of one surrogate network on one GPU."""
+
+    def __init__(self, n_parts, slot_area, slot_height, thresholds=(), hidden=512, seed=0, lr=1e-3,
+                 device=0):
+        self._lib = _lib()
+        sa = np.ascontiguousarray(slot_area, np.float32)
+        sh = np.ascontiguousarray(slot_height, np.float32)
+        thr = np.ascontiguousarray(thresholds, np.float64).reshape(-1)
+        self.thresholds = thr.copy()
+        self.n_machines = thr.size + 1
+        h = _P()
+        rc = self._lib.brkga_mlp_create(int(n_parts), int(self.n_machines),
+                                        thr.ctypes.data if thr.size else None, int(hidden),
+                                        sa.ctypes.data, sh.ctypes.data, int(seed), float(lr), int(device),
+                                        ctypes.byref(h))
+        if rc != 0:
+            raise DecoderError(rc, "brkga_mlp_create: " + self._lib.brkga_mlp_last_error().decode())
+        self._h = h
+        self.n_parts, self.hidden, self.device = int(n_parts), int(hidden), int(device)
+        np_, fp = ctypes.c_int64(), ctypes.c_int32()
+        self._lib.brkga_mlp_num_params(self._h, ctypes.byref(np_), ctypes.byref(fp))
+        self.n_params, self.fp = np_.value, fp.value
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise DecoderError(rc, "%s: %s" % (what, self._lib.brkga_mlp_last_error().decode()))
+
+    def _stream(self, stream):
+        import torch
+        return stream or torch.cuda.current_stream(torch.device("cuda", self.device))
+
+    def predict(self, keys, out=None, stream=None):
+        import torch
+        keys = keys.contiguous()
+        if out is None:
+            out = torch.empty(keys.shape[0], dtype=torch.float32, device=keys.device)
+        s = self._stream(stream)
+        self._check(self._lib.brkga_mlp_predict(self._h, keys.data_ptr(), keys.shape[0], out.data_ptr(),
+                                                _stream_ptr(s)), "brkga_mlp_predict")
+        return out
+
+    def gradients(self, keys, targets, stream=None):
+        import torch
+        g = torch.empty(self.n_params, dtype=torch.float32, device=keys.device)
+        s = self._stream(stream)
+        self._check(self._lib.brkga_mlp_gradients(self._h, keys.contiguous().data_ptr(),
+                                                  targets.contiguous().data_ptr(), keys.shape[0],
+                                                  g.data_ptr(), _stream_ptr(s)), "brkga_mlp_gradients")
+        return g
+
+    def train_step(self, keys, targets, stream=None):
+        loss = ctypes.c_float()
+        s = self._stream(stream)
+        self._check(self._lib.brkga_mlp_train_step(self._h, keys.contiguous().data_ptr(),
+                                                   targets.contiguous().data_ptr(), keys.shape[0],
+                                                   ctypes.byref(loss), _stream_ptr(s)),
+                    "brkga_mlp_train_step")
+        return loss.value
+
+    def features(self, keys, stream=None):
+        import torch
+        x = torch.empty((keys.shape[0], self.fp), dtype=torch.int16, device=keys.device)
+        s = self._stream(stream)
+        self._check(self._lib.brkga_mlp_features(self._h, keys.contiguous().data_ptr(), keys.shape[0],
+                                                 x.data_ptr(), _stream_ptr(s)), "brkga_mlp_features")
+        return x
+
+    def get_params(self):
+        p = np.empty(self.n_params, np.float32)
+        self._check(self._lib.brkga_mlp_get_params(self._h, p.ctypes.data), "brkga_mlp_get_params")
+        return p
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        self._check(self._lib.brkga_mlp_set_params(self._h, p.ctypes.data), "brkga_mlp_set_params")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.brkga_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def spearman(a, b):
+    """Spearman rank correlation (average ranks on ties)."""
+    from scipy.stats import spearmanr
+    a, b = np.asarray(a), np.asarray(b)
+    if a.size < 3 or np.all(a == a[0]) or np.all(b == b[0]):
+        return float("nan")
+    return float(spearmanr(a, b).correlation)
+
+
+class SurrogateBRKGA:
+    """Device-resident BRKGA with optional surrogate screening (see module doc)."""
+
+    def __init__(self, decoder, problem, instance_parts, num_individuals=1024, elite_frac=0.2,
+                 mutant_frac=0.15, elite_prob=0.7, pool=4096, warmup=5, seed=0,
+                 use_surrogate=True, hidden=512, lr=1e-3, train_steps=8, batch=1024,
+                 buffer=16384, fallback_rho=0.2, audit_every=0):
+        import torch
+        self.torch = torch
+        self.dec = decoder
+        self.n = decoder.n_parts
+        self.G2 = 2 * self.n
+        self.N = int(num_individuals)
+        self.ne = int(math.ceil(elite_frac * self.N))
+        self.nm = int(math.ceil(mutant_frac * self.N))
+        self.no = self.N - self.ne - self.nm
+        self.rho = float(elite_prob)
+        self.pool = max(int(pool), self.nm + self.no)
+        self.pm = int(round(self.pool * self.nm / (self.nm + self.no)))
+        self.po = self.pool - self.pm
+        self.warmup = int(warmup)
+        self.seed = int(seed)
+        self.use_surrogate = bool(use_surrogate)
+        self.train_steps, self.batch, self.buffer = int(train_steps), int(batch), int(buffer)
+        self.fallback_rho = float(fallback_rho)
+        self.audit_every = int(audit_every)
+        self.dev = torch.device("cuda", decoder.device)
+        self.stream = torch.cuda.current_stream(self.dev)
+        self._lib = _lib()
+        sa, sh = slot_features(problem, instance_parts)
+        self.net = MLPSurrogate(self.n, sa, sh, decoder.thresholds, hidden=hidden, seed=seed, lr=lr,
+                                device=decoder.device) if self.use_surrogate else None
+        g = torch.Generator(device="cpu").manual_seed(self.seed)
+        self.pop = torch.rand((self.N, self.G2), generator=g, dtype=torch.float32).to(self.dev)
+        self.fit = torch.empty(self.N, dtype=torch.float64, device=self.dev)
+        self.order = torch.empty(self.N, dtype=torch.int32, device=self.dev)
+        self.poolbuf = torch.empty((self.pool, self.G2), dtype=torch.float32, device=self.dev)
+        self.buf_x = torch.empty((self.buffer, self.G2), dtype=torch.float32, device=self.dev)
+        self.buf_y = torch.empty(self.buffer, dtype=torch.float64, device=self.dev)
+        self.buf_n = 0
+        self.buf_pos = 0
+        self.gen = 0
+        self.rng = np.random.default_rng(self.seed + 1)
+        self.log = {"min": [], "best": [], "time": [], "spearman": [], "screened": [],
+                    "loss": [], "regret": [], "audit_spearman": []}
+        self.decoder_calls = 0
+        self._decode(self.pop, self.fit)
+        self._add_training(self.pop, self.fit)
+        self.best = float(self.fit.min())
+
+    def _decode(self, keys, out):
+        self.dec.evaluate_device(keys, out, stream=self.stream, check_overflow=False)
+        self.decoder_calls += keys.shape[0]
+
+    def _add_training(self, keys, fit):
+        torch = self.torch
+        ok = fit < 1e15
+        k, f = keys[ok], fit[ok]
+        m = k.shape[0]
+        if m == 0:
+            return
+        idx = (torch.arange(m, device=self.dev) + self.buf_pos) % self.buffer
+        self.buf_x[idx] = k
+        self.buf_y[idx] = f
+        self.buf_pos = (self.buf_pos + m) % self.buffer
+        self.buf_n = min(self.buffer, self.buf_n + m)
+
+    def _train(self):
+        torch = self.torch
+        n = self.buf_n
+        if n < 32:
+            return float("nan")
+        y = self.buf_y[:n]
+        mu, sd = y.mean(), y.std().clamp_min(1e-12)
+        self.y_mu, self.y_sd = mu, sd
+        losses = []
+        for _ in range(self.train_steps):
+            b = min(self.batch, n)
+            idx = torch.from_numpy(self.rng.integers(0, n, size=b)).to(self.dev)
+            x = self.buf_x[idx].contiguous()
+            t = ((self.buf_y[idx] - mu) / sd).float().contiguous()
+            losses.append(self.net.train_step(x, t, stream=self.stream))
+        return float(np.mean(losses))
+
+    def step(self):
+        torch = self.torch
+        lib = self._lib
+        s = _stream_ptr(self.stream)
+        lib.brkga_sort_fitness(self.fit.data_ptr(), self.N, self.order.data_ptr(), s)
+        screen = (self.use_surrogate and self.gen >= self.warmup and
+                  not (self.log["spearman"] and self.log["spearman"][-1] < self.fallback_rho))
+        pm, po = (self.pm, self.po) if screen else (self.nm, self.no)
+        rc = lib.brkga_breed_pool(self.pop.data_ptr(), self.order.data_ptr(), self.N, self.G2, self.ne,
+                                  pm, po, self.rho, self.seed, self.gen, self.poolbuf.data_ptr(), s)
+        if rc != 0:
+            raise DecoderError(rc, "brkga_breed_pool")
+        if screen:
+            pred = self.net.predict(self.poolbuf[:pm + po], stream=self.stream)
+            sel_m = torch.topk(pred[:pm], self.nm, largest=False).indices
+            sel_o = torch.topk(pred[pm:pm + po], self.no, largest=False).indices + pm
+            chosen = torch.cat([self.poolbuf[sel_m], self.poolbuf[sel_o]])
+            chosen_pred = torch.cat([pred[sel_m], pred[sel_o]])
+        else:
+            chosen = self.poolbuf[:self.nm + self.no].clone()
+            chosen_pred = None
+            if self.use_surrogate and self.gen >= 1:
+                chosen_pred = self.net.predict(chosen, stream=self.stream)
+        fit_c = torch.empty(chosen.shape[0], dtype=torch.float64, device=self.dev)
+        self._decode(chosen, fit_c)
+        if self.audit_every and screen and self.gen % self.audit_every == 0:
+            self._audit(pred, pm, po, sel_m, sel_o)
+        elite = self.order[:self.ne].long()
+        self.pop = torch.cat([self.pop[elite], chosen])
+        self.fit = torch.cat([self.fit[elite], fit_c])
+        rho_s = float("nan")
+        if chosen_pred is not None:
+            f = fit_c.cpu().numpy()
+            ok = f < 1e15
+            rho_s = spearman(chosen_pred.cpu().numpy()[ok], f[ok])
+        if self.use_surrogate:
+            self.log["spearman"].append(rho_s)
+            self._add_training(chosen, fit_c)
+            self.log["loss"].append(self._train())
+        self.log["screened"].append(bool(screen))
+        self.gen += 1
+        m = float(self.fit.min())
+        self.best = min(self.best, m)
+        self.log["min"].append(m)
+        self.log["best"].append(self.best)
+
+    def _audit(self, pred, pm, po, sel_m, sel_o):
+        """Exact decode of the whole pool: regret of the screened pick and pool-wide rho."""
+        torch = self.torch
+        allf = torch.empty(pm + po, dtype=torch.float64, device=self.dev)
+        self.dec.evaluate_device(self.poolbuf[:pm + po].contiguous(), allf, stream=self.stream,
+                                 check_overflow=False)
+        f = allf.cpu().numpy()
+        chosen = np.concatenate([sel_m.cpu().numpy(), sel_o.cpu().numpy()])
+        self.log["regret"].append(float(f[chosen].min() - f.min()))
+        ok = f < 1e15
+        self.log["audit_spearman"].append(spearman(pred.cpu().numpy()[ok], f[ok]))
+
+    def run(self, generations, target=None):
+        """Run up to ``generations``; stop early once best <= target.  Returns seconds."""
+        t0 = time.perf_counter()
+        self.time_to_target = None
+        if target is not None and self.best <= target:
+            self.time_to_target = 0.0
+            return 0.0
+        for _ in range(generations):
+            self.step()
+            self.log["time"].append(time.perf_counter() - t0)
+            if target is not None and self.best <= target:
+                self.time_to_target = time.perf_counter() - t0
+                break
+        return time.perf_counter() - t0
+
+    def close(self):
+        if self.net is not None:
+            self.net.close()

@@ -180,6 +180,51 @@ int brkga_ga_population(const brkga_ga *ga, float *keys, double *fitness);
 const char *brkga_ga_last_error(void);
 void brkga_ga_destroy(brkga_ga *ga);
 
+/* Building blocks of a screened generation (device pointers, enqueued on
+ * stream).  brkga_sort_fitness: d_order = indices of d_fitness [n <= 4096]
+ * ascending (ties by index), as the reference's partition orders elites
+ * (BRKGA_alg3.py:178-186).  brkga_breed_pool: pool_mutants uniform
+ * chromosomes followed by pool_offspring biased-crossover children
+ * (BRKGA_alg3.py:188-208) of the sorted population, keyed by (seed, generation). */
+int brkga_sort_fitness(const double *d_fitness, int32_t n, int32_t *d_order, void *stream);
+int brkga_breed_pool(const float *d_population, const int32_t *d_order, int32_t n_individuals,
+                     int32_t n_genes, int32_t num_elites, int32_t pool_mutants, int32_t pool_offspring,
+                     double elite_prob, uint64_t seed, uint32_t generation, float *d_out, void *stream);
+
+/*
+ * Surrogate: BF16 MLP with FP32 accumulation predicting a chromosome's
+ * (normalised) makespan, trained online on exact decodes.  No counterpart in
+ * the reference; it only ranks oversized offspring pools so that fewer
+ * chromosomes need an exact decode.  Input features of a chromosome (4n): its
+ * 2n keys as the decoder reads them -- the sequence position of each slot
+ * divided by n, and the machine bucket of each slot's machine key
+ * (thresholds as in brkga_problem) divided by max(n_machines-1, 1) -- then
+ * slot_area and slot_height of its parts in ascending sequence-key order
+ * (ties by slot).  Layers: 4n -> hidden -> hidden -> 1, ReLU.  Parameters
+ * (float32 master copy) are laid out W1 [Fp][hidden] (rows >= 4n are zero),
+ * b1 [hidden], W2 [hidden][hidden], b2 [hidden], w3 [hidden], b3 [1], with Fp
+ * = 4n rounded up to 32.  d_* pointers are device memory.
+ */
+typedef struct brkga_mlp brkga_mlp;
+int brkga_mlp_create(int32_t n_parts, int32_t n_machines, const double *thresholds,
+                     int32_t hidden, const float *slot_area, const float *slot_height,
+                     uint64_t seed, float learning_rate, int32_t device, brkga_mlp **out);
+/* d_keys [B][2n] float32 -> d_pred [B] float32 */
+int brkga_mlp_predict(brkga_mlp *net, const float *d_keys, int64_t B, float *d_pred, void *stream);
+/* Mean-squared-error gradient of one batch into d_grad [n_params] (nullable). */
+int brkga_mlp_gradients(brkga_mlp *net, const float *d_keys, const float *d_target, int64_t B,
+                        float *d_grad, void *stream);
+/* One Adam step on one batch; *h_loss (nullable) receives the batch loss. */
+int brkga_mlp_train_step(brkga_mlp *net, const float *d_keys, const float *d_target, int64_t B,
+                         float *h_loss, void *stream);
+int brkga_mlp_num_params(const brkga_mlp *net, int64_t *n_params, int32_t *n_features_padded);
+int brkga_mlp_get_params(const brkga_mlp *net, float *h_params);
+int brkga_mlp_set_params(brkga_mlp *net, const float *h_params);
+/* Feature rows the network sees: d_x [B][Fp] bfloat16 bits. */
+int brkga_mlp_features(brkga_mlp *net, const float *d_keys, int64_t B, uint16_t *d_x, void *stream);
+const char *brkga_mlp_last_error(void);
+void brkga_mlp_destroy(brkga_mlp *net);
+
 #ifdef __cplusplus
 }
 #endif
