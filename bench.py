@@ -5,10 +5,16 @@ plates, seeded polygon masks of 20-100 cells per side, 4 rotations, population
 256 (one BRKGA generation's decode = one step).  Keys are float32 and resident
 in HBM before the timed region; every step decodes a different population.
 
-Prints one JSON line with the throughput, the roofline of the dominant
-kernel (the decode kernel run at full shape: every placement row, no early
-exit -- the work the reference's FFT path does), and the CPU oracle timed on
-this host's cores on a bounded sample of the same workload.
+Prints one JSON line with the throughput, the roofline of the decode kernel
+of the timed steps (algorithmic int8 ops of its collision tests -- the dense
+valid-region correlation the reference's FFT evaluates -- over its measured
+duration; the same kernel at full shape, every placement row and no early
+exit, is reported beside it), the CPU oracle timed on
+this host's cores on a bounded sample of the same workload, the plain
+single-chromosome GPU path, and the wall time to a target makespan of
+surrogate-screened vs plain BRKGA on the same instance (population 1280, so
+1024 non-elite chromosomes are decoded per generation; pool of 4096 screened;
+100 generations; 10 seeds).
 """
 import argparse
 import json
@@ -22,6 +28,21 @@ sys.path.insert(0, ROOT)
 INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA peak, 5.0 POPS
 
 
+def _traffic_from_profile(config, pop):
+    """HBM bytes per launch of the decode kernel, from the committed profiler
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by what the same
+    counters read for a copy of known size), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", "decode_kernel_%s_pop%d_traffic.md" % (config, pop))
+    try:
+        with open(path) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch:"):
+                    return float(ln.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -33,8 +54,15 @@ def main():
     ap.add_argument("--roofline-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--single", action="store_true",
-                    help="also time the plain single-chromosome path (population 1 per launch)")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the plain single-chromosome path (population 1 per launch)")
+    ap.add_argument("--no-ttt", action="store_true", help="skip the time-to-target comparison")
+    ap.add_argument("--ttt-generations", type=int, default=100)
+    ap.add_argument("--ttt-seeds", type=int, default=10)
+    ap.add_argument("--ttt-out", default="", help="write the full time-to-target record (JSON) here")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB ten times (calibrates the "
+                         "profiler's HBM byte counters against a known size) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported "
@@ -44,6 +72,15 @@ def main():
 
     import numpy as np
     import torch
+    if args.copy_probe:
+        nb = args.copy_probe << 20
+        a = torch.ones(nb, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nb, "launches": 10}), flush=True)
+        return
     from brkga_fft2_amd.native import Decoder
     from brkga_fft2_amd.problem import (SYNTHETIC_CONFIGS, pack_problem, synthetic_population,
                                         synthetic_problem, thresholds_for)
@@ -95,12 +132,25 @@ def main():
     print("bench: workgroup time by phase: " +
           ", ".join("%s %.1f%%" % (k, 100 * v / tot) for k, v in st["phase_us"].items()) +
           "; longest context %.0f us, mean context %.0f us, row groups/test %.2f, MFMA %.1f TOPS issued" %
-          (st["context_us_max"] / args.steps, st["context_us_sum"] / max(1, st["contexts"]),
+          (st["context_us_max"], st["context_us_sum"] / max(1, st["contexts"]),
            st["row_groups"] / max(1, st["tests"]), st["mfma_ops"] / (st["kernel_ms"] * 1e-3) / 1e12),
           file=sys.stderr, flush=True)
     dec.close()
 
-    # ---- roofline of the decode kernel at full shape (no pruning, no early exit) ----
+    # ---- roofline of the timed kernel: algorithmic int8 ops of the tests it
+    # made (2*h*w*(H-h+1)*(W-w+1) per test, the dense valid-region correlation)
+    # over its HIP-event duration on the launch stream ----
+    kern_ms = pruned["kernel_ms"] / args.steps
+    algo_launch = pruned["algo_ops"] / args.steps
+    achieved = algo_launch / (kern_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4),
+                "traffic": _traffic_from_profile(args.config, pop),
+                "kernel": "decode_kernel<false>", "kernel_ms": round(kern_ms, 4),
+                "algo_ops_per_launch": algo_launch,
+                "issued_mfma_tops": round(pruned["mfma_ops"] / args.steps / (kern_ms * 1e-3) / 1e12, 2)}
+
+    # ---- the same kernel at full shape (every placement row, no early exit) ----
     dd = Decoder(packed, device=0, dense=True)
     dd.evaluate_device(keys[0], outs[0], stream=stream, check_overflow=False)
     torch.cuda.synchronize()
@@ -117,19 +167,18 @@ def main():
           ", ".join("%s %.1f%%" % (k, 100 * v / tot) for k, v in ds["phase_us"].items()),
           file=sys.stderr, flush=True)
     dense_ms = ds["kernel_ms"] / rs
-    achieved = ds["algo_ops"] / rs / (dense_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 2), "peak": INT8_PEAK_TOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / INT8_PEAK_TOPS, 4), "traffic": None,
-                "kernel": "decode_kernel<4,true>", "kernel_ms": round(dense_ms, 4),
-                "algo_ops_per_launch": ds["algo_ops"] / rs,
-                "mfma_ops_per_launch": ds["mfma_ops"] / rs}
-    print("bench: dense decode %.3f ms/launch, %.1f TOPS algorithmic (%.1f%% of int8 peak), "
-          "issued MFMA %.1f TOPS" % (dense_ms, achieved, 100 * achieved / INT8_PEAK_TOPS,
-                                      ds["mfma_ops"] / rs / (dense_ms * 1e-3) / 1e12),
-          file=sys.stderr, flush=True)
+    dense_achieved = ds["algo_ops"] / rs / (dense_ms * 1e-3) / 1e12
+    roofline["dense"] = {"kernel": "decode_kernel<true>", "kernel_ms": round(dense_ms, 4),
+                         "achieved": round(dense_achieved, 2),
+                         "frac": round(dense_achieved / INT8_PEAK_TOPS, 4),
+                         "issued_mfma_tops": round(ds["mfma_ops"] / rs / (dense_ms * 1e-3) / 1e12, 2)}
+    print("bench: decode kernel %.3f ms/launch, %.1f TOPS algorithmic (%.1f%% of int8 peak); dense "
+          "%.3f ms/launch, %.1f TOPS algorithmic, issued MFMA %.1f TOPS" %
+          (kern_ms, achieved, 100 * achieved / INT8_PEAK_TOPS, dense_ms, dense_achieved,
+           roofline["dense"]["issued_mfma_tops"]), file=sys.stderr, flush=True)
 
     single = None
-    if args.single:
+    if not args.no_single:
         ds1 = Decoder(packed, device=0)
         one = [k[:1].contiguous() for k in keys[:8]]
         o1 = torch.empty(1, dtype=torch.float64, device=dev)
@@ -162,8 +211,30 @@ def main():
                "sample": "%d chromosomes (the first %d timed steps' populations), same instance, "
                          "C oracle with %d OpenMP threads, %.1f s" % (done, done // pop, cores, cpu_t)}
 
+    # ---- wall time to a target makespan: surrogate-screened vs plain BRKGA ----
+    ttt = None
+    if not args.no_ttt:
+        from brkga_fft2_amd.surrogate import time_to_target
+        dt = Decoder(packed, device=0)
+        rec = time_to_target(dt, prob, prob.instance_parts, generations=args.ttt_generations,
+                             seeds=args.ttt_seeds)
+        dt.close()
+        if args.ttt_out:
+            with open(args.ttt_out, "w") as f:
+                json.dump(rec, f)
+        rec.pop("runs")
+        ttt = dict(workload="%s instance, population 1280 (1024 non-elite decoded per generation), "
+                            "pool 4096 screened, %d generations, %d seeds; target = median over seeds "
+                            "0-%d of plain's best after %d generations" %
+                            (args.config, rec["generations"], rec["seeds"], rec["target_seeds"] - 1,
+                             rec["generations"] // 2), **rec)
+        print("bench: time to target %.5f: plain %s s (%d/%d reached), surrogate %s s (%d/%d), "
+              "speedup %s" % (rec["target"], rec["plain"]["median_s"], rec["plain"]["reached"], rec["seeds"],
+                              rec["surrogate"]["median_s"], rec["surrogate"]["reached"], rec["seeds"],
+                              rec["speedup"]), file=sys.stderr, flush=True)
+
     line = {
-        "metric": "decoded chromosomes/s (exact placements)",
+        "metric": "decoded chromosomes/s (exact placements); wall time to target makespan, 1 GPU",
         "value": round(value, 2),
         "unit": "chromosomes/s",
         "n_gpus": 1,
@@ -188,6 +259,7 @@ def main():
     if single is not None:
         line["single_chromosome_per_s"] = round(single, 2)
         line["batched_over_single"] = round(value / single, 2)
+    line["time_to_target"] = ttt
     print(json.dumps(line), flush=True)
 
 

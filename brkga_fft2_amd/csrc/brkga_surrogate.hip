@@ -37,46 +37,92 @@ __device__ __forceinline__ bf16_t f2bf(float f) {
 #define GEMM_TILE 64
 #define GEMM_KC 32
 #define GEMM_LDS_K 40   // padded row length (bf16) of the LDS tiles
+#define GEMM_SPLIT_MAX 8
 
 enum { EPI_RELU = 1, EPI_MASK = 2 };
 
+// One thread's 8-element share (16 bytes) of a 64-row x 32-k operand tile.
+// KCONT: element (r, k) sits at P[r*ld + k] (k contiguous); otherwise at
+// P[k*ld + r] (rows contiguous).  Rows >= rows and k >= kend read as zero; the
+// host guarantees whole vectors (K % 8 == 0 for KCONT, rows % 8 == 0 otherwise,
+// ld % 8 == 0), so a vector is either wholly inside or wholly outside.
+template <bool KCONT>
+__device__ __forceinline__ uint4 stage_load(const bf16_t *P, int ld, int rows, int kend, int row0, int k0,
+                                            int tid) {
+    uint4 v = {0u, 0u, 0u, 0u};
+    if (KCONT) {
+        const int r = tid >> 2, kk = (tid & 3) << 3;
+        if (row0 + r < rows && k0 + kk < kend) v = *(const uint4 *)(P + (long long)(row0 + r) * ld + k0 + kk);
+    } else {
+        const int kk = tid >> 3, r = (tid & 7) << 3;
+        if (k0 + kk < kend && row0 + r < rows) v = *(const uint4 *)(P + (long long)(k0 + kk) * ld + row0 + r);
+    }
+    return v;
+}
+
+// Store the share into the LDS tile S[row][k] (k contiguous, the MFMA operand order).
+template <bool KCONT>
+__device__ __forceinline__ void stage_store(bf16_t (*S)[GEMM_LDS_K], uint4 v, int tid) {
+    if (KCONT) {
+        *(uint4 *)&S[tid >> 2][(tid & 3) << 3] = v;
+    } else {
+        const int kk = tid >> 3, r = (tid & 7) << 3;
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            S[r + 2 * i][kk] = (bf16_t)(w[i] & 0xFFFFu);
+            S[r + 2 * i + 1][kk] = (bf16_t)(w[i] >> 16);
+        }
+    }
+}
+
 // C[M][N] = op(A) op(B) (+ bias[n]) (ReLU) (* [mask[m][n] > 0]); f32 and/or bf16 out.
 // op(A)(m,k) = TA ? A[k*lda + m] : A[m*lda + k];  op(B)(k,n) = TB ? B[n*ldb + k] : B[k*ldb + n].
-// One workgroup = 4 waves = one 64x64 tile (2x2 of 32x32 MFMA tiles).
+// One workgroup = 4 waves = one 64x64 tile (2x2 of 32x32 MFMA tiles) over the k
+// range [z*kper, (z+1)*kper) of its blockIdx.z.  With ws set, the raw partial
+// product goes to ws[z][M][N] and splitk_reduce_kernel finishes it; the next
+// k chunk is loaded into registers while the current one is multiplied.
 template <bool TA, bool TB>
-__global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, const bf16_t *A, int lda,
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, int kper, const bf16_t *A, int lda,
                                                         const bf16_t *B, int ldb, const float *bias,
                                                         const bf16_t *mask, int ldm, int flags,
-                                                        float *Cf, bf16_t *Cb, int ldc) {
+                                                        float *Cf, bf16_t *Cb, int ldc, float *ws) {
     __shared__ __align__(16) bf16_t As[GEMM_TILE][GEMM_LDS_K];
     __shared__ __align__(16) bf16_t Bs[GEMM_TILE][GEMM_LDS_K];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.y * GEMM_TILE, n0 = blockIdx.x * GEMM_TILE;
     const int wm = (wave >> 1) << 5, wn = (wave & 1) << 5;
+    const int kb = blockIdx.z * kper, ke = min(K, kb + kper);
     f32x16 acc = {};
-    for (int k0 = 0; k0 < K; k0 += GEMM_KC) {
-        // stage 64 x 32 of op(A) and 32 x 64 of op(B), both with k contiguous
-        for (int e = tid; e < GEMM_TILE * GEMM_KC; e += 256) {
-            int r, kk;
-            if (TA) { kk = e / GEMM_TILE; r = e % GEMM_TILE; } else { r = e / GEMM_KC; kk = e % GEMM_KC; }
-            const int m = m0 + r, k = k0 + kk;
-            As[r][kk] = (m < M && k < K) ? (TA ? A[(long long)k * lda + m] : A[(long long)m * lda + k]) : (bf16_t)0;
-            int c, kb;
-            if (TB) { c = e / GEMM_KC; kb = e % GEMM_KC; } else { kb = e / GEMM_TILE; c = e % GEMM_TILE; }
-            const int n = n0 + c, k2 = k0 + kb;
-            Bs[c][kb] = (n < N && k2 < K) ? (TB ? B[(long long)n * ldb + k2] : B[(long long)k2 * ldb + n]) : (bf16_t)0;
-        }
+    uint4 ra = stage_load<!TA>(A, lda, M, ke, m0, kb, tid);
+    uint4 rb = stage_load<TB>(B, ldb, N, ke, n0, kb, tid);
+    for (int k0 = kb; k0 < ke; k0 += GEMM_KC) {
         __syncthreads();
+        stage_store<!TA>(As, ra, tid);
+        stage_store<TB>(Bs, rb, tid);
+        __syncthreads();
+        if (k0 + GEMM_KC < ke) {
+            ra = stage_load<!TA>(A, lda, M, ke, m0, k0 + GEMM_KC, tid);
+            rb = stage_load<TB>(B, ldb, N, ke, n0, k0 + GEMM_KC, tid);
+        }
 #pragma unroll
         for (int ks = 0; ks < GEMM_KC; ks += 16) {
             const bf16x8 a = *(const bf16x8 *)&As[wm + (lane & 31)][ks + ((lane >> 5) << 3)];
             const bf16x8 b = *(const bf16x8 *)&Bs[wn + (lane & 31)][ks + ((lane >> 5) << 3)];
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
         }
-        __syncthreads();
     }
     const int n = n0 + wn + (lane & 31);
     if (n >= N) return;
+    if (ws) {
+        float *wz = ws + (long long)blockIdx.z * M * N;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int m = m0 + wm + ((i >> 2) << 3) + ((lane >> 5) << 2) + (i & 3);
+            if (m < M) wz[(long long)m * N + n] = acc[i];
+        }
+        return;
+    }
     const float bn = bias ? bias[n] : 0.0f;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
@@ -88,6 +134,15 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
         if (Cf) Cf[(long long)m * ldc + n] = v;
         if (Cb) Cb[(long long)m * ldc + n] = f2bf(v);
     }
+}
+
+// C[m][n] = sum over z of ws[z][m][n], in z order (deterministic split-k).
+__global__ void splitk_reduce_kernel(const float *ws, int S, int M, int N, float *C, int ldc) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)M * N) return;
+    float v = 0.0f;
+    for (int z = 0; z < S; z++) v += ws[(long long)z * M * N + t];
+    C[(t / N) * ldc + t % N] = v;
 }
 
 // Features of each chromosome (one workgroup per row): keys, then area and
@@ -153,11 +208,33 @@ __global__ void dh2_kernel(const float *dy, const float *w3, const bf16_t *H2, i
 }
 
 // g[j] (+)= sum_b M[b][j] * (w ? w[b] : 1)
-__global__ void colsum_kernel(const bf16_t *Mx, int B, int Hd, const float *w, float *g) {
+// Column sums g[j] = sum_b Mx[b][j] * (w ? w[b] : 1) in two fixed-order stages:
+// partial sums over 64-row blocks (one wave per 16 rows, 64 columns per
+// workgroup), then the row blocks in order.
+#define COLSUM_ROWS 64
+__global__ __launch_bounds__(256) void colsum_partial_kernel(const bf16_t *Mx, int B, int Hd, const float *w,
+                                                             float *part) {
+    __shared__ float sh[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + lane, rb = blockIdx.y;
+    float s = 0.0f;
+    if (j < Hd) {
+#pragma unroll
+        for (int i = 0; i < COLSUM_ROWS / 4; i++) {
+            const int b = rb * COLSUM_ROWS + wv * (COLSUM_ROWS / 4) + i;
+            if (b < B) s += bf2f(Mx[(long long)b * Hd + j]) * (w ? w[b] : 1.0f);
+        }
+    }
+    sh[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0 && j < Hd) part[(long long)rb * Hd + j] = (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]);
+}
+
+__global__ void colsum_final_kernel(const float *part, int R, int Hd, float *g) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= Hd) return;
     float s = 0.0f;
-    for (int b = 0; b < B; b++) s += bf2f(Mx[(long long)b * Hd + j]) * (w ? w[b] : 1.0f);
+    for (int r = 0; r < R; r++) s += part[(long long)r * Hd + j];
     g[j] = s;
 }
 
@@ -208,6 +285,8 @@ struct brkga_mlp {
     long long cap = 0;
     bf16_t *x = nullptr, *H1 = nullptr, *H2 = nullptr, *dH2 = nullptr, *dH1 = nullptr;
     float *y = nullptr, *dy = nullptr, *loss = nullptr;
+    float *part = nullptr;   // column-sum partials [cap / 64][Hd]
+    float *ws = nullptr;     // split-k partials [GEMM_SPLIT_MAX][max(Fp, Hd)][Hd]
     long long step = 0;
     float lr = 1e-3f, wd = 0.0f;
     std::vector<void *> allocs;
@@ -234,7 +313,7 @@ static int mlp_alloc(brkga_mlp *net, size_t bytes, void **q) {
 static int mlp_reserve(brkga_mlp *net, long long B) {
     if (B <= net->cap) return 0;
     for (void *q : {(void *)net->x, (void *)net->H1, (void *)net->H2, (void *)net->dH2, (void *)net->dH1,
-                    (void *)net->y, (void *)net->dy}) {
+                    (void *)net->y, (void *)net->dy, (void *)net->part}) {
         if (!q) continue;
         auto it = std::find(net->allocs.begin(), net->allocs.end(), q);
         if (it != net->allocs.end()) { (void)hipFree(q); net->allocs.erase(it); }
@@ -248,51 +327,87 @@ static int mlp_reserve(brkga_mlp *net, long long B) {
     if ((rc = mlp_alloc(net, (size_t)B * net->Hd * 2, &q))) return rc; net->dH1 = (bf16_t *)q;
     if ((rc = mlp_alloc(net, (size_t)B * 4, &q))) return rc; net->y = (float *)q;
     if ((rc = mlp_alloc(net, (size_t)B * 4, &q))) return rc; net->dy = (float *)q;
+    if ((rc = mlp_alloc(net, (size_t)((B + COLSUM_ROWS - 1) / COLSUM_ROWS) * net->Hd * 4, &q))) return rc;
+    net->part = (float *)q;
     net->cap = B;
     return 0;
 }
 
+// Shapes the staging vectors need (see stage_load); every call here meets them.
 template <bool TA, bool TB>
-static void gemm(hipStream_t s, int M, int N, int K, const bf16_t *A, int lda, const bf16_t *B, int ldb,
-                 const float *bias, const bf16_t *mask, int ldm, int flags, float *Cf, bf16_t *Cb, int ldc) {
-    dim3 grid((N + GEMM_TILE - 1) / GEMM_TILE, (M + GEMM_TILE - 1) / GEMM_TILE);
-    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, bias, mask,
-                       ldm, flags, Cf, Cb, ldc);
+static bool gemm_shape_ok(int M, int N, int K, int lda, int ldb) {
+    if (lda % 8 || ldb % 8) return false;
+    if (!TA && K % 8) return false;
+    if (TA && M % 8) return false;
+    if (TB && K % 8) return false;
+    if (!TB && N % 8) return false;
+    return true;
 }
 
-static void forward(brkga_mlp *net, long long B, hipStream_t s) {
+template <bool TA, bool TB>
+static int gemm(hipStream_t s, int M, int N, int K, const bf16_t *A, int lda, const bf16_t *B, int ldb,
+                const float *bias, const bf16_t *mask, int ldm, int flags, float *Cf, bf16_t *Cb, int ldc,
+                float *ws = nullptr) {
+    if (!gemm_shape_ok<TA, TB>(M, N, K, lda, ldb)) { g_mlp_error = "gemm shape"; return BRKGA_EINVAL; }
+    const int tiles = ((N + GEMM_TILE - 1) / GEMM_TILE) * ((M + GEMM_TILE - 1) / GEMM_TILE);
+    int S = 1;
+    if (ws && Cf && !Cb && !bias && !flags) {   // split k over the 256 CUs (weight gradients)
+        S = std::max(1, std::min(GEMM_SPLIT_MAX, 256 / tiles));
+        S = std::min(S, std::max(1, K / 128));
+    }
+    const int kper = ((K + S - 1) / S + GEMM_KC - 1) / GEMM_KC * GEMM_KC;
+    S = (K + kper - 1) / kper;
+    dim3 grid((N + GEMM_TILE - 1) / GEMM_TILE, (M + GEMM_TILE - 1) / GEMM_TILE, S);
+    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB>), grid, dim3(256), 0, s, M, N, K, kper, A, lda, B, ldb, bias,
+                       mask, ldm, flags, Cf, Cb, ldc, S > 1 ? ws : nullptr);
+    if (S > 1)
+        hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)(((long long)M * N + 255) / 256)), dim3(256), 0, s,
+                           ws, S, M, N, Cf, ldc);
+    return 0;
+}
+
+static void colsum(brkga_mlp *net, hipStream_t s, const bf16_t *Mx, int B, const float *w, float *g) {
+    const int Hd = net->Hd, R = (B + COLSUM_ROWS - 1) / COLSUM_ROWS;
+    hipLaunchKernelGGL(colsum_partial_kernel, dim3((Hd + 63) / 64, R), dim3(256), 0, s, Mx, B, Hd, w, net->part);
+    hipLaunchKernelGGL(colsum_final_kernel, dim3((Hd + 255) / 256), dim3(256), 0, s, net->part, R, Hd, g);
+}
+
+static int forward(brkga_mlp *net, long long B, hipStream_t s) {
     const int Hd = net->Hd;
-    gemm<false, false>(s, (int)B, Hd, net->Fp, net->x, net->Fp, net->pb + net->oW1, Hd, net->p + net->ob1,
-                       nullptr, 0, EPI_RELU, nullptr, net->H1, Hd);
-    gemm<false, false>(s, (int)B, Hd, Hd, net->H1, Hd, net->pb + net->oW2, Hd, net->p + net->ob2, nullptr, 0,
-                       EPI_RELU, nullptr, net->H2, Hd);
+    int rc;
+    if ((rc = gemm<false, false>(s, (int)B, Hd, net->Fp, net->x, net->Fp, net->pb + net->oW1, Hd,
+                                 net->p + net->ob1, nullptr, 0, EPI_RELU, nullptr, net->H1, Hd)) ||
+        (rc = gemm<false, false>(s, (int)B, Hd, Hd, net->H1, Hd, net->pb + net->oW2, Hd, net->p + net->ob2,
+                                 nullptr, 0, EPI_RELU, nullptr, net->H2, Hd)))
+        return rc;
     hipLaunchKernelGGL(output_kernel, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, s, net->H2, (int)B, Hd,
                        net->p + net->ow3, net->p + net->ob3, net->y);
+    return 0;
 }
 
 // gradients of the mean squared error into net->g (the inputs are already in net->x)
-static void backward(brkga_mlp *net, const float *t, long long B, hipStream_t s) {
+static int backward(brkga_mlp *net, const float *t, long long B, hipStream_t s) {
     const int Hd = net->Hd;
+    int rc;
     hipLaunchKernelGGL(loss_grad_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, net->y, t, (int)B,
                        net->dy, net->loss);
-    hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, s, net->H2, (int)B, Hd, net->dy,
-                       net->g + net->ow3);
+    colsum(net, s, net->H2, (int)B, net->dy, net->g + net->ow3);                  // dw3 = H2^T dy
     hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, net->dy, (int)B, net->g + net->ob3);
     hipLaunchKernelGGL(dh2_kernel, dim3((unsigned)((B * Hd + 255) / 256)), dim3(256), 0, s, net->dy,
                        net->p + net->ow3, net->H2, (int)B, Hd, net->dH2);
-    hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, s, net->dH2, (int)B, Hd, nullptr,
-                       net->g + net->ob2);
+    colsum(net, s, net->dH2, (int)B, nullptr, net->g + net->ob2);                 // db2
     // dW2 = H1^T dH2
-    gemm<true, false>(s, Hd, Hd, (int)B, net->H1, Hd, net->dH2, Hd, nullptr, nullptr, 0, 0, net->g + net->oW2,
-                      nullptr, Hd);
+    if ((rc = gemm<true, false>(s, Hd, Hd, (int)B, net->H1, Hd, net->dH2, Hd, nullptr, nullptr, 0, 0,
+                                net->g + net->oW2, nullptr, Hd, net->ws)))
+        return rc;
     // dH1 = (dH2 W2^T) * [H1 > 0]
-    gemm<false, true>(s, (int)B, Hd, Hd, net->dH2, Hd, net->pb + net->oW2, Hd, nullptr, net->H1, Hd, EPI_MASK,
-                      nullptr, net->dH1, Hd);
-    hipLaunchKernelGGL(colsum_kernel, dim3((Hd + 255) / 256), dim3(256), 0, s, net->dH1, (int)B, Hd, nullptr,
-                       net->g + net->ob1);
+    if ((rc = gemm<false, true>(s, (int)B, Hd, Hd, net->dH2, Hd, net->pb + net->oW2, Hd, nullptr, net->H1, Hd,
+                                EPI_MASK, nullptr, net->dH1, Hd)))
+        return rc;
+    colsum(net, s, net->dH1, (int)B, nullptr, net->g + net->ob1);                 // db1
     // dW1 = X^T dH1
-    gemm<true, false>(s, net->Fp, Hd, (int)B, net->x, net->Fp, net->dH1, Hd, nullptr, nullptr, 0, 0,
-                      net->g + net->oW1, nullptr, Hd);
+    return gemm<true, false>(s, net->Fp, Hd, (int)B, net->x, net->Fp, net->dH1, Hd, nullptr, nullptr, 0, 0,
+                             net->g + net->oW1, nullptr, Hd, net->ws);
 }
 
 extern "C" int brkga_mlp_create(int32_t n_parts, int32_t n_machines, const double *thresholds,
@@ -346,6 +461,8 @@ extern "C" int brkga_mlp_create(int32_t n_parts, int32_t n_machines, const doubl
     if ((rc = mlp_alloc(net, (size_t)n_parts * 4, &q))) return bail(rc); net->slot_area = (float *)q;
     if ((rc = mlp_alloc(net, (size_t)n_parts * 4, &q))) return bail(rc); net->slot_height = (float *)q;
     if ((rc = mlp_alloc(net, 16, &q))) return bail(rc); net->loss = (float *)q;
+    if ((rc = mlp_alloc(net, (size_t)GEMM_SPLIT_MAX * std::max(Fp, Hd) * Hd * 4, &q))) return bail(rc);
+    net->ws = (float *)q;
     std::vector<double> thr((size_t)std::max(n_machines - 1, 1), 0.0);
     for (int m = 0; m + 1 < n_machines; m++) thr[(size_t)m] = thresholds[m];
     if ((rc = mlp_alloc(net, thr.size() * 8, &q))) return bail(rc); net->thr = (double *)q;
@@ -380,8 +497,7 @@ extern "C" int brkga_mlp_predict(brkga_mlp *net, const float *d_keys, int64_t B,
     MLPCHK(hipSetDevice(net->device));
     hipStream_t s = (hipStream_t)stream;
     int rc = mlp_features(net, d_keys, B, s);
-    if (rc) return rc;
-    forward(net, B, s);
+    if (rc || (rc = forward(net, B, s))) return rc;
     MLPCHK(hipMemcpyAsync(d_pred, net->y, (size_t)B * 4, hipMemcpyDeviceToDevice, s));
     MLPCHK(hipGetLastError());
     return 0;
@@ -393,10 +509,9 @@ extern "C" int brkga_mlp_gradients(brkga_mlp *net, const float *d_keys, const fl
     MLPCHK(hipSetDevice(net->device));
     hipStream_t s = (hipStream_t)stream;
     int rc = mlp_features(net, d_keys, B, s);
-    if (rc) return rc;
-    forward(net, B, s);
+    if (rc || (rc = forward(net, B, s))) return rc;
     MLPCHK(hipMemsetAsync(net->loss, 0, 4, s));
-    backward(net, d_target, B, s);
+    if ((rc = backward(net, d_target, B, s))) return rc;
     if (d_grad) MLPCHK(hipMemcpyAsync(d_grad, net->g, (size_t)net->np_ * 4, hipMemcpyDeviceToDevice, s));
     MLPCHK(hipGetLastError());
     return 0;

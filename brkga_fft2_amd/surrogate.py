@@ -125,14 +125,16 @@ class MLPSurrogate:
                                                   g.data_ptr(), _stream_ptr(s)), "brkga_mlp_gradients")
         return g
 
-    def train_step(self, keys, targets, stream=None):
+    def train_step(self, keys, targets, stream=None, want_loss=True):
+        """One Adam step; returns the batch loss (a host sync) or None."""
         loss = ctypes.c_float()
         s = self._stream(stream)
         self._check(self._lib.brkga_mlp_train_step(self._h, keys.contiguous().data_ptr(),
                                                    targets.contiguous().data_ptr(), keys.shape[0],
-                                                   ctypes.byref(loss), _stream_ptr(s)),
+                                                   ctypes.byref(loss) if want_loss else None,
+                                                   _stream_ptr(s)),
                     "brkga_mlp_train_step")
-        return loss.value
+        return loss.value if want_loss else None
 
     def features(self, keys, stream=None):
         import torch
@@ -175,10 +177,10 @@ def spearman(a, b):
 class SurrogateBRKGA:
     """Device-resident BRKGA with optional surrogate screening (see module doc)."""
 
-    def __init__(self, decoder, problem, instance_parts, num_individuals=1024, elite_frac=0.2,
+    def __init__(self, decoder, problem, instance_parts, num_individuals=1280, elite_frac=0.2,
                  mutant_frac=0.15, elite_prob=0.7, pool=4096, warmup=5, seed=0,
-                 use_surrogate=True, hidden=512, lr=1e-3, train_steps=8, batch=1024,
-                 buffer=16384, fallback_rho=0.2, audit_every=0):
+                 use_surrogate=True, hidden=512, lr=1e-3, train_steps=16, batch=1024,
+                 buffer=16384, fallback_rho=0.2, audit_every=0, target="rank"):
         import torch
         self.torch = torch
         self.dec = decoder
@@ -198,6 +200,9 @@ class SurrogateBRKGA:
         self.train_steps, self.batch, self.buffer = int(train_steps), int(batch), int(buffer)
         self.fallback_rho = float(fallback_rho)
         self.audit_every = int(audit_every)
+        if target not in ("rank", "zscore"):
+            raise ValueError("target must be 'rank' or 'zscore'")
+        self.target = target
         self.dev = torch.device("cuda", decoder.device)
         self.stream = torch.cuda.current_stream(self.dev)
         self._lib = _lib()
@@ -216,7 +221,7 @@ class SurrogateBRKGA:
         self.gen = 0
         self.rng = np.random.default_rng(self.seed + 1)
         self.log = {"min": [], "best": [], "time": [], "spearman": [], "screened": [],
-                    "loss": [], "regret": [], "audit_spearman": []}
+                    "loss": [], "regret": [], "audit_spearman": [], "recall": []}
         self.decoder_calls = 0
         self._decode(self.pop, self.fit)
         self._add_training(self.pop, self.fit)
@@ -245,16 +250,23 @@ class SurrogateBRKGA:
         if n < 32:
             return float("nan")
         y = self.buf_y[:n]
-        mu, sd = y.mean(), y.std().clamp_min(1e-12)
-        self.y_mu, self.y_sd = mu, sd
-        losses = []
-        for _ in range(self.train_steps):
-            b = min(self.batch, n)
-            idx = torch.from_numpy(self.rng.integers(0, n, size=b)).to(self.dev)
+        if self.target == "rank":
+            # normal scores of the ranks: equal resolution for every quantile,
+            # so the near-best region is not squeezed by bad outliers
+            r = torch.empty(n, dtype=torch.float64, device=self.dev)
+            r[torch.argsort(y, stable=True)] = torch.arange(n, dtype=torch.float64, device=self.dev)
+            tt = (math.sqrt(2.0) * torch.erfinv(2.0 * (r + 0.5) / n - 1.0)).float()
+        else:
+            tt = ((y - y.mean()) / y.std().clamp_min(1e-12)).float()
+        b = min(self.batch, n)
+        idx_all = torch.from_numpy(self.rng.integers(0, n, size=(self.train_steps, b))).to(self.dev)
+        loss = float("nan")
+        for k in range(self.train_steps):
+            idx = idx_all[k]
             x = self.buf_x[idx].contiguous()
-            t = ((self.buf_y[idx] - mu) / sd).float().contiguous()
-            losses.append(self.net.train_step(x, t, stream=self.stream))
-        return float(np.mean(losses))
+            t = tt[idx].contiguous()
+            loss = self.net.train_step(x, t, stream=self.stream, want_loss=(k == self.train_steps - 1))
+        return loss
 
     def step(self):
         torch = self.torch
@@ -311,6 +323,9 @@ class SurrogateBRKGA:
         f = allf.cpu().numpy()
         chosen = np.concatenate([sel_m.cpu().numpy(), sel_o.cpu().numpy()])
         self.log["regret"].append(float(f[chosen].min() - f.min()))
+        # share of the pool's true best len(chosen) that the screen picked
+        best = np.argsort(f, kind="stable")[:chosen.size]
+        self.log["recall"].append(float(np.intersect1d(best, chosen).size) / chosen.size)
         ok = f < 1e15
         self.log["audit_spearman"].append(spearman(pred.cpu().numpy()[ok], f[ok]))
 
@@ -332,3 +347,69 @@ class SurrogateBRKGA:
     def close(self):
         if self.net is not None:
             self.net.close()
+
+
+def time_to_target(decoder, problem, instance_parts, generations=100, seeds=10, target_seeds=5,
+                   audit_every=10, **kw):
+    """Wall time to a target makespan, plain vs surrogate-screened BRKGA.
+
+    Both methods share everything but the screening (``SurrogateBRKGA`` with
+    ``use_surrogate`` False / True), run ``generations`` generations for each
+    seed and log the best exact makespan after every generation.  The target
+    is the median, over seeds ``0 .. target_seeds-1``, of the best makespan
+    plain BRKGA holds after ``generations // 2`` generations.  A run's time to
+    target is the wall time (from its first generation; set-up and the initial
+    population's decode excluded) of the first generation whose best reaches
+    the target, infinite when none does.  One extra surrogate run (seed 0,
+    not timed) decodes the whole pool every ``audit_every`` screened
+    generations for the regret and pool-wide rank correlation.
+    """
+    import torch
+    w = SurrogateBRKGA(decoder, problem, instance_parts, seed=10 ** 6, warmup=1, **kw)
+    w.run(3)   # first-call set-up of every kernel, outside the timed runs
+    w.close()
+    torch.cuda.synchronize()
+    runs = {"plain": [], "surrogate": []}
+    for kind in runs:
+        for seed in range(seeds):
+            ga = SurrogateBRKGA(decoder, problem, instance_parts, seed=seed,
+                                use_surrogate=(kind == "surrogate"), **kw)
+            ga.run(generations)
+            runs[kind].append({"best": list(ga.log["best"]), "time": list(ga.log["time"]),
+                               "spearman": list(ga.log["spearman"]),
+                               "screened": list(ga.log["screened"])})
+            ga.close()
+    half = max(1, generations // 2)
+    target = float(np.median([r["best"][half - 1] for r in runs["plain"][:target_seeds]]))
+
+    def ttt(r):
+        b = np.asarray(r["best"])
+        hit = np.nonzero(b <= target)[0]
+        return float(r["time"][hit[0]]) if hit.size else float("inf")
+
+    out = {"target": target, "generations": generations, "seeds": seeds, "target_seeds": target_seeds}
+    for kind, rs in runs.items():
+        t = [ttt(r) for r in rs]
+        med = float(np.median(t))
+        out[kind] = {"median_s": med if math.isfinite(med) else None,
+                     "per_seed_s": [x if math.isfinite(x) else None for x in t],
+                     "reached": int(np.sum(np.isfinite(t))),
+                     "ms_per_generation": 1e3 * float(np.mean([r["time"][-1] / generations for r in rs])),
+                     "final_best_mean": float(np.mean([r["best"][-1] for r in rs]))}
+    sp = [x for r in runs["surrogate"] for x, sc in zip(r["spearman"], r["screened"])
+          if sc and math.isfinite(x)]
+    out["surrogate"]["spearman_decoded_mean"] = float(np.mean(sp)) if sp else None
+    a, b = out["plain"]["median_s"], out["surrogate"]["median_s"]
+    out["speedup"] = (a / b) if (a is not None and b) else None
+    if audit_every:
+        ga = SurrogateBRKGA(decoder, problem, instance_parts, seed=0, audit_every=audit_every, **kw)
+        ga.run(generations)
+        out["surrogate"]["regret_mean"] = float(np.mean(ga.log["regret"])) if ga.log["regret"] else None
+        out["surrogate"]["regret_max"] = float(np.max(ga.log["regret"])) if ga.log["regret"] else None
+        ps = [x for x in ga.log["audit_spearman"] if math.isfinite(x)]
+        out["surrogate"]["spearman_pool_mean"] = float(np.mean(ps)) if ps else None
+        out["surrogate"]["recall_mean"] = float(np.mean(ga.log["recall"])) if ga.log["recall"] else None
+        out["surrogate"]["audits"] = len(ga.log["regret"])
+        ga.close()
+    out["runs"] = runs
+    return out
