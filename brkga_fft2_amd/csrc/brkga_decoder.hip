@@ -18,8 +18,10 @@
 //   * makespan = sum over bins (pt + pth + setup), max over machines
 //     placement.py:85-89,147-156
 //
-// Wave r of the workgroup owns rotation r of the part being placed, so the
-// up-to-four rotations of a bin are tested concurrently.
+// Wave r of the workgroup runs the vacancy filter of rotation r (a bitset of
+// the top rows whose every row has a long enough free run); the collision
+// tests then run one rotation at a time on the whole workgroup, the four
+// waves splitting the mask rows and AND-combining their zero masks in LDS.
 //
 // Correlation as a GEMM.  For a group of 32 candidate top rows y_m and an
 // x-tile of 32 placements starting at x0 = 32t:
@@ -27,7 +29,9 @@
 //     A_{a,c}[m][k] = G[y_m + a][x0 + 32c + k]        (grid bytes, k < 32)
 //     B_{a,c}[k][n] = P[a][32c + k - n]               (banded Toeplitz of mask row a)
 // issued as V_MFMA_I32_32X32X32_I8 with int32 accumulation, so counts are
-// exact integers.  B is built once per rotation at create time.
+// exact integers.  A fragments are read from an LDS band of grid rows; B
+// fragments are expanded in registers from the mask row's bits (one 32-bit
+// word per 32 columns, kept per rotation at create time).
 //
 // Row pruning: a placement with top row y can only be collision-free if every
 // grid row y+a has a free run >= the mask row's longest 1-run (the vacancy
@@ -89,6 +93,11 @@ struct DevProblem {
     const int *plate_h, *plate_w;
     const double *setup, *proc_time, *pth;
     const unsigned char *fits;   // [M][n_types] rotation 0 fits the plate in some orientation
+    // Tile shapes of the collision passes per (machine, mask width class
+    // nc = ceil(w/32), clamped to TUNE_NC-1): [M][TUNE_NC][2] = {cap on x-tiles
+    // per pass, cap on x-tiles per staged LDS window}.  Any values give the
+    // same results; only the speed changes.
+    const unsigned char *tiles;
 };
 
 struct DevWork {
@@ -253,6 +262,7 @@ __device__ __forceinline__ int decode_zero(lds_u64 *s_zero, int tcnt, int xbase,
 // Tiles per pass shrink as the chunk count grows so that the A double buffer,
 // the B fragments and the accumulators fit the register budget.
 #define MAX_NC 12
+#define TUNE_NC BRKGA_TILE_CLASSES
 __host__ __device__ constexpr int pass_tiles(int nc) { return nc <= 3 ? 8 : (nc <= 5 ? 4 : (nc <= MAX_NC ? 2 : 1)); }
 
 // One pass of TP tiles: correlate, combine the waves' zero masks, decode.
@@ -275,7 +285,7 @@ __device__ __forceinline__ int one_pass(lds_cu8 *band, int stride, int rowoff, i
 template <int NC>
 __device__ int window_passes(lds_cu8 *band, int stride, int rowoff, int h,
                              lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
-                             lds_u64 *s_zero, unsigned long long &mfma) {
+                             lds_u64 *s_zero, unsigned long long &mfma, int tpcap) {
     constexpr int TPM = pass_tiles(NC);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int a0 = (h * wave) >> 2, a1 = (h * (wave + 1)) >> 2;
@@ -283,13 +293,13 @@ __device__ int window_passes(lds_cu8 *band, int stride, int rowoff, int h,
     for (int t0 = 0; t0 < tl;) {
         const int rem = tl - t0;
         int k, tp;
-        if (TPM >= 8 && rem >= 8) {
+        if (TPM >= 8 && tpcap >= 8 && rem >= 8) {
             k = one_pass<(TPM >= 8 ? 8 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zero);
             tp = 8;
-        } else if (TPM >= 4 && rem >= 4) {
+        } else if (TPM >= 4 && tpcap >= 4 && rem >= 4) {
             k = one_pass<(TPM >= 4 ? 4 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zero);
             tp = 4;
-        } else if (TPM >= 2 && rem >= 2) {
+        } else if (TPM >= 2 && tpcap >= 2 && rem >= 2) {
             k = one_pass<(TPM >= 2 ? 2 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zero);
             tp = 2;
         } else {
@@ -339,8 +349,8 @@ __device__ int window_passes_generic(lds_cu8 *band, int stride, int rowoff, int 
 
 __device__ __forceinline__ int window_dispatch(int nc, lds_cu8 *band, int stride, int rowoff, int h,
                                lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
-                               lds_u64 *s_zero, unsigned long long &mfma) {
-#define CASE(K) case K: return window_passes<K>(band, stride, rowoff, h, qbits, tl, xbase, xmax, nrows, s_zero, mfma);
+                               lds_u64 *s_zero, unsigned long long &mfma, int tpcap) {
+#define CASE(K) case K: return window_passes<K>(band, stride, rowoff, h, qbits, tl, xbase, xmax, nrows, s_zero, mfma, tpcap);
     switch (nc) {
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
         default:
@@ -384,13 +394,13 @@ template <bool DENSE>
 __device__ int test_rotation(const unsigned char *grid, int gpitch, int W, int h, int w, int nc,
                              const unsigned *qbits_g, const short *cand, int ncand, lds_u8 *band,
                              int band_bytes, lds_u32 *s_q, lds_u64 *s_zero, unsigned long long &groups,
-                             unsigned long long &mfma, unsigned long long *ph) {
+                             unsigned long long &mfma, unsigned long long *ph, int tpcap, int wcap) {
     const int tid = threadIdx.x, lane = tid & 63;
     // mask bits of the rotation -> LDS (read by every wave for every group)
     for (int k = tid; k < h * (nc + 1); k += WG_THREADS) s_q[k] = qbits_g[k];
     const int T = (W - w + 1 + 31) >> 5;
     int stride, rcap;
-    const int TW = band_window(T, nc, h, band_bytes, WINDOW_CAP, &stride, &rcap);
+    const int TW = band_window(T, nc, h, band_bytes, wcap, &stride, &rcap);
     const int maxrows = min(GROUP_ROWS, rcap - h + 1);
     int found = KEY_NONE;
     for (int g0 = 0; g0 < ncand;) {
@@ -417,7 +427,7 @@ __device__ int test_rotation(const unsigned char *grid, int gpitch, int W, int h
             unsigned long long t1 = tid == 0 ? tick() : 0;
             if (tid == 0) ph[2] += t1 - t0;
             const int k = window_dispatch(nc, band, stride, rowoff, h, s_q, tl, tw0 << 5, W - w,
-                                          nrows, s_zero, mfma);
+                                          nrows, s_zero, mfma, tpcap);
             gkey = k < gkey ? k : gkey;
             if (tid == 0) ph[3] += tick() - t1;
         }
@@ -662,9 +672,12 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                     c_tests++;
                     c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
                     const short *cand = s_cand + r * hmax;
-                    const int key = test_rotation<DENSE>(grid, Wk.pitch, W, h, w, P.rot_nc[rg],
+                    const int ncr = P.rot_nc[rg];
+                    const unsigned char *tc = P.tiles + ((long long)mach * TUNE_NC + min(ncr, TUNE_NC - 1)) * 2;
+                    const int key = test_rotation<DENSE>(grid, Wk.pitch, W, h, w, ncr,
                                                          P.qbits + P.qbits_off[rg], cand, ncand,
-                                                         band, Wk.band_bytes, s_q, (lds_u64 *)s_zero, c_groups, c_mfma, ph);
+                                                         band, Wk.band_bytes, s_q, (lds_u64 *)s_zero, c_groups, c_mfma, ph,
+                                                         uni(tc[0]), uni(tc[1]));
                     if (key == KEY_NONE) continue;
                     // density (new enclosure length) -> row -> col (binClassNew.py:101-117)
                     const int y = uni(cand[key / XKEY]), x = key % XKEY, row = y + h - 1;
@@ -822,7 +835,8 @@ struct brkga_handle {
     int n_cu = 0;
     int n_wg = 0;
     int dense = 0;
-    int tiles_per_pass = 4;
+    std::vector<unsigned char> tiles;   // host copy of DevProblem::tiles
+    unsigned char *d_tiles = nullptr;
     brkga_problem prob{};     // scalars only
     DevProblem dp{};
     DevWork dw{};
@@ -966,7 +980,11 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     if (o) opt = *o;
     h->device = opt.device;
     h->dense = opt.dense ? 1 : 0;
-    h->tiles_per_pass = opt.tiles_per_pass == 2 ? 2 : 4;
+    if (!(opt.tiles_per_pass == 0 || opt.tiles_per_pass == 1 || opt.tiles_per_pass == 2 ||
+          opt.tiles_per_pass == 4 || opt.tiles_per_pass == 8)) {
+        delete h;
+        return fail(nullptr, BRKGA_EINVAL, "tiles_per_pass must be 0, 1, 2, 4 or 8");
+    }
     h->prob = *p;
     auto bail = [&](int code) {
         std::string e = h->err;
@@ -1054,6 +1072,16 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
         (rc = upload(h, p->proc_time_height, (size_t)M * T, &d.pth)) ||
         (rc = upload(h, fits.data(), fits.size(), &d.fits)))
         return bail(rc);
+    // tile table: per-width default pass size (or the option's cap), full windows
+    h->tiles.assign((size_t)M * TUNE_NC * 2, 0);
+    for (int m = 0; m < M; m++)
+        for (int c = 0; c < TUNE_NC; c++) {
+            const int def = pass_tiles(c < 1 ? 1 : c);
+            h->tiles[((size_t)m * TUNE_NC + c) * 2] = (unsigned char)(opt.tiles_per_pass ? std::min(def, opt.tiles_per_pass) : def);
+            h->tiles[((size_t)m * TUNE_NC + c) * 2 + 1] = (unsigned char)WINDOW_CAP;
+        }
+    if ((rc = upload(h, h->tiles.data(), h->tiles.size(), &d.tiles))) return bail(rc);
+    h->d_tiles = (unsigned char *)d.tiles;
     h->prob.thresholds = nullptr;   // host pointers are not kept
     DevWork &w = h->dw;
     w.hmax = hmax;
@@ -1251,6 +1279,42 @@ extern "C" int brkga_evaluate(brkga_handle *h, const void *keys, int key_dtype, 
     if (log) HIPCHK(h, hipMemcpyAsync(log, h->d_log, lb, hipMemcpyDeviceToHost, s));
     HIPCHK(h, hipStreamSynchronize(s));
     return 0;
+}
+
+extern "C" int brkga_set_tiles(brkga_handle *h, const int32_t *tiles_per_pass, const int32_t *window_tiles) {
+    if (!h) return fail(nullptr, BRKGA_EINVAL, "handle is NULL");
+    const int M = h->prob.n_machines;
+    std::vector<unsigned char> t = h->tiles;
+    for (int m = 0; m < M; m++)
+        for (int c = 0; c < TUNE_NC; c++) {
+            const size_t i = (size_t)m * TUNE_NC + c;
+            if (tiles_per_pass) {
+                const int v = tiles_per_pass[i];
+                if (!(v == 0 || v == 1 || v == 2 || v == 4 || v == 8))
+                    return fail(h, BRKGA_EINVAL, "tiles_per_pass entries must be 0, 1, 2, 4 or 8");
+                t[i * 2] = (unsigned char)(v ? std::min(v, pass_tiles(c < 1 ? 1 : c)) : pass_tiles(c < 1 ? 1 : c));
+            }
+            if (window_tiles) {
+                const int v = window_tiles[i];
+                if (v < 0 || v > WINDOW_CAP) return fail(h, BRKGA_EINVAL, "window_tiles entries must be in [0, 64]");
+                t[i * 2 + 1] = (unsigned char)(v ? v : WINDOW_CAP);
+            }
+        }
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());   // no launch may be reading the table
+    HIPCHK(h, hipMemcpy(h->d_tiles, t.data(), t.size(), hipMemcpyHostToDevice));
+    h->tiles = t;
+    return BRKGA_OK;
+}
+
+extern "C" int brkga_get_tiles(const brkga_handle *h, int32_t *tiles_per_pass, int32_t *window_tiles) {
+    if (!h) return BRKGA_EINVAL;
+    const size_t n = (size_t)h->prob.n_machines * TUNE_NC;
+    for (size_t i = 0; i < n; i++) {
+        if (tiles_per_pass) tiles_per_pass[i] = h->tiles[i * 2];
+        if (window_tiles) window_tiles[i] = h->tiles[i * 2 + 1];
+    }
+    return BRKGA_OK;
 }
 
 extern "C" int brkga_get_stats(const brkga_handle *hc, brkga_stats *out) {

@@ -56,8 +56,11 @@ class BrkgaGaParams(ctypes.Structure):
                 ("elite_prob", ctypes.c_double), ("seed", ctypes.c_uint64)]
 
 
+TILE_CLASSES = 16   # BRKGA_TILE_CLASSES: mask width classes nc = ceil(w / 32)
+
 EXPORTED = ["brkga_create", "brkga_evaluate", "brkga_evaluate_device", "brkga_get_stats",
-            "brkga_reset_stats", "brkga_last_error", "brkga_destroy",
+            "brkga_reset_stats", "brkga_set_tiles", "brkga_get_tiles", "brkga_last_error",
+            "brkga_destroy",
             "brkga_ga_create", "brkga_ga_run", "brkga_ga_best", "brkga_ga_population",
             "brkga_ga_last_error", "brkga_ga_destroy", "brkga_sort_fitness", "brkga_breed_pool",
             "brkga_mlp_create", "brkga_mlp_predict", "brkga_mlp_gradients", "brkga_mlp_train_step",
@@ -95,6 +98,10 @@ def load_library(path=LIB_PATH):
     lib.brkga_get_stats.restype = ctypes.c_int
     lib.brkga_reset_stats.argtypes = [_P]
     lib.brkga_reset_stats.restype = ctypes.c_int
+    lib.brkga_set_tiles.argtypes = [_P, _P, _P]
+    lib.brkga_set_tiles.restype = ctypes.c_int
+    lib.brkga_get_tiles.argtypes = [_P, _P, _P]
+    lib.brkga_get_tiles.restype = ctypes.c_int
     lib.brkga_last_error.argtypes = [_P]
     lib.brkga_last_error.restype = ctypes.c_char_p
     lib.brkga_destroy.argtypes = [_P]
@@ -209,6 +216,25 @@ class Decoder:
 
     def reset_stats(self):
         self._check(self._lib.brkga_reset_stats(self._h), "brkga_reset_stats")
+
+    def get_tiles(self):
+        """Tile table ``(tiles_per_pass, window_tiles)``, int32 ``[n_machines, TILE_CLASSES]``."""
+        t = np.zeros((self.n_machines, TILE_CLASSES), np.int32)
+        w = np.zeros((self.n_machines, TILE_CLASSES), np.int32)
+        self._check(self._lib.brkga_get_tiles(self._h, t.ctypes.data, w.ctypes.data), "brkga_get_tiles")
+        return t, w
+
+    def set_tiles(self, tiles_per_pass=None, window_tiles=None):
+        """Set the tile table (either may be None = unchanged); results never change."""
+        def arr(a):
+            if a is None:
+                return None, None
+            a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.int32),
+                                                     (self.n_machines, TILE_CLASSES)))
+            return a, a.ctypes.data
+        t, tp = arr(tiles_per_pass)
+        w, wp = arr(window_tiles)
+        self._check(self._lib.brkga_set_tiles(self._h, tp, wp), "brkga_set_tiles")
 
     def close(self):
         if getattr(self, "_h", None):

@@ -78,11 +78,14 @@ typedef struct brkga_problem {
 
 typedef struct brkga_options {
     int32_t device;          /* HIP device ordinal */
-    int32_t wgs_per_cu;      /* persistent workgroups per compute unit (0 = default 2) */
-    int32_t max_bins;        /* bins per context in the workspace (0 = default) */
+    int32_t wgs_per_cu;      /* persistent workgroups per compute unit (0 = 2 when the
+                                LDS budget allows, else 1) */
+    int32_t max_bins;        /* bins per context in the workspace (0 = min(n_parts, 32));
+                                brkga_evaluate grows it when a context needs more */
     int32_t dense;           /* 1: correlation over every placement row, no early exit
                                 (same results; used to measure the kernel at full shape) */
-    int32_t tiles_per_pass;  /* correlation x-tiles per pass, 4 or 8 (0 = autotuned) */
+    int32_t tiles_per_pass;  /* cap on correlation x-tiles per pass for every class of the
+                                tile table: 1, 2, 4 or 8 (0 = per-width default) */
     int32_t reserved[3];
 } brkga_options;
 
@@ -136,6 +139,23 @@ int brkga_evaluate_device(brkga_handle *h, const void *d_keys, int key_dtype, in
 
 int brkga_get_stats(const brkga_handle *h, brkga_stats *out);
 int brkga_reset_stats(brkga_handle *h);
+
+/*
+ * Tile shapes of the collision passes (the autotuner's knobs; no counterpart
+ * in the reference, whose FFT has none).  Tables are [n_machines][BRKGA_TILE_CLASSES],
+ * indexed by machine and mask width class nc = ceil(w / 32) (wider masks share
+ * the last class):
+ *   tiles_per_pass: cap on 32-column x-tiles one MFMA pass covers, 1, 2, 4 or 8
+ *                   (0 = the width's default: 8 up to 96 columns, 4 up to 160, else 2);
+ *   window_tiles:   cap on x-tiles staged per LDS window, 1..64 (0 = 64).
+ * Every setting gives identical results.  Either table pointer may be NULL
+ * (left unchanged).  brkga_set_tiles waits for the device to be idle; it must
+ * not race with launches on the handle.  brkga_get_tiles returns the
+ * effective values.
+ */
+#define BRKGA_TILE_CLASSES 16
+int brkga_set_tiles(brkga_handle *h, const int32_t *tiles_per_pass, const int32_t *window_tiles);
+int brkga_get_tiles(const brkga_handle *h, int32_t *tiles_per_pass, int32_t *window_tiles);
 
 /* Message of the last failed call on h (or of the last failed brkga_create
  * when h is NULL).  Valid until the next call on the same handle. */

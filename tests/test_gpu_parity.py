@@ -38,19 +38,58 @@ def test_synthetic_parity(cfg, n_chrom):
     dec.close()
 
 
-@pytest.mark.parametrize("tiles", [4, 8])
-def test_dense_mode_and_tile_shapes_agree(tiles):
-    """Full-shape correlation (no row pruning, no early exit) gives identical results."""
-    _say("dense tiles=%d" % tiles)
+@pytest.mark.parametrize("dense,tiles", [(True, 8), (True, 1), (False, 2)])
+def test_dense_mode_and_tile_shapes_agree(dense, tiles):
+    """Full-shape correlation (no row pruning, no early exit) and every pass
+    size give identical results."""
+    _say("dense=%s tiles=%d" % (dense, tiles))
     prob, n, M, packed = syn_instance("C2")
     keys = syn_keys("C2", 16, n, seed=11)
     ref_ms, ref_log, _ = oracle.decode(packed, keys, with_log=True)
-    dec = _decoder(packed, dense=True, tiles_per_pass=tiles)
+    dec = _decoder(packed, dense=dense, tiles_per_pass=tiles)
+    t, w = dec.get_tiles()
+    assert t.max() <= tiles and w.max() == 64
     ms, log = dec.evaluate(keys, with_log=True)
     np.testing.assert_array_equal(ms, ref_ms)
     assert feasible_logs_equal(ref_ms, log, ref_log) is None
     st = dec.stats()
     assert st["mfma_ops"] > 0 and st["algo_ops"] > 0
+    dec.close()
+
+
+@pytest.mark.parametrize("cfg", ["C2", "C3"])
+def test_random_tile_tables_agree(cfg):
+    """Random per-(machine, width class) pass and window sizes: same placements."""
+    _say("random tile tables %s" % cfg)
+    from brkga_fft2_amd.native import TILE_CLASSES
+    prob, n, M, packed = syn_instance(cfg)
+    keys = syn_keys(cfg, 12, n, seed=5)
+    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=os.cpu_count() or 4, with_log=True)
+    dec = _decoder(packed)
+    rng = np.random.default_rng(3)
+    for trial in range(3):
+        t = rng.choice([0, 1, 2, 4, 8], size=(M, TILE_CLASSES))
+        w = rng.choice([0, 1, 3, 4, 7, 16, 64], size=(M, TILE_CLASSES))
+        dec.set_tiles(t, w)
+        ms, log = dec.evaluate(keys, with_log=True)
+        np.testing.assert_array_equal(ms, ref_ms)
+        assert feasible_logs_equal(ref_ms, log, ref_log) is None
+    dec.close()
+
+
+def test_autotune_keeps_results():
+    _say("autotune C2")
+    import torch
+    from brkga_fft2_amd import autotune
+    prob, n, M, packed = syn_instance("C2")
+    keys = syn_keys("C2", 64, n, seed=8)
+    ref_ms, _, _ = oracle.decode(packed, keys, n_threads=os.cpu_count() or 4)
+    dec = _decoder(packed)
+    dk = torch.from_numpy(keys).cuda()
+    rec = autotune.autotune(dec, packed, dk, reps=1, windows=(8, 64), cache=None)
+    assert rec["trials"] > 0 and rec["tuned_ms"] > 0
+    ms, _ = dec.evaluate(keys)
+    np.testing.assert_array_equal(ms, ref_ms)
     dec.close()
 
 
