@@ -43,6 +43,53 @@ def _traffic_from_profile(config, pop):
     return None
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _batched_vs_single(config, seed, dev, stream, pop=1024, launches=3, singles=8):
+    """Chromosomes/s of the batched persistent decode (population ``pop`` per
+    launch) and of the same decoder fed one chromosome per launch."""
+    import numpy as np
+    import torch
+    from brkga_fft2_amd.native import Decoder
+    from brkga_fft2_amd.problem import pack_problem, synthetic_population, synthetic_problem, thresholds_for
+    prob, n, M = synthetic_problem(config, seed=seed)
+    packed = pack_problem(prob, n, M, thresholds_for(M), prob.instance_parts)
+    keys = [torch.from_numpy(synthetic_population(pop, n, seed=2000 + s, dtype=np.float32)).to(dev)
+            for s in range(launches + 1)]
+    out = torch.empty(pop, dtype=torch.float64, device=dev)
+    d = Decoder(packed, device=0)
+    d.evaluate_device(keys[0], out, stream=stream, check_overflow=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in keys[1:]:
+        d.evaluate_device(k, out, stream=stream, check_overflow=False)
+    torch.cuda.synchronize()
+    batched = pop * launches / (time.perf_counter() - t0)
+    one = [keys[1][i:i + 1].contiguous() for i in range(singles + 1)]
+    o1 = torch.empty(1, dtype=torch.float64, device=dev)
+    d.evaluate_device(one[0], o1, stream=stream, check_overflow=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in one[1:]:
+        d.evaluate_device(k, o1, stream=stream, check_overflow=False)
+    torch.cuda.synchronize()
+    single = singles / (time.perf_counter() - t0)
+    d.close()
+    return {"workload": "C3: %d parts, %d machines, plates 250/300/350/400, population %d per launch"
+                        % (n, M, pop),
+            "batched_per_s": round(batched, 2), "single_per_s": round(single, 2),
+            "batched_over_single": round(batched / single, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +107,10 @@ def main():
     ap.add_argument("--ttt-generations", type=int, default=100)
     ap.add_argument("--ttt-seeds", type=int, default=10)
     ap.add_argument("--ttt-out", default="", help="write the full time-to-target record (JSON) here")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tile table instead of tuning it before the timed steps")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the C3 batched vs single-chromosome comparison")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB ten times (calibrates the "
                          "profiler's HBM byte counters against a known size) and exit")
@@ -100,6 +151,17 @@ def main():
     stream = torch.cuda.current_stream(dev)
     dec = Decoder(packed, device=0)
     torch.cuda.synchronize()
+    tuned = None
+    if not args.no_autotune:
+        from brkga_fft2_amd import autotune
+        t_def, w_def = dec.get_tiles()
+        rec = autotune.autotune(dec, packed, keys[0], reps=3)
+        tuned = {k: rec[k] for k in ("default_ms", "tuned_ms", "trials")}
+        tuned["changed_entries"] = int(np.sum(np.asarray(rec["tiles_per_pass"]) != t_def) +
+                                       np.sum(np.asarray(rec["window_tiles"]) != w_def))
+        print("bench: autotune %d trials, default %.3f ms -> tuned %.3f ms, %d table entries changed" %
+              (rec["trials"], rec["default_ms"], rec["tuned_ms"], tuned["changed_entries"]),
+              file=sys.stderr, flush=True)
 
     for s in range(args.warmup):
         dec.evaluate_device(keys[s], outs[s], stream=stream, check_overflow=False)
@@ -126,12 +188,20 @@ def main():
     torch.cuda.synchronize()
     if not torch.equal(chk, outs[-1]):
         raise RuntimeError("decode is not deterministic")
+    # host-buffer entry (keys over PCIe in, makespans out) on the same populations
+    nh = min(10, args.steps)
+    dec.evaluate(host_pops[args.warmup])
+    t0 = time.perf_counter()
+    for s in range(args.warmup, args.warmup + nh):
+        dec.evaluate(host_pops[s])
+    host_rate = pop * nh / (time.perf_counter() - t0)
     pruned = dict(tests=st["tests"], algo_ops=st["algo_ops"], mfma_ops=st["mfma_ops"],
                   kernel_ms=st["kernel_ms"])
     tot = sum(st["phase_us"].values()) or 1.0
     print("bench: workgroup time by phase: " +
           ", ".join("%s %.1f%%" % (k, 100 * v / tot) for k, v in st["phase_us"].items()) +
-          "; longest context %.0f us, mean context %.0f us, row groups/test %.2f, MFMA %.1f TOPS issued" %
+          "; longest context (max over launches) %.0f us, mean context %.0f us, row groups/test %.2f, "
+          "MFMA %.1f TOPS issued" %
           (st["context_us_max"], st["context_us_sum"] / max(1, st["contexts"]),
            st["row_groups"] / max(1, st["tests"]), st["mfma_ops"] / (st["kernel_ms"] * 1e-3) / 1e12),
           file=sys.stderr, flush=True)
@@ -206,10 +276,25 @@ def main():
         want, _, _ = oracle.decode(packed, sample[:8], n_threads=cores)
         if not np.array_equal(want, outs[args.warmup][:8].cpu().numpy()):
             raise RuntimeError("GPU makespans differ from the CPU oracle")
+        t0 = time.perf_counter()
+        done1 = 0
+        while time.perf_counter() - t0 < args.cpu_seconds / 2 and done1 < len(sample):
+            oracle.decode(packed, sample[done1:done1 + 16], n_threads=1)
+            done1 += min(16, len(sample) - done1)
+        cpu1 = done1 / (time.perf_counter() - t0)
         cpu = {"value": round(done / cpu_t, 2), "unit": "chromosomes/s", "cores": cores,
                "kind": "port",
                "sample": "%d chromosomes (the first %d timed steps' populations), same instance, "
-                         "C oracle with %d OpenMP threads, %.1f s" % (done, done // pop, cores, cpu_t)}
+                         "C oracle with %d OpenMP threads, %.1f s" % (done, done // pop, cores, cpu_t),
+               "value_1_thread": round(cpu1, 2), "cpu_model": _cpu_model()}
+
+    # ---- C3: batched persistent decode vs the single-chromosome path ----
+    c3 = None
+    if not args.no_c3:
+        c3 = _batched_vs_single("C3", args.seed, dev, stream)
+        print("bench: C3 batched %.1f chrom/s, single %.1f chrom/s, ratio %.1f" %
+              (c3["batched_per_s"], c3["single_per_s"], c3["batched_over_single"]),
+              file=sys.stderr, flush=True)
 
     # ---- wall time to a target makespan: surrogate-screened vs plain BRKGA ----
     ttt = None
@@ -259,6 +344,9 @@ def main():
     if single is not None:
         line["single_chromosome_per_s"] = round(single, 2)
         line["batched_over_single"] = round(value / single, 2)
+    line["host_buffers_per_s"] = round(host_rate, 2)
+    line["autotune"] = tuned
+    line["c3_batched_vs_single"] = c3
     line["time_to_target"] = ttt
     print(json.dumps(line), flush=True)
 
