@@ -29,9 +29,11 @@
 //     A_{a,c}[m][k] = G[y_m + a][x0 + 32c + k]        (grid bytes, k < 32)
 //     B_{a,c}[k][n] = P[a][32c + k - n]               (banded Toeplitz of mask row a)
 // issued as V_MFMA_I32_32X32X32_I8 with int32 accumulation, so counts are
-// exact integers.  A fragments are read from an LDS band of grid rows; B
-// fragments are expanded in registers from the mask row's bits (one 32-bit
-// word per 32 columns, kept per rotation at create time).
+// exact integers (B carries a positive weight of 1, 2, 4 or 8 per set mask
+// cell, which leaves "count == 0", the only test made, unchanged).  A
+// fragments are read from an LDS band of grid rows; B fragments are expanded
+// in registers from the mask row's bits (one 32-bit word per 32 columns, kept
+// per rotation at create time).
 //
 // Row pruning: a placement with top row y can only be collision-free if every
 // grid row y+a has a free run >= the mask row's longest 1-run (the vacancy
@@ -72,6 +74,8 @@ typedef LDSP unsigned long long lds_u64;
 #define GROUP_ROWS 32
 #define LDS_RESERVE 2048    // static LDS of the kernel + slack
 #define XKEY 8192           // key = m * XKEY + x ; plates must have W < XKEY
+#define ZW_WAVE (8 * 16)    // zero masks per wave and pass: up to 8 tiles x 16 accumulators
+#define ZW_PASS (4 * ZW_WAVE)
 #define KEY_NONE 0x7fffffff
 
 enum {
@@ -112,6 +116,7 @@ struct DevWork {
     int q_off;               // LDS offset of the staged mask bits
     int vc_bins;             // bins whose vacancy vector is cached in LDS
     int band_off, band_bytes;  // LDS band for staged grid rows
+    int zw_off;                // LDS zero masks: [2 passes][4 waves][8 tiles * 16] u64
 };
 
 struct LaunchArgs {
@@ -157,16 +162,19 @@ __device__ __forceinline__ int wave_min(int v) {
 // needs the 16 bytes B[16g + j][n] = P[a][32c + 16g + j - n] = Q[32c + shift + j]
 // with shift = 32 + 16g - n in [1, 48], i.e. 16 bits of the 64-bit word
 // (Q dword c+1 : Q dword c) shifted right by `shift`.  Each bit becomes one
-// byte (0 or 1): a nibble times 0x00204081 spreads its bits to the bytes'
-// low bits.
+// byte: v_perm replicates the byte holding four of the bits into all four
+// bytes and the AND keeps bit j in byte j, so a set bit becomes 1, 2, 4 or 8.
+// Only "count == 0" is ever tested and every product is non-negative (grid
+// bytes are 0 or 1), so any positive value for a set mask cell gives the same
+// decisions; counts stay below 8 * 200 * 200, far from int32 overflow.
 __device__ __forceinline__ v4i bfrag_from_bits(unsigned lo, unsigned hi, int shift) {
     const unsigned long long w = ((unsigned long long)hi << 32) | lo;
-    const unsigned b = (unsigned)(w >> shift) & 0xffffu;
+    const unsigned b = (unsigned)(w >> shift), b4 = b >> 4;
     v4i r;
-    r[0] = (int)(((b & 0xfu) * 0x00204081u) & 0x01010101u);
-    r[1] = (int)((((b >> 4) & 0xfu) * 0x00204081u) & 0x01010101u);
-    r[2] = (int)((((b >> 8) & 0xfu) * 0x00204081u) & 0x01010101u);
-    r[3] = (int)((((b >> 12) & 0xfu) * 0x00204081u) & 0x01010101u);
+    r[0] = (int)(__builtin_amdgcn_perm(b, b, 0u) & 0x08040201u);
+    r[1] = (int)(__builtin_amdgcn_perm(b4, b4, 0u) & 0x08040201u);
+    r[2] = (int)(__builtin_amdgcn_perm(b, b, 0x01010101u) & 0x08040201u);
+    r[3] = (int)(__builtin_amdgcn_perm(b4, b4, 0x01010101u) & 0x08040201u);
     return r;
 }
 
@@ -175,15 +183,35 @@ __device__ __forceinline__ v4i bfrag_from_bits(unsigned lo, unsigned hi, int shi
 // workgroup take disjoint mask-row ranges of the same tiles: every partial
 // overlap count is a sum of non-negative products, so the total is zero iff
 // every wave's partial is zero, and the waves combine their results with an
-// AND of zero masks in LDS (s_zero[t*16 + i], bit = lane) instead of adding
-// counts.  A operands come from the LDS band, B operands are built from the
+// AND of zero masks instead of adding counts: each wave stores its masks
+// (zw[wave][t*16 + i], bit = lane) with plain LDS stores and the decode ANDs
+// the four.  A operands come from the LDS band, B operands are built from the
 // mask bits; with NC (Toeplitz chunks) known at compile time each A fragment
 // feeds up to NC MFMAs and each B fragment feeds all TP tiles, and the next
 // mask row's A fragments are loaded while the current row's MFMAs run.
+// Store one wave's zero masks of `tcnt` tiles: lane i < 16 of the wave
+// collects accumulator i's ballot, then one 8-byte store per lane.
+template <int TP>
+__device__ __forceinline__ void store_zero_masks(const v16i (&acc)[TP], int tcnt, lds_u64 *zw) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int t = 0; t < TP; t++) {
+        if (t < tcnt) {
+            unsigned long long keep = 0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const unsigned long long z = __ballot(acc[t][i] == 0);
+                keep = lane == i ? z : keep;
+            }
+            if (lane < 16) zw[t * 16 + lane] = keep;
+        }
+    }
+}
+
 template <int TP, int NC>
 __device__ __forceinline__ void corr_pass(lds_cu8 *band, int stride, int rowoff, int a0,
                                           int a1, lds_cu32 *qbits, int tloc, int tcnt,
-                                          lds_u64 *s_zero) {
+                                          lds_u64 *zw) {
     constexpr int NA = TP + NC - 1;
     const int lane = threadIdx.x & 63;
     const int shift = 32 + ((lane >> 5) << 4) - (lane & 31);
@@ -220,16 +248,7 @@ __device__ __forceinline__ void corr_pass(lds_cu8 *band, int stride, int rowoff,
             for (int j = 0; j < NA; j++) A[j] = An[j];
         }
     }
-#pragma unroll
-    for (int t = 0; t < TP; t++) {
-        if (t < tcnt) {
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const unsigned long long z = __ballot(acc[t][i] == 0);
-                if (lane == 0 && z != ~0ull) __hip_atomic_fetch_and(&s_zero[t * 16 + i], z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-        }
-    }
+    store_zero_masks<TP>(acc, tcnt, zw);
 }
 
 // Bottom-most then left-most zero in the combined masks of one pass, as
@@ -237,7 +256,7 @@ __device__ __forceinline__ void corr_pass(lds_cu8 *band, int stride, int rowoff,
 // MFMA: lane holds column n = lane&31 and rows m = 8*(i/4) + 4*(lane/32) + i%4
 // of accumulator i, so row m lives in ballot word i = 4*(m/8) + m%4, half
 // (m/4)%2.  Lanes 0..31 each scan one row m; every wave computes the result.
-__device__ __forceinline__ int decode_zero(lds_u64 *s_zero, int tcnt, int xbase,
+__device__ __forceinline__ int decode_zero(lds_u64 *zp, int tcnt, int xbase,
                                            int xmax, int nrows) {
     const int lane = threadIdx.x & 63;
     int key = KEY_NONE;
@@ -246,7 +265,9 @@ __device__ __forceinline__ int decode_zero(lds_u64 *s_zero, int tcnt, int xbase,
         const int i = ((m >> 3) << 2) + (m & 3);
         const int half = ((m >> 2) & 1) << 5;
         for (int t = 0; t < tcnt; t++) {
-            unsigned bits = (unsigned)(s_zero[t * 16 + i] >> half);
+            const int e = t * 16 + i;
+            const unsigned long long z = zp[e] & zp[ZW_WAVE + e] & zp[2 * ZW_WAVE + e] & zp[3 * ZW_WAVE + e];
+            unsigned bits = (unsigned)(z >> half);
             const int lim = xmax - (xbase + (t << 5));   // last valid column offset in this tile
             if (lim < 0) break;
             if (lim < 31) bits &= (2u << lim) - 1u;
@@ -266,44 +287,46 @@ __device__ __forceinline__ int decode_zero(lds_u64 *s_zero, int tcnt, int xbase,
 __host__ __device__ constexpr int pass_tiles(int nc) { return nc <= 3 ? 8 : (nc <= 5 ? 4 : (nc <= MAX_NC ? 2 : 1)); }
 
 // One pass of TP tiles: correlate, combine the waves' zero masks, decode.
+// Passes alternate between two mask buffers, so one barrier per pass is
+// enough: a wave can only write buffer p%2 again (pass p+2) after every wave
+// has passed the barrier of pass p+1, i.e. has finished decoding pass p.
 template <int TP, int NC>
 __device__ __forceinline__ int one_pass(lds_cu8 *band, int stride, int rowoff, int a0, int a1,
                                         lds_cu32 *qbits, int t0, int xbase, int xmax, int nrows,
-                                        lds_u64 *s_zero) {
-    corr_pass<TP, NC>(band, stride, rowoff, a0, a1, qbits, t0, TP, s_zero);
+                                        lds_u64 *s_zw, int &parity) {
+    lds_u64 *zp = s_zw + parity * ZW_PASS;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    corr_pass<TP, NC>(band, stride, rowoff, a0, a1, qbits, t0, TP, zp + wave * ZW_WAVE);
     __syncthreads();
-    const int k = decode_zero(s_zero, TP, xbase + (t0 << 5), xmax, nrows);
-    __syncthreads();
-    if (threadIdx.x < TP * 16) s_zero[threadIdx.x] = ~0ull;
-    __syncthreads();
-    return k;
+    parity ^= 1;
+    return decode_zero(zp, TP, xbase + (t0 << 5), xmax, nrows);
 }
 
 // All passes over one staged x-window (tiles [0, tl) of the window), run by
 // the whole workgroup.  The window is cut into passes of 8/4/2/1 tiles (no
 // MFMA is spent on tiles past the window).  Returns the window's minimum key.
 template <int NC>
-__device__ int window_passes(lds_cu8 *band, int stride, int rowoff, int h,
+__device__ __attribute__((always_inline)) int window_passes(lds_cu8 *band, int stride, int rowoff, int h,
                              lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
-                             lds_u64 *s_zero, unsigned long long &mfma, int tpcap) {
+                             lds_u64 *s_zw, unsigned long long &mfma, int tpcap) {
     constexpr int TPM = pass_tiles(NC);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int a0 = (h * wave) >> 2, a1 = (h * (wave + 1)) >> 2;
-    int key = KEY_NONE;
+    int key = KEY_NONE, parity = 0;
     for (int t0 = 0; t0 < tl;) {
         const int rem = tl - t0;
         int k, tp;
         if (TPM >= 8 && tpcap >= 8 && rem >= 8) {
-            k = one_pass<(TPM >= 8 ? 8 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zero);
+            k = one_pass<(TPM >= 8 ? 8 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 8;
         } else if (TPM >= 4 && tpcap >= 4 && rem >= 4) {
-            k = one_pass<(TPM >= 4 ? 4 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zero);
+            k = one_pass<(TPM >= 4 ? 4 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 4;
         } else if (TPM >= 2 && tpcap >= 2 && rem >= 2) {
-            k = one_pass<(TPM >= 2 ? 2 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zero);
+            k = one_pass<(TPM >= 2 ? 2 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 2;
         } else {
-            k = one_pass<1, NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zero);
+            k = one_pass<1, NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 1;
         }
         mfma += (unsigned long long)tp * (a1 - a0) * NC;
@@ -317,44 +340,40 @@ __device__ int window_passes(lds_cu8 *band, int stride, int rowoff, int h,
 // chunk loop, one LDS fragment per MFMA.
 __device__ int window_passes_generic(lds_cu8 *band, int stride, int rowoff, int h, int nc,
                                      lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
-                                     lds_u64 *s_zero, unsigned long long &mfma) {
+                                     lds_u64 *s_zw, unsigned long long &mfma) {
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int shift = 32 + ((lane >> 5) << 4) - (lane & 31);
     const int a0 = (h * wave) >> 2, a1 = (h * (wave + 1)) >> 2;
-    int key = KEY_NONE;
+    int key = KEY_NONE, parity = 0;
     for (int t0 = 0; t0 < tl; t0++) {
-        v16i acc = (v16i){};
+        v16i acc[1] = {(v16i){}};
         lds_cu8 *base = band + rowoff * stride + (t0 << 5) + ((lane >> 5) << 4);
         for (int a = a0; a < a1; a++) {
             lds_cu8 *rp = base + a * stride;
             lds_cu32 *q = qbits + a * (nc + 1);
             for (int c = 0; c < nc; c++)
-                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(*(lds_cv4i *)(rp + (c << 5)),
-                                                            bfrag_from_bits(q[c], q[c + 1], shift), acc, 0, 0, 0);
+                acc[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(*(lds_cv4i *)(rp + (c << 5)),
+                                                               bfrag_from_bits(q[c], q[c + 1], shift), acc[0], 0, 0, 0);
         }
         mfma += (unsigned long long)(a1 - a0) * nc;
-        for (int i = 0; i < 16; i++) {
-            const unsigned long long z = __ballot(acc[i] == 0);
-            if (lane == 0 && z != ~0ull) __hip_atomic_fetch_and(&s_zero[i], z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
+        lds_u64 *zp = s_zw + parity * ZW_PASS;
+        store_zero_masks<1>(acc, 1, zp + wave * ZW_WAVE);
         __syncthreads();
-        const int k = decode_zero(s_zero, 1, xbase + (t0 << 5), xmax, nrows);
+        parity ^= 1;
+        const int k = decode_zero(zp, 1, xbase + (t0 << 5), xmax, nrows);
         key = k < key ? k : key;
-        __syncthreads();
-        if (threadIdx.x < 16) s_zero[threadIdx.x] = ~0ull;
-        __syncthreads();
     }
     return key;
 }
 
 __device__ __forceinline__ int window_dispatch(int nc, lds_cu8 *band, int stride, int rowoff, int h,
                                lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
-                               lds_u64 *s_zero, unsigned long long &mfma, int tpcap) {
-#define CASE(K) case K: return window_passes<K>(band, stride, rowoff, h, qbits, tl, xbase, xmax, nrows, s_zero, mfma, tpcap);
+                               lds_u64 *s_zw, unsigned long long &mfma, int tpcap) {
+#define CASE(K) case K: return window_passes<K>(band, stride, rowoff, h, qbits, tl, xbase, xmax, nrows, s_zw, mfma, tpcap);
     switch (nc) {
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
         default:
-            return window_passes_generic(band, stride, rowoff, h, nc, qbits, tl, xbase, xmax, nrows, s_zero, mfma);
+            return window_passes_generic(band, stride, rowoff, h, nc, qbits, tl, xbase, xmax, nrows, s_zw, mfma);
     }
 #undef CASE
 }
@@ -391,9 +410,9 @@ __host__ __device__ __forceinline__ int band_window(int T, int nc, int h, int ba
 // left-most, placement with zero overlap, or KEY_NONE.  Uniform across the
 // workgroup.
 template <bool DENSE>
-__device__ int test_rotation(const unsigned char *grid, int gpitch, int W, int h, int w, int nc,
+__device__ __attribute__((always_inline)) int test_rotation(const unsigned char *grid, int gpitch, int W, int h, int w, int nc,
                              const unsigned *qbits_g, const short *cand, int ncand, lds_u8 *band,
-                             int band_bytes, lds_u32 *s_q, lds_u64 *s_zero, unsigned long long &groups,
+                             int band_bytes, lds_u32 *s_q, lds_u64 *s_zw, unsigned long long &groups,
                              unsigned long long &mfma, unsigned long long *ph, int tpcap, int wcap) {
     const int tid = threadIdx.x, lane = tid & 63;
     // mask bits of the rotation -> LDS (read by every wave for every group)
@@ -405,8 +424,13 @@ __device__ int test_rotation(const unsigned char *grid, int gpitch, int W, int h
     int found = KEY_NONE;
     for (int g0 = 0; g0 < ncand;) {
         const int ytop = uni(cand[g0]);
-        int g1 = g0 + 1;
-        while (g1 < ncand && g1 - g0 < maxrows && ytop + h - uni(cand[g1]) <= rcap) g1++;
+        // the group: candidate g0 and the following ones (at most maxrows in
+        // all) whose rows still fit the band; candidates descend, so these
+        // form a prefix and one ballot counts them
+        const int kk = g0 + lane;
+        const bool take = lane == 0 ||
+                          (lane < maxrows && kk < ncand && ytop + h - cand[min(kk, ncand - 1)] <= rcap);
+        const int g1 = g0 + (int)__popcll(__ballot(take) & 0xffffffffull);
         const int ylo = uni(cand[g1 - 1]);
         const int rows = ytop + h - ylo;
         const int nrows = g1 - g0;
@@ -427,7 +451,7 @@ __device__ int test_rotation(const unsigned char *grid, int gpitch, int W, int h
             unsigned long long t1 = tid == 0 ? tick() : 0;
             if (tid == 0) ph[2] += t1 - t0;
             const int k = window_dispatch(nc, band, stride, rowoff, h, s_q, tl, tw0 << 5, W - w,
-                                          nrows, s_zero, mfma, tpcap);
+                                          nrows, s_zw, mfma, tpcap);
             gkey = k < gkey ? k : gkey;
             if (tid == 0) ph[3] += tick() - t1;
         }
@@ -509,9 +533,9 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
     short *s_vcache = s_vac + hmax;                         // [vc_bins][hmax] vacancy of bins < vc_bins
     lds_u32 *s_q = (lds_u32 *)(smem + Wk.q_off);            // [hrot * (max nc + 1)]
     lds_u8 *band = (lds_u8 *)(smem + Wk.band_off);          // [band_bytes]
+    lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);          // [2][4][ZW_WAVE] zero masks
     __shared__ int s_ctx, s_cnt, s_off, s_infeasible, s_nb;
     __shared__ int s_ncand[MAX_ROT];
-    __shared__ unsigned long long s_zero[8 * 16];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
@@ -528,7 +552,6 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                        c_algo = 0, c_over = 0;
     unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ctx_max = 0, ctx_sum = 0;
     const long long n_ctx = L.n_chrom * M;
-    if (tid < 8 * 16) s_zero[tid] = ~0ull;
 
     for (;;) {
         unsigned long long tc0 = tid == 0 ? tick() : 0, tp = tc0;
@@ -676,7 +699,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                     const unsigned char *tc = P.tiles + ((long long)mach * TUNE_NC + min(ncr, TUNE_NC - 1)) * 2;
                     const int key = test_rotation<DENSE>(grid, Wk.pitch, W, h, w, ncr,
                                                          P.qbits + P.qbits_off[rg], cand, ncand,
-                                                         band, Wk.band_bytes, s_q, (lds_u64 *)s_zero, c_groups, c_mfma, ph,
+                                                         band, Wk.band_bytes, s_q, s_zw, c_groups, c_mfma, ph,
                                                          uni(tc[0]), uni(tc[1]));
                     if (key == KEY_NONE) continue;
                     // density (new enclosure length) -> row -> col (binClassNew.py:101-117)
@@ -1106,6 +1129,8 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     for (int r = 0; r < R; r++) qwords = std::max(qwords, (size_t)p->rot_h[r] * (nc[r] + 1));
     w.q_off = (int)fixed;
     fixed += (qwords * 4 + 15) & ~(size_t)15;
+    w.zw_off = (int)fixed;
+    fixed += (size_t)2 * ZW_PASS * sizeof(unsigned long long);
     size_t need_full = 0, need_min = 0;
     for (int m = 0; m < M; m++)
         for (int r = 0; r < R; r++) {
