@@ -68,6 +68,8 @@ typedef LDSP unsigned lds_u32;
 typedef LDSP const v4i lds_cv4i;
 typedef LDSP v4i lds_v4i;
 typedef LDSP unsigned long long lds_u64;
+typedef LDSP const short lds_cs16;
+typedef LDSP const int lds_ci32;
 
 #define WG_THREADS 256
 #define MAX_ROT 4
@@ -146,6 +148,16 @@ __device__ __forceinline__ int machine_of(double mv, const double *thr, int M) {
         if (mv > thr[m - 1] && mv <= thr[m]) return m;
     return M - 1;
 }
+
+// floor(k / d) for the small loop counters of the staging and stamp loops:
+// with m = floor((2^32 - 1) / d) + 1 = ceil(2^32 / d), umulhi(k, m) is exact
+// whenever k * d < 2^32 (here k < 2^17, d < 2^10).  d must be uniform.
+struct DivBy {
+    int d;
+    unsigned m;
+    __device__ __forceinline__ explicit DivBy(int d_) : d(d_), m(d_ > 1 ? 0xFFFFFFFFu / (unsigned)d_ + 1u : 0u) {}
+    __device__ __forceinline__ int operator()(int k) const { return d > 1 ? (int)__umulhi((unsigned)k, m) : k; }
+};
 
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
@@ -441,9 +453,10 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
             unsigned long long t0 = tid == 0 ? tick() : 0;
             const int tl = min(TW, T - tw0);
             const int q16 = (tl + nc - 1) * 2;
+            const DivBy byq(q16);
             const unsigned char *src = grid + (long long)ylo * gpitch + (tw0 << 5);
             for (int k = tid; k < rows * q16; k += WG_THREADS) {
-                const int r = k / q16, q = k - r * q16;
+                const int r = byq(k), q = k - r * q16;
                 *(lds_v4i *)(band + r * stride + (q << 4)) =
                     *(const v4i *)(src + (long long)r * gpitch + (q << 4));
             }
@@ -516,6 +529,41 @@ __device__ __forceinline__ int row_free_run(lds_cu8 *rp, int W) {
 struct __align__(8) PlaceRes { int has, y, x, pad; };
 
 __global__ void reset_counters_kernel(unsigned int *queue) { *queue = 0u; }
+
+// Vacancy filter of one rotation, run by one wave: bit l of word k says top
+// row y = 64k + l passes, i.e. every grid row y+a (a < h) has a free run >=
+// the mask row's longest 1-run (numba_utils.py:131-157 applies the same test
+// to decide whether the bin is tried).  For mask row a, lane l tests grid row
+// 64k + a + l, so the ballot is already aligned with the candidate word.
+// Four mask rows per step keep 4 x NK reads in flight (clamped indices, the
+// guard folded into the predicate); NK = ceil((H - h + 1) / 64) words.
+template <int NK>
+__device__ __forceinline__ void vacancy_words(unsigned long long (&cw)[8], lds_cs16 *vrow,
+                                              lds_ci32 *den, int h, int H, int ny) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int left = ny - (k << 6);
+        cw[k] = k >= NK ? 0ull : (left >= 64 ? ~0ull : ((1ull << left) - 1ull));
+    }
+    for (int a = 0; a < h; a += 4) {
+        unsigned long long any = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int aj = min(a + j, h - 1);
+            const int d = den[aj];
+#pragma unroll
+            for (int k = 0; k < NK; k++) {
+                const int r = (k << 6) + aj + lane;
+                const int v = vrow[min(r, H - 1)];
+                cw[k] &= __ballot(r >= H || v >= d);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NK; k++) any |= cw[k];
+        if (!any) break;
+    }
+}
 
 template <bool DENSE>
 __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, DevWork Wk, LaunchArgs L) {
@@ -642,30 +690,15 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                         const int ny = H - h + 1;
                         const int nk = (ny + 63) >> 6;   // <= 8 for plates <= 512 rows
                         unsigned long long cw[8];
-#pragma unroll
-                        for (int k = 0; k < 8; k++) {
-                            const int left = ny - (k << 6);
-                            cw[k] = k >= nk ? 0ull : (left >= 64 ? ~0ull : ((1ull << left) - 1ull));
-                        }
-                        // loads are unconditional (clamped index) and the guards are
-                        // folded into the predicate, so all 4 x 8 reads of a step are
-                        // in flight together
-                        for (int a = 0; a < h; a += 4) {
-                            unsigned long long any = 0;
-#pragma unroll
-                            for (int j = 0; j < 4; j++) {
-                                const int aj = min(a + j, h - 1);
-                                const int d = den[aj];
-#pragma unroll
-                                for (int k = 0; k < 8; k++) {
-                                    const int r = (k << 6) + aj + lane;
-                                    const int v = vrow[min(r, H - 1)];
-                                    cw[k] &= __ballot(r >= H || v >= d);
-                                }
-                            }
-#pragma unroll
-                            for (int k = 0; k < 8; k++) any |= cw[k];
-                            if (!any) break;
+                        switch (nk) {
+                            case 1: vacancy_words<1>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                            case 2: vacancy_words<2>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                            case 3: vacancy_words<3>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                            case 4: vacancy_words<4>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                            case 5: vacancy_words<5>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                            case 6: vacancy_words<6>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                            case 7: vacancy_words<7>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                            default: vacancy_words<8>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
                         }
                         for (int k = nk - 1; k >= 0; k--) {   // descending y
                             const int y = (k << 6) + 63 - lane;
@@ -745,12 +778,13 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                 // rows [py, py+h) pass through LDS in chunks: load (a fresh bin's
                 // rows are zero), stamp the mask, write back, recompute free runs
                 const int wq = (W + 15) >> 4;               // 16-byte columns per row
+                const DivBy bywq(wq);
                 const int chunk = min(h, Wk.band_bytes / (wq << 4));
                 int *vv = vacs + (long long)pb * hmax;
                 for (int a0 = 0; a0 < h; a0 += chunk) {
                     const int rows = min(chunk, h - a0);
                     for (int k = tid; k < rows * wq; k += WG_THREADS) {
-                        const int r = k / wq, q = k - r * wq;
+                        const int r = bywq(k), q = k - r * wq;
                         *(lds_v4i *)(band + ((r * wq + q) << 4)) =
                             fresh ? (v4i){0, 0, 0, 0}
                                   : *(const v4i *)(grid + (long long)(py + a0 + r) * Wk.pitch + (q << 4));
@@ -759,14 +793,14 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                     const unsigned *qg = P.qbits + P.qbits_off[rg] + (long long)a0 * nq;
                     for (int k = tid; k < rows * nq; k += WG_THREADS) s_q[k] = qg[k];
                     __syncthreads();
-                    for (int k = tid; k < rows * w; k += WG_THREADS) {
-                        const int r = k / w, bcol = k - r * w;
-                        const unsigned bits = s_q[r * nq + ((bcol + 32) >> 5)];
-                        if ((bits >> ((bcol + 32) & 31)) & 1u) band[((r * wq) << 4) + px + bcol] = 1;
-                    }
+                    for (int r = wave; r < rows; r += 4)   // wave per row, lane per cell
+                        for (int bcol = lane; bcol < w; bcol += 64) {
+                            const unsigned bits = s_q[r * nq + ((bcol + 32) >> 5)];
+                            if ((bits >> ((bcol + 32) & 31)) & 1u) band[((r * wq) << 4) + px + bcol] = 1;
+                        }
                     __syncthreads();
                     for (int k = tid; k < rows * wq; k += WG_THREADS) {
-                        const int r = k / wq, q = k - r * wq;
+                        const int r = bywq(k), q = k - r * wq;
                         *(v4i *)(grid + (long long)(py + a0 + r) * Wk.pitch + (q << 4)) =
                             *(lds_cv4i *)(band + ((r * wq + q) << 4));
                     }
