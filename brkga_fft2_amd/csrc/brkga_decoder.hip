@@ -159,13 +159,31 @@ struct DivBy {
     __device__ __forceinline__ int operator()(int k) const { return d > 1 ? (int)__umulhi((unsigned)k, m) : k; }
 };
 
-__device__ __forceinline__ int wave_min(int v) {
+// Wave reductions by a bit-by-bit ballot search: each step is one compare
+// and one ballot, with no cross-lane data movement (a shuffle tree costs six
+// dependent LDS round trips).
+//   wave_max_bits<B>: maximum of values in [0, 2^B);
+//   wave_min_key: minimum of decode keys m * XKEY + x (< 32 * XKEY = 2^18) or KEY_NONE.
+template <int B>
+__device__ __forceinline__ int wave_max_bits(int v) {
+    int r = 0;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        int o = __shfl_xor(v, off, 64);
-        v = o < v ? o : v;
+    for (int b = B - 1; b >= 0; b--) {
+        const int c = r | (1 << b);
+        if (__ballot(v >= c)) r = c;
     }
-    return v;
+    return r;
+}
+
+__device__ __forceinline__ int wave_min_key(int v) {
+    if (!__ballot(v != KEY_NONE)) return KEY_NONE;
+    int r = 0;
+#pragma unroll
+    for (int b = 17; b >= 0; b--) {
+        const int c = r | (1 << b);
+        if (!__ballot(v < c)) r = c;
+    }
+    return r;
 }
 
 // B fragment of the 32x32x32 MFMA for Toeplitz chunk c of one mask row,
@@ -289,7 +307,7 @@ __device__ __forceinline__ int decode_zero(lds_u64 *zp, int tcnt, int xbase,
             }
         }
     }
-    return wave_min(key);
+    return wave_min_key(key);
 }
 
 // Tiles per pass shrink as the chunk count grows so that the A double buffer,
@@ -477,53 +495,62 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
     return found;
 }
 
-// Longest free run of one grid row, computed by one wave: each lane scans a
-// segment, then an ordered tree combine of (prefix run, suffix run, max run,
-// all-free, length) summaries (numba_utils.py:161-188 computes the same
-// sequentially).
+// Longest free run (longest run of zero cells) of one grid row in LDS,
+// computed by one wave (numba_utils.py:161-188 computes the same value
+// sequentially).  Blocks of 512 cells: lane l reads cells 8l..8l+7 with one
+// 8-byte LDS read (cells past W count as occupied) and keeps their free cells
+// at the low end (pre) and high end (suf) and their longest free run (a
+// 16-pattern table per 4 cells and the run across the two halves).  A run
+// crossing lanes starts in the suffix of a lane, spans the all-free lanes
+// after it (one ballot) and ends in the prefix of the next lane that is not
+// all free (one shuffle); the run at the block start adds the free cells
+// carried from the previous block.
+__device__ __forceinline__ unsigned occupied4(unsigned v) {   // bit j: byte j non-zero
+    v |= v >> 4;
+    v |= v >> 2;
+    v |= v >> 1;
+    return ((v & 0x01010101u) * 0x01020408u) >> 24;   // gathers bit 8j to bit 24 + j
+}
+
+__device__ __forceinline__ int free_run4(unsigned o4) {   // longest free run of 4 cells
+    return (int)(((o4 < 8 ? 0x11122234u : 0x01121123u) >> ((o4 & 7u) << 2)) & 0xFu);
+}
+
 __device__ __forceinline__ int row_free_run(lds_cu8 *rp, int W) {
     const int lane = threadIdx.x & 63;
-    const int seg = (W + 63) >> 6;   // <= 8 for rows <= 512 cells; longer rows loop
-    const int c0 = min(W, lane * seg), c1 = min(W, c0 + seg);
-    int p = 0, mx = 0, run = 0, len = c1 - c0;
-    bool full = true, seen = false;
-    for (int cb = c0; cb < c1; cb += 8) {
-        unsigned char cell[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) cell[j] = rp[min(cb + j, W - 1)];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            if (cb + j < c1) {
-                if (cell[j]) {
-                    if (!seen) { p = run; seen = true; }
-                    run = 0;
-                    full = false;
-                } else {
-                    run++;
-                    mx = run > mx ? run : mx;
-                }
-            }
+    int best = 0, carry = 0;
+    for (int c0 = 0; c0 < W; c0 += 512) {
+        const int x = c0 + (lane << 3);
+        unsigned o = 0xFFu;   // bit j: cell x + j occupied
+        if (x < W) {
+            // x + 7 stays inside the row's 16-byte padded span
+            const unsigned long long v = *(LDSP const unsigned long long *)(rp + x);
+            o = occupied4((unsigned)v) | (occupied4((unsigned)(v >> 32)) << 4);
+            const int valid = W - x;
+            if (valid < 8) o |= 0xFFu & (0xFFu << valid);
         }
-    }
-    if (!seen) p = run;
-    int s = run;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int op = __shfl_down(p, off, 64), os = __shfl_down(s, off, 64);
-        const int om = __shfl_down(mx, off, 64), ol = __shfl_down(len, off, 64);
-        const int of = __shfl_down((int)full, off, 64);
-        if ((lane & (2 * off - 1)) == 0) {
-            const int nm = max(max(mx, om), s + op);
-            const int np = full ? len + op : p;
-            const int ns = of ? ol + s : os;
-            full = full && of;
-            len += ol;
-            p = np;
-            s = ns;
-            mx = nm;
+        const unsigned lo4 = o & 0xFu, hi4 = o >> 4;
+        const int pre = __builtin_ctz(o | 0x100u);
+        const int suf = o ? __builtin_clz(o) - 24 : 8;
+        const int across = (lo4 ? __builtin_clz(lo4) - 28 : 4) + __builtin_ctz(hi4 | 0x10u);
+        int b = max(max(free_run4(lo4), free_run4(hi4)), across);
+        const unsigned long long F = __ballot(o == 0);   // all-free lanes
+        const unsigned long long rest = lane < 63 ? (F >> (lane + 1)) : 0ull;
+        const int c = min((int)__builtin_ctzll(~rest), 63 - lane);
+        const int nxt = lane + 1 + c;
+        const int pre_n = __shfl(pre, min(nxt, 63), 64);
+        if (o != 0 && nxt <= 63) b = max(b, suf + 8 * c + pre_n);
+        best = max(best, wave_max_bits<10>(b));   // b <= 8 + 8 * 62 + 8 = 512
+        if (F == ~0ull) {
+            carry += 512;
+            continue;
         }
+        const int f = __builtin_ctzll(~F);            // first lane that is not all free
+        best = max(best, carry + 8 * f + __builtin_amdgcn_readlane(pre, f));
+        const int last = 63 - __builtin_clzll(~F);     // last lane that is not all free
+        carry = __builtin_amdgcn_readlane(suf, last) + 8 * (63 - last);
     }
-    return __shfl(mx, 0, 64);
+    return max(best, carry);
 }
 
 struct __align__(8) PlaceRes { int has, y, x, pad; };
