@@ -286,25 +286,26 @@ __device__ __forceinline__ void corr_pass(lds_cu8 *band, int stride, int rowoff,
 // MFMA: lane holds column n = lane&31 and rows m = 8*(i/4) + 4*(lane/32) + i%4
 // of accumulator i, so row m lives in ballot word i = 4*(m/8) + m%4, half
 // (m/4)%2.  Lanes 0..31 each scan one row m; every wave computes the result.
-__device__ __forceinline__ int decode_zero(lds_u64 *zp, int tcnt, int xbase,
-                                           int xmax, int nrows) {
+template <int TP>
+__device__ __forceinline__ int decode_zero(lds_u64 *zp, int xbase, int xmax, int nrows) {
     const int lane = threadIdx.x & 63;
-    int key = KEY_NONE;
     const int m = lane & 31;
+    const int i = ((m >> 3) << 2) + (m & 3);
+    const int half = ((m >> 2) & 1) << 5;
+    unsigned bits[TP];
+#pragma unroll
+    for (int t = 0; t < TP; t++) {   // all 4 x TP reads in flight together
+        const int e = t * 16 + i;
+        const unsigned long long z = zp[e] & zp[ZW_WAVE + e] & zp[2 * ZW_WAVE + e] & zp[3 * ZW_WAVE + e];
+        bits[t] = (unsigned)(z >> half);
+    }
+    int key = KEY_NONE;
     if (lane < 32 && m < nrows) {
-        const int i = ((m >> 3) << 2) + (m & 3);
-        const int half = ((m >> 2) & 1) << 5;
-        for (int t = 0; t < tcnt; t++) {
-            const int e = t * 16 + i;
-            const unsigned long long z = zp[e] & zp[ZW_WAVE + e] & zp[2 * ZW_WAVE + e] & zp[3 * ZW_WAVE + e];
-            unsigned bits = (unsigned)(z >> half);
+#pragma unroll
+        for (int t = TP - 1; t >= 0; t--) {   // the smallest tile with a zero wins
             const int lim = xmax - (xbase + (t << 5));   // last valid column offset in this tile
-            if (lim < 0) break;
-            if (lim < 31) bits &= (2u << lim) - 1u;
-            if (bits) {
-                key = m * XKEY + xbase + (t << 5) + __builtin_ctz(bits);
-                break;
-            }
+            unsigned b = lim < 0 ? 0u : (lim < 31 ? bits[t] & ((2u << lim) - 1u) : bits[t]);
+            if (b) key = m * XKEY + xbase + (t << 5) + __builtin_ctz(b);
         }
     }
     return wave_min_key(key);
@@ -329,7 +330,7 @@ __device__ __forceinline__ int one_pass(lds_cu8 *band, int stride, int rowoff, i
     corr_pass<TP, NC>(band, stride, rowoff, a0, a1, qbits, t0, TP, zp + wave * ZW_WAVE);
     __syncthreads();
     parity ^= 1;
-    return decode_zero(zp, TP, xbase + (t0 << 5), xmax, nrows);
+    return decode_zero<TP>(zp, xbase + (t0 << 5), xmax, nrows);
 }
 
 // All passes over one staged x-window (tiles [0, tl) of the window), run by
@@ -390,7 +391,7 @@ __device__ int window_passes_generic(lds_cu8 *band, int stride, int rowoff, int 
         store_zero_masks<1>(acc, 1, zp + wave * ZW_WAVE);
         __syncthreads();
         parity ^= 1;
-        const int k = decode_zero(zp, 1, xbase + (t0 << 5), xmax, nrows);
+        const int k = decode_zero<1>(zp, xbase + (t0 << 5), xmax, nrows);
         key = k < key ? k : key;
     }
     return key;
@@ -433,6 +434,26 @@ __host__ __device__ __forceinline__ int band_window(int T, int nc, int h, int ba
 
 #define WINDOW_CAP 64   // x-tiles per staged window at most
 
+// Copy n 16-byte vectors with the whole workgroup, COPY_BATCH loads in
+// flight per thread: load(k) returns vector k, store(k, v) puts it.
+#define COPY_BATCH 8
+template <typename Load, typename Store>
+__device__ __forceinline__ void wg_copy(int n, Load load, Store store) {
+    for (int k0 = threadIdx.x; k0 < n; k0 += COPY_BATCH * WG_THREADS) {
+        v4i v[COPY_BATCH];
+#pragma unroll
+        for (int u = 0; u < COPY_BATCH; u++) {
+            const int k = k0 + u * WG_THREADS;
+            if (k < n) v[u] = load(k);
+        }
+#pragma unroll
+        for (int u = 0; u < COPY_BATCH; u++) {
+            const int k = k0 + u * WG_THREADS;
+            if (k < n) store(k, v[u]);
+        }
+    }
+}
+
 // Collision test of one rotation against one bin, run by the whole workgroup.
 // cand: candidate top rows, descending.  Groups of up to 32 candidates are
 // staged into the LDS band (rows ylo..ytop+h-1, one x-window at a time) and
@@ -473,11 +494,11 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
             const int q16 = (tl + nc - 1) * 2;
             const DivBy byq(q16);
             const unsigned char *src = grid + (long long)ylo * gpitch + (tw0 << 5);
-            for (int k = tid; k < rows * q16; k += WG_THREADS) {
-                const int r = byq(k), q = k - r * q16;
-                *(lds_v4i *)(band + r * stride + (q << 4)) =
-                    *(const v4i *)(src + (long long)r * gpitch + (q << 4));
-            }
+            wg_copy(rows * q16,
+                    [&](int k) { const int r = byq(k), q = k - r * q16;
+                                 return *(const v4i *)(src + (long long)r * gpitch + (q << 4)); },
+                    [&](int k, v4i v) { const int r = byq(k), q = k - r * q16;
+                                        *(lds_v4i *)(band + r * stride + (q << 4)) = v; });
             __syncthreads();
             unsigned long long t1 = tid == 0 ? tick() : 0;
             if (tid == 0) ph[2] += t1 - t0;
@@ -810,11 +831,15 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                 int *vv = vacs + (long long)pb * hmax;
                 for (int a0 = 0; a0 < h; a0 += chunk) {
                     const int rows = min(chunk, h - a0);
-                    for (int k = tid; k < rows * wq; k += WG_THREADS) {
-                        const int r = bywq(k), q = k - r * wq;
-                        *(lds_v4i *)(band + ((r * wq + q) << 4)) =
-                            fresh ? (v4i){0, 0, 0, 0}
-                                  : *(const v4i *)(grid + (long long)(py + a0 + r) * Wk.pitch + (q << 4));
+                    if (fresh) {
+                        for (int k = tid; k < rows * wq; k += WG_THREADS)
+                            *(lds_v4i *)(band + (k << 4)) = (v4i){0, 0, 0, 0};
+                    } else {
+                        const unsigned char *g0 = grid + (long long)(py + a0) * Wk.pitch;
+                        wg_copy(rows * wq,
+                                [&](int k) { const int r = bywq(k), q = k - r * wq;
+                                             return *(const v4i *)(g0 + (long long)r * Wk.pitch + (q << 4)); },
+                                [&](int k, v4i v) { *(lds_v4i *)(band + (k << 4)) = v; });
                     }
                     const int nq = P.rot_nc[rg] + 1;   // mask bits of these rows -> LDS
                     const unsigned *qg = P.qbits + P.qbits_off[rg] + (long long)a0 * nq;
