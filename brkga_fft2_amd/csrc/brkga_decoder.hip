@@ -434,13 +434,14 @@ __host__ __device__ __forceinline__ int band_window(int T, int nc, int h, int ba
 
 #define WINDOW_CAP 64   // x-tiles per staged window at most
 
-// Copy n 16-byte vectors with the whole workgroup, COPY_BATCH loads in
-// flight per thread: load(k) returns vector k, store(k, v) puts it.
+// Copy n elements (16-byte vectors by default) with the whole workgroup,
+// COPY_BATCH loads in flight per thread: load(k) returns element k,
+// store(k, v) puts it.
 #define COPY_BATCH 8
-template <typename Load, typename Store>
+template <typename T = v4i, typename Load, typename Store>
 __device__ __forceinline__ void wg_copy(int n, Load load, Store store) {
     for (int k0 = threadIdx.x; k0 < n; k0 += COPY_BATCH * WG_THREADS) {
-        v4i v[COPY_BATCH];
+        T v[COPY_BATCH];
 #pragma unroll
         for (int u = 0; u < COPY_BATCH; u++) {
             const int k = k0 + u * WG_THREADS;
@@ -467,7 +468,8 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
                              unsigned long long &mfma, unsigned long long *ph, int tpcap, int wcap) {
     const int tid = threadIdx.x, lane = tid & 63;
     // mask bits of the rotation -> LDS (read by every wave for every group)
-    for (int k = tid; k < h * (nc + 1); k += WG_THREADS) s_q[k] = qbits_g[k];
+    wg_copy<unsigned>(h * (nc + 1), [&](int k) { return qbits_g[k]; },
+                      [&](int k, unsigned v) { s_q[k] = v; });
     const int T = (W - w + 1 + 31) >> 5;
     int stride, rcap;
     const int TW = band_window(T, nc, h, band_bytes, wcap, &stride, &rcap);
@@ -699,11 +701,28 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
             const int nr = P.nrot[ty];
             const int rb = P.rot_base[ty];
             const int nb = uni(s_nb);
-            // densities of this part's rotations -> LDS
-            for (int r = 0; r < nr; r++) {
-                const int h = P.rot_h[rb + r];
-                const int *den = P.dens + P.dens_off[rb + r];
-                for (int a = tid; a < h; a += WG_THREADS) s_den[r * Wk.hrot + a] = den[a];
+            // densities of this part's rotations -> LDS, one batched pass
+            {
+                int hr[MAX_ROT];
+                long long dof[MAX_ROT];
+#pragma unroll
+                for (int r = 0; r < MAX_ROT; r++) {
+                    hr[r] = r < nr ? P.rot_h[rb + r] : 0;
+                    dof[r] = r < nr ? P.dens_off[rb + r] : 0;
+                }
+                const int hrot = Wk.hrot;
+                const DivBy byh(hrot);
+                wg_copy<int>(nr * hrot,
+                             [&](int k) {
+                                 const int r = byh(k), a = k - r * hrot;
+                                 int h = hr[0];
+                                 long long o = dof[0];
+#pragma unroll
+                                 for (int q = 1; q < MAX_ROT; q++)
+                                     if (r == q) { h = hr[q]; o = dof[q]; }
+                                 return a < h ? P.dens[o + a] : 0;
+                             },
+                             [&](int k, int v) { s_den[k] = v; });
             }
             __syncthreads();   // s_den is read by every wave's vacancy filter
             bool placed = false;
@@ -843,7 +862,8 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                     }
                     const int nq = P.rot_nc[rg] + 1;   // mask bits of these rows -> LDS
                     const unsigned *qg = P.qbits + P.qbits_off[rg] + (long long)a0 * nq;
-                    for (int k = tid; k < rows * nq; k += WG_THREADS) s_q[k] = qg[k];
+                    wg_copy<unsigned>(rows * nq, [&](int k) { return qg[k]; },
+                                      [&](int k, unsigned v) { s_q[k] = v; });
                     __syncthreads();
                     for (int r = wave; r < rows; r += 4)   // wave per row, lane per cell
                         for (int bcol = lane; bcol < w; bcol += 64) {
