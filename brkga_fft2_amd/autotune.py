@@ -3,7 +3,7 @@
 The exact correlation runs in passes of up to 8 MFMA x-tiles (32 placements
 each) over LDS windows of up to 64 x-tiles (``csrc/brkga_decoder.hip``).  The
 best pass and window sizes depend on the mask width (how many 32-column
-chunks feed each tile, hence register pressure) and on the plate width (how
+Toeplitz chunks, ceil((w + 31) / 32), feed each tile, hence register pressure) and on the plate width (how
 many tiles a row has).  ``autotune`` times the decoder on a sample population
 for each (plate size, mask width class) entry of the tile table and keeps the
 fastest setting, coordinate by coordinate; ``apply_cached`` reloads a result
@@ -15,7 +15,6 @@ sample before returning.
 """
 import hashlib
 import json
-import math
 import os
 
 import numpy as np
@@ -27,18 +26,20 @@ WINDOW_CHOICES = (4, 8, 16, 32, 64)
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "brkga_mi355x", "tiles.json")
 
 
+def chunk_class(w):
+    """Tile-table class of a mask w columns wide: the Toeplitz chunk count
+    ceil((w + 31) / 32) the kernel uses, clamped to the last class."""
+    return int(min((int(w) + 62) // 32, TILE_CLASSES - 1))
+
+
 def _default_tiles(nc):
     nc = max(nc, 1)
-    return 8 if nc <= 3 else (4 if nc <= 5 else 2)
+    return 8 if nc <= 5 else (4 if nc <= 8 else 2)
 
 
 def tile_classes(packed):
     """``{(plate_h, plate_w): sorted list of width classes}`` present in the instance."""
-    W = np.asarray(packed["rot_w"], np.int64)
-    H = np.asarray(packed["rot_h"], np.int64)
-    ncs = set()
-    for h, w in zip(H, W):
-        ncs.add(int(min(math.ceil(w / 32), TILE_CLASSES - 1)))
+    ncs = {chunk_class(w) for w in np.asarray(packed["rot_w"], np.int64)}
     out = {}
     for m in range(int(packed["n_machines"])):
         key = (int(packed["plate_h"][m]), int(packed["plate_w"][m]))

@@ -99,8 +99,8 @@ struct DevProblem {
     const int *plate_h, *plate_w;
     const double *setup, *proc_time, *pth;
     const unsigned char *fits;   // [M][n_types] rotation 0 fits the plate in some orientation
-    // Tile shapes of the collision passes per (machine, mask width class
-    // nc = ceil(w/32), clamped to TUNE_NC-1): [M][TUNE_NC][2] = {cap on x-tiles
+    // Tile shapes of the collision passes per (machine, Toeplitz chunk count
+    // nc = ceil((w+31)/32), clamped to TUNE_NC-1): [M][TUNE_NC][2] = {cap on x-tiles
     // per pass, cap on x-tiles per staged LDS window}.  Any values give the
     // same results; only the speed changes.
     const unsigned char *tiles;
@@ -217,8 +217,7 @@ __device__ __forceinline__ v4i bfrag_from_bits(unsigned lo, unsigned hi, int shi
 // (zw[wave][t*16 + i], bit = lane) with plain LDS stores and the decode ANDs
 // the four.  A operands come from the LDS band, B operands are built from the
 // mask bits; with NC (Toeplitz chunks) known at compile time each A fragment
-// feeds up to NC MFMAs and each B fragment feeds all TP tiles, and the next
-// mask row's A fragments are loaded while the current row's MFMAs run.
+// feeds up to NC MFMAs and each B fragment feeds all TP tiles.
 // Store one wave's zero masks of `tcnt` tiles: lane i < 16 of the wave
 // collects accumulator i's ballot, then one 8-byte store per lane.
 template <int TP>
@@ -250,11 +249,16 @@ __device__ __forceinline__ void corr_pass(lds_cu8 *band, int stride, int rowoff,
     for (int t = 0; t < TP; t++) acc[t] = (v16i){};
     lds_cu8 *base = band + rowoff * stride + (tloc << 5) + ((lane >> 5) << 4);
     if (a0 < a1) {
-        v4i A[NA];
+        // A fragments in order j = 0..NA-1 per mask row, each feeding the
+        // pairs (t = j - c, c); fragment j + D is loaded while j is used (the
+        // last D of a row load the next row's first D), so only ~D+1
+        // fragments are live instead of two full rows.
+        constexpr int D = NA < 2 ? NA : 2;
+        v4i N[D];
         unsigned qv[NC + 1];
         lds_cu8 *r0 = base + a0 * stride;
 #pragma unroll
-        for (int j = 0; j < NA; j++) A[j] = *(lds_cv4i *)(r0 + (j << 5));
+        for (int j = 0; j < D; j++) N[j] = *(lds_cv4i *)(r0 + (j << 5));
 #pragma unroll
         for (int c = 0; c <= NC; c++) qv[c] = qbits[a0 * (NC + 1) + c];
         for (int a = a0; a < a1; a++) {
@@ -262,20 +266,25 @@ __device__ __forceinline__ void corr_pass(lds_cu8 *band, int stride, int rowoff,
 #pragma unroll
             for (int c = 0; c < NC; c++) B[c] = bfrag_from_bits(qv[c], qv[c + 1], shift);
             const int an = a + 1 < a1 ? a + 1 : a;
-            v4i An[NA];
+            lds_cu8 *rc = base + a * stride;
             lds_cu8 *rn = base + an * stride;
 #pragma unroll
-            for (int j = 0; j < NA; j++) An[j] = *(lds_cv4i *)(rn + (j << 5));
-#pragma unroll
             for (int c = 0; c <= NC; c++) qv[c] = qbits[an * (NC + 1) + c];
+            v4i A[NA];
 #pragma unroll
-            for (int c = 0; c < NC; c++) {
+            for (int j = 0; j < D; j++) A[j] = N[j];
 #pragma unroll
-                for (int t = 0; t < TP; t++)
-                    acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[t + c], B[c], acc[t], 0, 0, 0);
+            for (int j = 0; j < NA; j++) {
+                const int p = j + D;
+                if (p < NA) A[p] = *(lds_cv4i *)(rc + (p << 5));
+                else N[p - NA] = *(lds_cv4i *)(rn + ((p - NA) << 5));
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    const int t = j - c;
+                    if (t >= 0 && t < TP)
+                        acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[j], B[c], acc[t], 0, 0, 0);
+                }
             }
-#pragma unroll
-            for (int j = 0; j < NA; j++) A[j] = An[j];
         }
     }
     store_zero_masks<TP>(acc, tcnt, zw);
@@ -311,11 +320,11 @@ __device__ __forceinline__ int decode_zero(lds_u64 *zp, int xbase, int xmax, int
     return wave_min_key(key);
 }
 
-// Tiles per pass shrink as the chunk count grows so that the A double buffer,
-// the B fragments and the accumulators fit the register budget.
+// Tiles per pass shrink as the chunk count grows so that the accumulators,
+// the B fragments and the A fragments in flight fit the register budget.
 #define MAX_NC 12
 #define TUNE_NC BRKGA_TILE_CLASSES
-__host__ __device__ constexpr int pass_tiles(int nc) { return nc <= 3 ? 8 : (nc <= 5 ? 4 : (nc <= MAX_NC ? 2 : 1)); }
+__host__ __device__ constexpr int pass_tiles(int nc) { return nc <= 5 ? 8 : (nc <= 8 ? 4 : (nc <= MAX_NC ? 2 : 1)); }
 
 // One pass of TP tiles: correlate, combine the waves' zero masks, decode.
 // Passes alternate between two mask buffers, so one barrier per pass is

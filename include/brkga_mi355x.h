@@ -143,10 +143,12 @@ int brkga_reset_stats(brkga_handle *h);
 /*
  * Tile shapes of the collision passes (the autotuner's knobs; no counterpart
  * in the reference, whose FFT has none).  Tables are [n_machines][BRKGA_TILE_CLASSES],
- * indexed by machine and mask width class nc = ceil(w / 32) (wider masks share
- * the last class):
+ * indexed by machine and the mask's Toeplitz chunk count nc = ceil((w + 31) / 32)
+ * (the 32-column chunks a 32-placement tile spans; larger counts share the last
+ * class):
  *   tiles_per_pass: cap on 32-column x-tiles one MFMA pass covers, 1, 2, 4 or 8
- *                   (0 = the width's default: 8 up to 96 columns, 4 up to 160, else 2);
+ *                   (0 = the default: 8 up to nc = 5, i.e. masks up to 129 columns,
+ *                   4 up to nc = 8 (225 columns), else 2);
  *   window_tiles:   cap on x-tiles staged per LDS window, 1..64 (0 = 64).
  * Every setting gives identical results.  Either table pointer may be NULL
  * (left unchanged).  brkga_set_tiles waits for the device to be idle; it must
