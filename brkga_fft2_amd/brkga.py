@@ -17,7 +17,9 @@ numbers as the reference's:
 Fitness comes from the MI355X decoder through ``cal_fitness`` (the
 ``native_full`` / ``wave_batch`` evaluators of ``BRKGA_alg3.py:119-123``).
 """
+import json
 import math
+import os
 import time
 
 import numpy as np
@@ -128,3 +130,39 @@ def brkga_sizes(nb_parts, mult=10):
     n = mult * nb_parts
     return dict(num_individuals=n, num_elites=math.ceil(n * 0.1),
                 num_mutants=math.ceil(n * 0.15), eliteCProb=0.70)
+
+
+def write_history(path, runs, **meta):
+    """Write a run report as JSON (replaces the reference's Excel export of
+    ``model.history``, one sheet per repetition, ``BRKGA_alg3.py:611-629``).
+
+    ``runs``: one entry per repetition -- a fitted ``BRKGA`` (its ``history``
+    columns ``mean``, ``min``, ``time`` and its best fitness and chromosome)
+    or a plain ``{"mean": [...], "min": [...], "time": [...]}`` mapping.
+    ``meta``: anything else to record (instance name, sizes, seed ...).
+    The file holds ``{"meta": meta, "runs": [{"history": {...},
+    "best_fitness": f, "best_chromosome": [...]}, ...]}``, values as plain
+    JSON numbers (the 1e16 infeasibility penalty included).
+    """
+    out = []
+    for r in runs:
+        hist = r.history if hasattr(r, "history") else r
+        rec = {"history": {k: [float(v) for v in hist[k]] for k in hist}}
+        if hasattr(r, "best_fitness"):
+            rec["best_fitness"] = float(r.best_fitness)
+        sol = getattr(r, "solution", None)
+        if sol is not None:
+            rec["best_chromosome"] = [float(v) for v in np.asarray(sol).ravel()]
+        out.append(rec)
+    doc = {"meta": meta, "runs": out}
+    tmp = str(path) + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(doc, f)
+    os.replace(tmp, path)
+    return doc
+
+
+def read_history(path):
+    """Load a report written by ``write_history``."""
+    with open(path) as f:
+        return json.load(f)

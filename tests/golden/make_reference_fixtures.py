@@ -4,7 +4,8 @@ Run once in a container where the reference tree is mounted read-only:
 
     python tests/golden/make_reference_fixtures.py /root/reference
 
-Nothing here imports or runs reference code.  It reads data files only:
+Nothing here imports or runs reference code.  It reads data files only
+(with ``brkga_fft2_amd.instances``):
 
 * part masks ``data/partsMatrices/matrix_<id>.npy`` (int64 {0,1}) with
   ``numpy.load(allow_pickle=False)``; stored here bit-packed;
@@ -23,7 +24,7 @@ Outputs (all under ``tests/golden/``):
 
 * ``ref_parts.npz``      masks of parts 0..99, bit-packed, plus shapes;
 * ``ref_specs.json``     part volume/support/height/area and machine L/W/ST/VT/SPT/HT;
-* ``ref_instances.json`` part-id lists of the instances used by the tests;
+* ``ref_instances.json`` part-id lists of all 30 instances;
 * ``golden_P75M2-0.json`` chromosome + per-machine placement log + makespans;
 * ``golden_P50M2-0_c462.npz`` the 8 PP bin grids (bit-packed) + part orders;
 * ``fingerprints.json``  first-5-feasible makespans and feasible counts.
@@ -32,66 +33,24 @@ import json
 import os
 import re
 import sys
-import zipfile
-import xml.etree.ElementTree as ET
 
 import numpy as np
 
-NS = {"m": "http://schemas.openxmlformats.org/spreadsheetml/2006/main"}
-RELNS = "http://schemas.openxmlformats.org/officeDocument/2006/relationships"
 OUT = os.path.dirname(os.path.abspath(__file__))
-
-INSTANCES = ["P25M2-2", "P25M2-4", "P50M2-0", "P50M2-1", "P50M2-3", "P75M2-0",
-             "P75M2-4", "P100M4-0", "P100M4-3"]
-
-
-def _xlsx_sheets(path):
-    """Return {sheet_name: [[cell text or None, ...] per row]} (header row first)."""
-    z = zipfile.ZipFile(path)
-    shared = []
-    if "xl/sharedStrings.xml" in z.namelist():
-        root = ET.fromstring(z.read("xl/sharedStrings.xml"))
-        for si in root.findall("m:si", NS):
-            shared.append("".join(t.text or "" for t in si.iter("{%s}t" % NS["m"])))
-    wb = ET.fromstring(z.read("xl/workbook.xml"))
-    rels = ET.fromstring(z.read("xl/_rels/workbook.xml.rels"))
-    target = {r.get("Id"): r.get("Target") for r in rels}
-    out = {}
-    for s in wb.find("m:sheets", NS):
-        name = s.get("name")
-        sheet = ET.fromstring(z.read("xl/" + target[s.get("{%s}id" % RELNS)]))
-        rows = []
-        for row in sheet.iter("{%s}row" % NS["m"]):
-            cells = {}
-            for c in row.findall("m:c", NS):
-                col = re.match(r"[A-Z]+", c.get("r")).group(0)
-                v = c.find("m:v", NS)
-                val = None if v is None else v.text
-                if c.get("t") == "s" and val is not None:
-                    val = shared[int(val)]
-                cells[col] = val
-            ncol = max((ord(k) - 64 for k in cells), default=0)
-            rows.append([cells.get(chr(65 + i)) for i in range(ncol)])
-        out[name] = rows
-    return out
-
-
-def _num(text):
-    """Spreadsheet number text -> int when integral text, else float (as openpyxl)."""
-    if re.fullmatch(r"-?\d+", text):
-        return int(text)
-    return float(text)
+sys.path.insert(0, os.path.dirname(os.path.dirname(OUT)))
+from brkga_fft2_amd.instances import instance_names, read_instance, read_mask, read_specs  # noqa: E402
 
 
 def main(ref):
+    data = os.path.join(ref, "data")
     # ---- masks -----------------------------------------------------------
+    specs = read_specs(data)
+    assert specs["part_ids"] == list(range(100))
     shapes, bits, offs = [], [], [0]
-    for pid in range(100):
-        m = np.load(os.path.join(ref, "data/partsMatrices/matrix_%d.npy" % pid),
-                    allow_pickle=False)
-        assert m.ndim == 2 and set(np.unique(m)) <= {0, 1}
+    for pid in specs["part_ids"]:
+        m = read_mask(data, pid)
         shapes.append(m.shape)
-        packed = np.packbits(m.astype(np.uint8).ravel())
+        packed = np.packbits(m.ravel())
         bits.append(packed)
         offs.append(offs[-1] + packed.size)
     np.savez_compressed(os.path.join(OUT, "ref_parts.npz"),
@@ -100,43 +59,12 @@ def main(ref):
                         bits=np.concatenate(bits))
 
     # ---- specs -----------------------------------------------------------
-    pm = _xlsx_sheets(os.path.join(ref, "data/PartsMachines/part-machine-information.xlsx"))
-    part_rows = pm["part"]
-    hdr = part_rows[0]
-    parts = {}
-    for r in part_rows[1:]:
-        rec = {hdr[i]: _num(r[i]) for i in range(len(hdr)) if r[i] is not None}
-        parts[int(rec["part ID"])] = rec
-    mach_rows = pm["machine"]
-    mh = mach_rows[0]
-    machines = [{mh[i]: _num(r[i]) for i in range(len(mh))} for r in mach_rows[1:]]
-    areas = _xlsx_sheets(os.path.join(ref, "data/PartsMachines/polygon_areas.xlsx"))
-    arows = list(areas.values())[0]
-    assert arows[0] == ["part ID", "Area"]
-    area = [None] * 100
-    for r in arows[1:]:
-        area[int(r[0])] = float(r[1])
-    assert [int(r[0]) for r in arows[1:]] == list(range(100))
-    specs = {
-        "part_ids": list(range(100)),
-        "volume": [float(parts[i]["volume(mm3)"]) for i in range(100)],
-        "support": [float(parts[i]["support(mm3)"]) for i in range(100)],
-        "height": [float(parts[i]["height(mm)"]) for i in range(100)],
-        "area": area,
-        "machines": [{"L": int(m["L(mm)"]), "W": int(m["W(mm)"]),
-                      "ST": m["ST(s)"], "VT": float(m["VT(s/mm3)"]),
-                      "SPT": float(m["SPT(s/mm3)"]), "HT": float(m["HT(s/mm3)"])}
-                     for m in machines],
-        "source": "data/PartsMachines/{part-machine-information,polygon_areas}.xlsx",
-    }
+    specs["source"] = "data/PartsMachines/{part-machine-information,polygon_areas}.xlsx"
     with open(os.path.join(OUT, "ref_specs.json"), "w") as f:
         json.dump(specs, f, indent=1)
 
-    # ---- instances -------------------------------------------------------
-    inst = {}
-    for name in INSTANCES:
-        with open(os.path.join(ref, "data/Instances/%s.txt" % name)) as f:
-            inst[name] = [int(x) for x in f.read().split()]
+    # ---- instances (all of data/Instances) -------------------------------
+    inst = {name: read_instance(data, name) for name in instance_names(data)}
     with open(os.path.join(OUT, "ref_instances.json"), "w") as f:
         json.dump(inst, f)
 

@@ -123,6 +123,46 @@ def test_reference_fingerprints(name):
     dec.close()
 
 
+with open(os.path.join(GOLDEN, "ref_instances.json")) as _f:
+    ALL_INSTANCES = list(json.load(_f))
+
+
+def _routed_keys(packed, n_chrom, rng):
+    """Half random keys; half with every part's machine key inside the bucket
+    of a machine whose plate fits the part (so most chromosomes are feasible)."""
+    n, M = int(packed["n_parts"]), int(packed["n_machines"])
+    keys = rng.random((n_chrom, 2 * n))
+    edges = np.concatenate([[0.0], np.asarray(packed["thresholds"], np.float64)[:M - 1], [1.0]])
+    for c in range(n_chrom // 2, n_chrom):
+        for j in range(n):
+            t = int(packed["slot_type"][j])
+            r0 = int(packed["rot_base"][t])
+            h0, w0 = int(packed["rot_h"][r0]), int(packed["rot_w"][r0])
+            ok = [m for m in range(M)
+                  if (h0 <= packed["plate_h"][m] and w0 <= packed["plate_w"][m]) or
+                     (w0 <= packed["plate_h"][m] and h0 <= packed["plate_w"][m])]
+            if ok:
+                m = ok[rng.integers(len(ok))]
+                lo, hi = edges[m], edges[m + 1]
+                keys[c, n + j] = lo + (hi - lo) * (0.05 + 0.9 * rng.random())
+    return keys.astype(np.float32)
+
+
+@pytest.mark.parametrize("name", ALL_INSTANCES)
+def test_every_reference_instance_vs_oracle(name):
+    """All 30 reference instances: placement logs and makespans vs the oracle."""
+    _say("instance %s" % name)
+    prob, n, M, packed = ref_instance(name)
+    keys = _routed_keys(packed, 6, np.random.default_rng(ALL_INSTANCES.index(name)))
+    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=os.cpu_count() or 4, with_log=True)
+    assert np.sum(ref_ms < 1e15) >= 1
+    dec = _decoder(packed)
+    ms, log = dec.evaluate(keys, with_log=True)
+    np.testing.assert_array_equal(ms, ref_ms)
+    assert feasible_logs_equal(ref_ms, log, ref_log) is None
+    dec.close()
+
+
 def test_full_population_vs_oracle_p50m2_0():
     """verify_full_population_vs_pp.py:88-194: seed-42 population, element-wise."""
     _say("full population P50M2-0")
