@@ -109,8 +109,6 @@ struct DevProblem {
 struct DevWork {
     unsigned char *grids;    // [n_wg][max_bins][hmax][pitch]
     int *vac;                // [n_wg][max_bins][hmax]
-    double *bin_area, *bin_pt, *bin_pth;  // [n_wg][max_bins]
-    int *bin_minrow;         // [n_wg][max_bins]
     unsigned int *queue;
     unsigned long long *counters;
     int pitch, hmax, max_bins;
@@ -119,6 +117,7 @@ struct DevWork {
     int vc_bins;             // bins whose vacancy vector is cached in LDS
     int band_off, band_bytes;  // LDS band for staged grid rows
     int zw_off;                // LDS zero masks: [2 passes][4 waves][8 tiles * 16] u64
+    int bins_off;              // LDS bin bookkeeping: area, pt, pth [n_parts] f64, minrow [n_parts] i32
 };
 
 struct LaunchArgs {
@@ -633,8 +632,8 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
     // LDS layout (sizes computed on the host, brkga_create)
     double *s_sv = (double *)smem;                          // [n]
     int *s_mach = (int *)(s_sv + n);                        // [n]
-    int *s_seq = s_mach + n;                                // [n]
-    int *s_den = s_seq + n;                                 // [MAX_ROT][hrot]
+    int *s_ty = s_mach + n;                                 // [n] part type at sequence position i
+    int *s_den = s_ty + n;                                  // [MAX_ROT][hrot]
     short *s_cand = (short *)(s_den + MAX_ROT * Wk.hrot);   // [MAX_ROT][hmax]
     short *s_vac = s_cand + MAX_ROT * hmax;                 // [hmax] vacancy of an uncached bin
     short *s_vcache = s_vac + hmax;                         // [vc_bins][hmax] vacancy of bins < vc_bins
@@ -650,10 +649,11 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
     const long long wg = blockIdx.x;
     unsigned char *grids = Wk.grids + wg * (long long)Wk.max_bins * hmax * Wk.pitch;
     int *vacs = Wk.vac + wg * (long long)Wk.max_bins * hmax;
-    double *b_area = Wk.bin_area + wg * Wk.max_bins;
-    double *b_pt = Wk.bin_pt + wg * Wk.max_bins;
-    double *b_pth = Wk.bin_pth + wg * Wk.max_bins;
-    int *b_minrow = Wk.bin_minrow + wg * Wk.max_bins;
+    // bin bookkeeping of the running context (a context opens at most n bins)
+    LDSP double *b_area = (LDSP double *)(smem + Wk.bins_off);
+    LDSP double *b_pt = b_area + n;
+    LDSP double *b_pth = b_pt + n;
+    LDSP int *b_minrow = (LDSP int *)(b_pth + n);
 
     unsigned long long c_ctx = 0, c_place = 0, c_bins = 0, c_tests = 0, c_groups = 0, c_mfma = 0,
                        c_algo = 0, c_over = 0;
@@ -687,7 +687,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
             int rank = 0;
             for (int k = 0; k < n; k++)
                 if (s_mach[k] == mach && (s_sv[k] < kj || (s_sv[k] == kj && k < j))) rank++;
-            s_seq[rank] = j;
+            s_ty[rank] = P.slot_type[j];
             if (!P.fits[(long long)mach * P.n_types + P.slot_type[j]]) s_infeasible = 1;
         }
         __syncthreads();
@@ -705,7 +705,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
         c_ctx++;
         bool overflow = false;
         for (int i = 0; i < cnt; i++) {
-            const int ty = uni(P.slot_type[uni(s_seq[i])]);
+            const int ty = uni(s_ty[i]);
             const double parea = P.area[ty];
             const int nr = P.nrot[ty];
             const int rb = P.rot_base[ty];
@@ -1081,9 +1081,8 @@ static int validate(const brkga_problem *p, std::string &msg) {
 static int device_workspace(brkga_handle *h, int max_bins) {
     DevWork &w = h->dw;
     const size_t nwg = h->n_wg;
-    if (w.grids) {   // free the previous workspace (only grids/vac/bin arrays are resized)
-        for (void *p : {(void *)w.grids, (void *)w.vac, (void *)w.bin_area, (void *)w.bin_pt,
-                        (void *)w.bin_pth, (void *)w.bin_minrow}) {
+    if (w.grids) {   // free the previous workspace (only grids/vac are resized)
+        for (void *p : {(void *)w.grids, (void *)w.vac}) {
             auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
             if (it != h->allocs.end()) { (void)hipFree(p); h->allocs.erase(it); }
         }
@@ -1095,14 +1094,6 @@ static int device_workspace(brkga_handle *h, int max_bins) {
     w.grids = (unsigned char *)p;
     if ((rc = alloc_dev(h, nwg * max_bins * (size_t)w.hmax * sizeof(int), &p))) return rc;
     w.vac = (int *)p;
-    if ((rc = alloc_dev(h, nwg * max_bins * sizeof(double), &p))) return rc;
-    w.bin_area = (double *)p;
-    if ((rc = alloc_dev(h, nwg * max_bins * sizeof(double), &p))) return rc;
-    w.bin_pt = (double *)p;
-    if ((rc = alloc_dev(h, nwg * max_bins * sizeof(double), &p))) return rc;
-    w.bin_pth = (double *)p;
-    if ((rc = alloc_dev(h, nwg * max_bins * sizeof(int), &p))) return rc;
-    w.bin_minrow = (int *)p;
     return 0;
 }
 
@@ -1246,6 +1237,8 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     fixed += (qwords * 4 + 15) & ~(size_t)15;
     w.zw_off = (int)fixed;
     fixed += (size_t)2 * ZW_PASS * sizeof(unsigned long long);
+    w.bins_off = (int)fixed;
+    fixed += ((size_t)p->n_parts * (3 * sizeof(double) + sizeof(int)) + 15) & ~(size_t)15;
     size_t need_full = 0, need_min = 0;
     for (int m = 0; m < M; m++)
         for (int r = 0; r < R; r++) {
