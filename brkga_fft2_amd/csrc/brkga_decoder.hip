@@ -623,6 +623,174 @@ __device__ __forceinline__ void vacancy_words(unsigned long long (&cw)[8], lds_c
     }
 }
 
+// Densities (longest 1-run per mask row) of rotations rb..rb+nr-1 into
+// s_den[r * hrot + a]: one batched pass of the whole workgroup (the caller
+// synchronises before the vacancy filter reads them).
+__device__ __forceinline__ void load_densities(const DevProblem &P, int rb, int nr, int hrot, int *s_den) {
+    int hr[MAX_ROT];
+    long long dof[MAX_ROT];
+#pragma unroll
+    for (int r = 0; r < MAX_ROT; r++) {
+        hr[r] = r < nr ? P.rot_h[rb + r] : 0;
+        dof[r] = r < nr ? P.dens_off[rb + r] : 0;
+    }
+    const DivBy byh(hrot);
+    wg_copy<int>(nr * hrot,
+                 [&](int k) {
+                     const int r = byh(k), a = k - r * hrot;
+                     int h = hr[0];
+                     long long o = dof[0];
+#pragma unroll
+                     for (int q = 1; q < MAX_ROT; q++)
+                         if (r == q) { h = hr[q]; o = dof[q]; }
+                     return a < h ? P.dens[o + a] : 0;
+                 },
+                 [&](int k, int v) { s_den[k] = v; });
+}
+
+// Vacancy filter of a part's rotations against one bin, wave r: rotation r.
+// A top row y can only hold the part if every grid row y+a has a free run >=
+// the mask row's longest 1-run (numba_utils.py:131-157 applies the same test
+// to decide whether the bin is tried).  The candidate top rows, descending,
+// go to s_cand[r * hmax ...] and their count to s_ncand[r]; the caller
+// synchronises.  Dense mode lists every row of the valid region (same results).
+template <bool DENSE>
+__device__ __forceinline__ void vacancy_filter(const DevProblem &P, int rb, int nr, int H, int W, const short *vrow,
+                                               const int *s_den, int hrot, short *s_cand, int hmax, int *s_ncand) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (wave < nr) {
+        const int rg = rb + wave;
+        const int h = P.rot_h[rg], w = P.rot_w[rg];
+        short *cand = s_cand + wave * hmax;
+        const int *den = s_den + wave * hrot;
+        int nc_ = 0;
+        if (h <= H && w <= W) {
+            // Bitset form: word k holds top rows y in [64k, 64k+64); for
+            // mask row a, lane l tests grid row 64k + a + l, so the ballot
+            // is already aligned with the candidate word.  Four mask rows
+            // per step keep many LDS reads in flight.
+            const int ny = H - h + 1;
+            const int nk = (ny + 63) >> 6;   // <= 8 for plates <= 512 rows
+            unsigned long long cw[8];
+            switch (nk) {
+                case 1: vacancy_words<1>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                case 2: vacancy_words<2>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                case 3: vacancy_words<3>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                case 4: vacancy_words<4>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                case 5: vacancy_words<5>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                case 6: vacancy_words<6>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                case 7: vacancy_words<7>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+                default: vacancy_words<8>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
+            }
+            for (int k = nk - 1; k >= 0; k--) {   // descending y
+                const int y = (k << 6) + 63 - lane;
+                const bool ok = (cw[k] >> (63 - lane)) & 1ull;
+                const unsigned long long bal = __ballot(ok);
+                const int pos = __popcll(bal & ((1ull << lane) - 1ull));
+                if (ok) cand[nc_ + pos] = (short)y;
+                nc_ += __popcll(bal);
+            }
+            if (DENSE && nc_ > 0) {   // full valid region, bottom-up
+                nc_ = H - h + 1;
+                for (int k = lane; k < nc_; k += 64) cand[k] = (short)(H - h - k);
+            }
+        }
+        if (lane == 0) s_ncand[wave] = nc_;
+    }
+}
+
+// Collision tests of a part's rotations against one bin (one rotation at a
+// time, whole workgroup) and the choice among them: smallest new enclosure
+// length (the density), then larger bottom row, then smaller column, the
+// first rotation on ties (binClassNew.py:101-117).  Returns the rotation, or
+// -1, with its top-left (by, bx); uniform across the workgroup.
+template <bool DENSE>
+__device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork &Wk, int mach, int rb, int nr,
+                                              int H, int W, int enc, const unsigned char *grid,
+                                              const short *s_cand, const int *s_ncand, int hmax, lds_u8 *band,
+                                              lds_u32 *s_q, lds_u64 *s_zw, unsigned long long &c_tests,
+                                              unsigned long long &c_algo, unsigned long long &c_groups,
+                                              unsigned long long &c_mfma, unsigned long long *ph, int &by, int &bx) {
+    int best = -1, bl = 0, brow = 0;
+    for (int r = 0; r < nr; r++) {
+        const int ncand = uni(s_ncand[r]);
+        if (ncand == 0) continue;
+        const int rg = rb + r;
+        const int h = P.rot_h[rg], w = P.rot_w[rg];
+        c_tests++;
+        c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
+        const short *cand = s_cand + r * hmax;
+        const int ncr = P.rot_nc[rg];
+        const unsigned char *tc = P.tiles + ((long long)mach * TUNE_NC + min(ncr, TUNE_NC - 1)) * 2;
+        const int key = test_rotation<DENSE>(grid, Wk.pitch, W, h, w, ncr,
+                                             P.qbits + P.qbits_off[rg], cand, ncand,
+                                             band, Wk.band_bytes, s_q, s_zw, c_groups, c_mfma, ph,
+                                             uni(tc[0]), uni(tc[1]));
+        if (key == KEY_NONE) continue;
+        // density (new enclosure length) -> row -> col (binClassNew.py:101-117)
+        const int y = uni(cand[key / XKEY]), x = key % XKEY, row = y + h - 1;
+        const int nl = enc > H - y ? enc : H - y;
+        if (best < 0 || nl < bl || (nl == bl && row > brow) || (nl == bl && row == brow && x < bx)) {
+            best = r; bl = nl; brow = row; bx = x; by = y;
+        }
+    }
+    return best;
+}
+
+// Stamp rotation rg at top-left (py, px) into a bin (binClassNew.py:155-160,
+// cuda_batch_update.py:90-126), whole workgroup: rows [py, py+h) pass through
+// the LDS band in chunks -- load (a fresh bin's rows are zero), stamp the mask
+// (a wave per row, a lane per cell), write back, recompute the rows' free runs
+// into vv (HBM) and into vcache when it is not null.
+__device__ __forceinline__ void stamp_part(const DevProblem &P, const DevWork &Wk, int rg, int W, unsigned char *grid,
+                                           int *vv, short *vcache, bool fresh, int py, int px, lds_u8 *band,
+                                           lds_u32 *s_q) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int h = P.rot_h[rg], w = P.rot_w[rg];
+    // rows [py, py+h) pass through LDS in chunks: load (a fresh bin's
+    // rows are zero), stamp the mask, write back, recompute free runs
+    const int wq = (W + 15) >> 4;               // 16-byte columns per row
+    const DivBy bywq(wq);
+    const int chunk = min(h, Wk.band_bytes / (wq << 4));
+    for (int a0 = 0; a0 < h; a0 += chunk) {
+        const int rows = min(chunk, h - a0);
+        if (fresh) {
+            for (int k = tid; k < rows * wq; k += WG_THREADS)
+                *(lds_v4i *)(band + (k << 4)) = (v4i){0, 0, 0, 0};
+        } else {
+            const unsigned char *g0 = grid + (long long)(py + a0) * Wk.pitch;
+            wg_copy(rows * wq,
+                    [&](int k) { const int r = bywq(k), q = k - r * wq;
+                                 return *(const v4i *)(g0 + (long long)r * Wk.pitch + (q << 4)); },
+                    [&](int k, v4i v) { *(lds_v4i *)(band + (k << 4)) = v; });
+        }
+        const int nq = P.rot_nc[rg] + 1;   // mask bits of these rows -> LDS
+        const unsigned *qg = P.qbits + P.qbits_off[rg] + (long long)a0 * nq;
+        wg_copy<unsigned>(rows * nq, [&](int k) { return qg[k]; },
+                          [&](int k, unsigned v) { s_q[k] = v; });
+        __syncthreads();
+        for (int r = wave; r < rows; r += 4)   // wave per row, lane per cell
+            for (int bcol = lane; bcol < w; bcol += 64) {
+                const unsigned bits = s_q[r * nq + ((bcol + 32) >> 5)];
+                if ((bits >> ((bcol + 32) & 31)) & 1u) band[((r * wq) << 4) + px + bcol] = 1;
+            }
+        __syncthreads();
+        for (int k = tid; k < rows * wq; k += WG_THREADS) {
+            const int r = bywq(k), q = k - r * wq;
+            *(v4i *)(grid + (long long)(py + a0 + r) * Wk.pitch + (q << 4)) =
+                *(lds_cv4i *)(band + ((r * wq + q) << 4));
+        }
+        for (int r = wave; r < rows; r += 4) {   // numba_utils.py:161-188
+            const int v = row_free_run(band + ((r * wq) << 4), W);
+            if (lane == 0) {
+                vv[py + a0 + r] = v;
+                if (vcache) vcache[py + a0 + r] = (short)v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 template <bool DENSE>
 __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, DevWork Wk, LaunchArgs L) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -710,29 +878,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
             const int nr = P.nrot[ty];
             const int rb = P.rot_base[ty];
             const int nb = uni(s_nb);
-            // densities of this part's rotations -> LDS, one batched pass
-            {
-                int hr[MAX_ROT];
-                long long dof[MAX_ROT];
-#pragma unroll
-                for (int r = 0; r < MAX_ROT; r++) {
-                    hr[r] = r < nr ? P.rot_h[rb + r] : 0;
-                    dof[r] = r < nr ? P.dens_off[rb + r] : 0;
-                }
-                const int hrot = Wk.hrot;
-                const DivBy byh(hrot);
-                wg_copy<int>(nr * hrot,
-                             [&](int k) {
-                                 const int r = byh(k), a = k - r * hrot;
-                                 int h = hr[0];
-                                 long long o = dof[0];
-#pragma unroll
-                                 for (int q = 1; q < MAX_ROT; q++)
-                                     if (r == q) { h = hr[q]; o = dof[q]; }
-                                 return a < h ? P.dens[o + a] : 0;
-                             },
-                             [&](int k, int v) { s_den[k] = v; });
-            }
+            load_densities(P, rb, nr, Wk.hrot, s_den);
             __syncthreads();   // s_den is read by every wave's vacancy filter
             bool placed = false;
             int pb = 0, py = 0, px = 0, prot = 0;
@@ -749,75 +895,14 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                     __syncthreads();
                 }
                 // ---- vacancy filter, per candidate top row (wave r: rotation r) ----
-                // A top row y can only hold the part if every grid row y+a has a
-                // free run >= the mask row's longest 1-run (numba_utils.py:131-157
-                // applies the same test to decide whether the bin is tried).
-                if (wave < nr) {
-                    const int rg = rb + wave;
-                    const int h = P.rot_h[rg], w = P.rot_w[rg];
-                    short *cand = s_cand + wave * hmax;
-                    const int *den = s_den + wave * Wk.hrot;
-                    int nc_ = 0;
-                    if (h <= H && w <= W) {
-                        // Bitset form: word k holds top rows y in [64k, 64k+64); for
-                        // mask row a, lane l tests grid row 64k + a + l, so the ballot
-                        // is already aligned with the candidate word.  Four mask rows
-                        // per step keep many LDS reads in flight.
-                        const int ny = H - h + 1;
-                        const int nk = (ny + 63) >> 6;   // <= 8 for plates <= 512 rows
-                        unsigned long long cw[8];
-                        switch (nk) {
-                            case 1: vacancy_words<1>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                            case 2: vacancy_words<2>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                            case 3: vacancy_words<3>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                            case 4: vacancy_words<4>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                            case 5: vacancy_words<5>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                            case 6: vacancy_words<6>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                            case 7: vacancy_words<7>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                            default: vacancy_words<8>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                        }
-                        for (int k = nk - 1; k >= 0; k--) {   // descending y
-                            const int y = (k << 6) + 63 - lane;
-                            const bool ok = (cw[k] >> (63 - lane)) & 1ull;
-                            const unsigned long long bal = __ballot(ok);
-                            const int pos = __popcll(bal & ((1ull << lane) - 1ull));
-                            if (ok) cand[nc_ + pos] = (short)y;
-                            nc_ += __popcll(bal);
-                        }
-                        if (DENSE && nc_ > 0) {   // full valid region, bottom-up
-                            nc_ = H - h + 1;
-                            for (int k = lane; k < nc_; k += 64) cand[k] = (short)(H - h - k);
-                        }
-                    }
-                    if (lane == 0) s_ncand[wave] = nc_;
-                }
+                vacancy_filter<DENSE>(P, rb, nr, H, W, vrow, s_den, Wk.hrot, s_cand, hmax, s_ncand);
                 __syncthreads();
                 if (tid == 0) ph[1] += tick() - tp;
                 // ---- collision tests, one rotation at a time (whole workgroup) ----
-                int best = -1, bl = 0, brow = 0, bx = 0, by = 0;
-                const int enc = H - b_minrow[b];
-                for (int r = 0; r < nr; r++) {
-                    const int ncand = uni(s_ncand[r]);
-                    if (ncand == 0) continue;
-                    const int rg = rb + r;
-                    const int h = P.rot_h[rg], w = P.rot_w[rg];
-                    c_tests++;
-                    c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
-                    const short *cand = s_cand + r * hmax;
-                    const int ncr = P.rot_nc[rg];
-                    const unsigned char *tc = P.tiles + ((long long)mach * TUNE_NC + min(ncr, TUNE_NC - 1)) * 2;
-                    const int key = test_rotation<DENSE>(grid, Wk.pitch, W, h, w, ncr,
-                                                         P.qbits + P.qbits_off[rg], cand, ncand,
-                                                         band, Wk.band_bytes, s_q, s_zw, c_groups, c_mfma, ph,
-                                                         uni(tc[0]), uni(tc[1]));
-                    if (key == KEY_NONE) continue;
-                    // density (new enclosure length) -> row -> col (binClassNew.py:101-117)
-                    const int y = uni(cand[key / XKEY]), x = key % XKEY, row = y + h - 1;
-                    const int nl = enc > H - y ? enc : H - y;
-                    if (best < 0 || nl < bl || (nl == bl && row > brow) || (nl == bl && row == brow && x < bx)) {
-                        best = r; bl = nl; brow = row; bx = x; by = y;
-                    }
-                }
+                int bx = 0, by = 0;
+                const int best = best_placement<DENSE>(P, Wk, mach, rb, nr, H, W, H - b_minrow[b], grid, s_cand,
+                                                       s_ncand, hmax, band, s_q, s_zw, c_tests, c_algo, c_groups,
+                                                       c_mfma, ph, by, bx);
                 if (best >= 0) {
                     placed = true;
                     pb = b; prot = best; py = by; px = bx;
@@ -849,51 +934,9 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
             // ---- stamp the mask (binClassNew.py:155-160, cuda_batch_update.py:90-126) ----
             {
                 const int rg = rb + prot;
-                const int h = P.rot_h[rg], w = P.rot_w[rg];
-                unsigned char *grid = grids + (long long)pb * hmax * Wk.pitch;
-                // rows [py, py+h) pass through LDS in chunks: load (a fresh bin's
-                // rows are zero), stamp the mask, write back, recompute free runs
-                const int wq = (W + 15) >> 4;               // 16-byte columns per row
-                const DivBy bywq(wq);
-                const int chunk = min(h, Wk.band_bytes / (wq << 4));
-                int *vv = vacs + (long long)pb * hmax;
-                for (int a0 = 0; a0 < h; a0 += chunk) {
-                    const int rows = min(chunk, h - a0);
-                    if (fresh) {
-                        for (int k = tid; k < rows * wq; k += WG_THREADS)
-                            *(lds_v4i *)(band + (k << 4)) = (v4i){0, 0, 0, 0};
-                    } else {
-                        const unsigned char *g0 = grid + (long long)(py + a0) * Wk.pitch;
-                        wg_copy(rows * wq,
-                                [&](int k) { const int r = bywq(k), q = k - r * wq;
-                                             return *(const v4i *)(g0 + (long long)r * Wk.pitch + (q << 4)); },
-                                [&](int k, v4i v) { *(lds_v4i *)(band + (k << 4)) = v; });
-                    }
-                    const int nq = P.rot_nc[rg] + 1;   // mask bits of these rows -> LDS
-                    const unsigned *qg = P.qbits + P.qbits_off[rg] + (long long)a0 * nq;
-                    wg_copy<unsigned>(rows * nq, [&](int k) { return qg[k]; },
-                                      [&](int k, unsigned v) { s_q[k] = v; });
-                    __syncthreads();
-                    for (int r = wave; r < rows; r += 4)   // wave per row, lane per cell
-                        for (int bcol = lane; bcol < w; bcol += 64) {
-                            const unsigned bits = s_q[r * nq + ((bcol + 32) >> 5)];
-                            if ((bits >> ((bcol + 32) & 31)) & 1u) band[((r * wq) << 4) + px + bcol] = 1;
-                        }
-                    __syncthreads();
-                    for (int k = tid; k < rows * wq; k += WG_THREADS) {
-                        const int r = bywq(k), q = k - r * wq;
-                        *(v4i *)(grid + (long long)(py + a0 + r) * Wk.pitch + (q << 4)) =
-                            *(lds_cv4i *)(band + ((r * wq + q) << 4));
-                    }
-                    for (int r = wave; r < rows; r += 4) {   // numba_utils.py:161-188
-                        const int v = row_free_run(band + ((r * wq) << 4), W);
-                        if (lane == 0) {
-                            vv[py + a0 + r] = v;
-                            if (pb < Wk.vc_bins) s_vcache[pb * hmax + py + a0 + r] = (short)v;
-                        }
-                    }
-                    __syncthreads();
-                }
+                const int h = P.rot_h[rg];
+                stamp_part(P, Wk, rg, W, grids + (long long)pb * hmax * Wk.pitch, vacs + (long long)pb * hmax,
+                           pb < Wk.vc_bins ? s_vcache + pb * hmax : nullptr, fresh, py, px, band, s_q);
                 if (tid == 0) {   // binClassNew.py:147-153,172-174 ; placement.py:81-82
                     const long long mp = (long long)mach * P.n_types + ty;
                     b_area[pb] = b_area[pb] + parea;
