@@ -125,6 +125,33 @@ class BRKGA:
         return "feasible"
 
 
+def initial_solution(problem_data, nbParts, nbMachines, thresholds, instanceParts, decoder=None,
+                     seed=None, with_log=False):
+    """Greedy initial solution of ``BRKGA_alg3.py:445-603`` on the GPU.
+
+    Returns ``(array, makespan)`` like the reference's ``array`` (float64
+    ``[2n]``, the ``initialSol`` argument of ``BRKGA``) and its printed
+    "Makespan of initial solution"; with ``with_log`` also the placements.
+    The reference draws each machine's key with ``random.uniform``; here a
+    ``numpy`` generator seeded with ``seed`` draws it inside the same bucket
+    (any such key decodes identically).  ``decoder``: an open ``Decoder`` of
+    this problem, else one is created for the call.
+    """
+    from .native import Decoder
+    from .problem import initial_machine_keys, pack_problem, type_heights
+    packed = pack_problem(problem_data, nbParts, nbMachines, thresholds, instanceParts)
+    own = decoder is None
+    dec = Decoder(packed) if own else decoder
+    try:
+        mk = initial_machine_keys(thresholds, nbMachines, np.random.default_rng(seed))
+        keys, ms, log = dec.initial_solution(type_heights(problem_data, packed), mk, with_log=with_log)
+    finally:
+        if own:
+            dec.close()
+    out = (keys.astype(np.float64), ms)
+    return out + (log,) if with_log else out
+
+
 def brkga_sizes(nb_parts, mult=10):
     """Population sizes of the reference's entry point (``BRKGA_alg3.py:608-618``)."""
     n = mult * nb_parts

@@ -989,6 +989,79 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
 }
 
 // fitness = max over machines, 1e16 if any machine infeasible (placement.py:147-156)
+// ---------------------------------------------------------------------------
+// Greedy initial solution (BRKGA_alg3.py:445-603): the host runs the greedy;
+// these kernels test one part against every open batch at once and stamp the
+// chosen placement.  Batches live in [machine][batch] slots of grids/vac.
+// ---------------------------------------------------------------------------
+struct GreedyArgs {
+    unsigned char *grids;   // [M][max_batches][hmax][pitch]
+    int *vac;               // [M][max_batches][hmax]
+    int max_batches;
+    int ty;                 // part type under test
+    const int *jobs;        // [n_jobs][3]: machine, batch, enclosure length
+    int *res;               // [n_jobs][4]: found, col, bottom row, rotation
+};
+
+// One workgroup per (machine, batch): binClassInitialSol.py:41-111 can_insert
+// -- per rotation the vacancy filter and the bottom-left collision test, then
+// the density -> row -> col choice (no area check; the greedy has none).
+__global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P, DevWork Wk, GreedyArgs A) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int n = P.n_parts, hmax = Wk.hmax, tid = threadIdx.x;
+    int *s_den = (int *)((double *)smem + n) + 2 * n;       // same layout as decode_kernel
+    short *s_cand = (short *)(s_den + MAX_ROT * Wk.hrot);
+    short *s_vac = s_cand + MAX_ROT * hmax;
+    lds_u32 *s_q = (lds_u32 *)(smem + Wk.q_off);
+    lds_u8 *band = (lds_u8 *)(smem + Wk.band_off);
+    lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);
+    __shared__ int s_ncand[MAX_ROT];
+    const int *job = A.jobs + 3 * blockIdx.x;
+    const int m = job[0], b = job[1], enc = job[2], ty = A.ty;
+    const int H = P.plate_h[m], W = P.plate_w[m], nr = P.nrot[ty], rb = P.rot_base[ty];
+    const long long slot = (long long)m * A.max_batches + b;
+    const unsigned char *grid = A.grids + slot * hmax * Wk.pitch;
+    const int *vac = A.vac + slot * hmax;
+    load_densities(P, rb, nr, Wk.hrot, s_den);
+    for (int y = tid; y < H; y += WG_THREADS) s_vac[y] = (short)vac[y];
+    __syncthreads();
+    vacancy_filter<false>(P, rb, nr, H, W, s_vac, s_den, Wk.hrot, s_cand, hmax, s_ncand);
+    __syncthreads();
+    unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int bx = 0, by = 0;
+    const int best = best_placement<false>(P, Wk, m, rb, nr, H, W, enc, grid, s_cand, s_ncand, hmax, band, s_q,
+                                           s_zw, c0, c1, c2, c3, ph, by, bx);
+    if (tid == 0) {
+        int *r = A.res + 4 * blockIdx.x;
+        r[0] = best >= 0;
+        r[1] = bx;
+        r[2] = best >= 0 ? by + P.rot_h[rb + best] - 1 : 0;
+        r[3] = best;
+    }
+}
+
+struct GreedyInsert {
+    unsigned char *grid;
+    int *vac;
+    int rg, H, W, py, px, fresh;
+};
+
+// Stamp the chosen placement (binClassInitialSol.py:122-145); a new batch is
+// zeroed first (empty grid, full free runs).
+__global__ __launch_bounds__(WG_THREADS, 2) void greedy_insert_kernel(DevProblem P, DevWork Wk, GreedyInsert I) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    lds_u32 *s_q = (lds_u32 *)(smem + Wk.q_off);
+    lds_u8 *band = (lds_u8 *)(smem + Wk.band_off);
+    if (I.fresh) {
+        v4i *g4 = (v4i *)I.grid;
+        const int nvec = I.H * Wk.pitch / 16;
+        for (int k = threadIdx.x; k < nvec; k += WG_THREADS) g4[k] = (v4i){0, 0, 0, 0};
+        for (int k = threadIdx.x; k < I.H; k += WG_THREADS) I.vac[k] = I.W;
+        __syncthreads();
+    }
+    stamp_part(P, Wk, I.rg, I.W, I.grid, I.vac, nullptr, I.fresh != 0, I.py, I.px, band, s_q);
+}
+
 __global__ void finalize_kernel(const double *ctx_ms, long long n_chrom, int M, double *out,
                                 unsigned long long *counters) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1213,6 +1286,16 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
             const int h0 = p->rot_h[r0], w0 = p->rot_w[r0];
             fits[(size_t)m * T + t] = !((h0 > p->plate_h[m] || w0 > p->plate_w[m]) &&
                                         (w0 > p->plate_h[m] || h0 > p->plate_w[m]));
+            // a new bin takes the best rotation at (0, H-1) (placement.py:66-83);
+            // the reference fails when that rotation is larger than the plate
+            const int rb = r0 + p->best_rot[t];
+            if (fits[(size_t)m * T + t] && (p->rot_h[rb] > p->plate_h[m] || p->rot_w[rb] > p->plate_w[m])) {
+                char b2[160];
+                snprintf(b2, sizeof b2, "part type %d fits machine %d only in a rotation other than its best "
+                         "rotation, which a new bin would not hold", t, m);
+                h->err = b2;
+                return bail(BRKGA_EINVAL);
+            }
         }
     int64_t mask_bytes = 0;
     for (int r = 0; r < R; r++) mask_bytes = std::max<int64_t>(mask_bytes, p->mask_off[r] + (int64_t)p->rot_h[r] * p->rot_w[r]);
@@ -1318,8 +1401,9 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     h->smem = fixed + w.band_bytes;
     h->n_wg = std::max(1, h->n_cu * wpc2);   // persistent grid at the occupancy the LDS allows
     if ((rc = device_workspace(h, mb))) return bail(rc);
-    for (auto kern : {decode_kernel<false>, decode_kernel<true>})
-        if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->smem) != hipSuccess) {
+    for (const void *kern : {(const void *)decode_kernel<false>, (const void *)decode_kernel<true>,
+                             (const void *)greedy_test_kernel, (const void *)greedy_insert_kernel})
+        if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->smem) != hipSuccess) {
             h->err = "hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed";
             return bail(BRKGA_EDEVICE);
         }
@@ -1490,6 +1574,166 @@ extern "C" int brkga_get_tiles(const brkga_handle *h, int32_t *tiles_per_pass, i
         if (tiles_per_pass) tiles_per_pass[i] = h->tiles[i * 2];
         if (window_tiles) window_tiles[i] = h->tiles[i * 2 + 1];
     }
+    return BRKGA_OK;
+}
+
+// Greedy initial solution: the loop of BRKGA_alg3.py:445-603 on the host, the
+// per-part tests of all open batches in one launch, the stamp in another.
+extern "C" int brkga_initial_solution(brkga_handle *h, const double *type_height, const double *machine_keys,
+                                      float *out_chromosome, double *out_makespan, int32_t *out_log,
+                                      void *stream) {
+    if (!h) return fail(nullptr, BRKGA_EINVAL, "handle is NULL");
+    if (!type_height || !machine_keys || !out_chromosome || !out_makespan)
+        return fail(h, BRKGA_EINVAL, "type_height, machine_keys, out_chromosome and out_makespan are required");
+    const int n = h->prob.n_parts, M = h->prob.n_machines, T = h->prob.n_types, R = h->prob.n_rots;
+    const int hmax = h->dw.hmax;
+    const size_t pitch = (size_t)h->dw.pitch;
+    hipStream_t s = (hipStream_t)stream;
+    HIPCHK(h, hipSetDevice(h->device));
+    // host copies of the tables the greedy's decisions read
+    std::vector<int> slot_type(n), type_id(T), nrot(T), rot_base(T), best_rot(T), rot_h(R), rot_w(R), ph(M), pw(M);
+    std::vector<double> area(T), setup(M), pt((size_t)M * T), pth((size_t)M * T);
+    const DevProblem &d = h->dp;
+    HIPCHK(h, hipMemcpy(slot_type.data(), d.slot_type, n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(type_id.data(), d.type_id, T * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(nrot.data(), d.nrot, T * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(rot_base.data(), d.rot_base, T * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(best_rot.data(), d.best_rot, T * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(rot_h.data(), d.rot_h, R * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(rot_w.data(), d.rot_w, R * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(ph.data(), d.plate_h, M * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(pw.data(), d.plate_w, M * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(area.data(), d.area, T * 8, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(setup.data(), d.setup, M * 8, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(pt.data(), d.proc_time, (size_t)M * T * 8, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(pth.data(), d.pth, (size_t)M * T * 8, hipMemcpyDeviceToHost));
+    // slots by decreasing height, then decreasing area, ties in instance order
+    std::vector<int> order(n);
+    for (int j = 0; j < n; j++) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const int ta = slot_type[a], tb = slot_type[b];
+        if (type_height[ta] != type_height[tb]) return type_height[ta] > type_height[tb];
+        return area[ta] > area[tb];
+    });
+    // device batches: every machine may open up to n
+    const size_t nslot = (size_t)M * n;
+    unsigned char *d_grids = nullptr;
+    int *d_vac = nullptr, *d_jobs = nullptr, *d_res = nullptr;
+    int rc = BRKGA_OK;
+    auto cleanup = [&]() {
+        if (d_grids) (void)hipFree(d_grids);
+        if (d_vac) (void)hipFree(d_vac);
+        if (d_jobs) (void)hipFree(d_jobs);
+        if (d_res) (void)hipFree(d_res);
+    };
+    if (hipMalloc((void **)&d_grids, nslot * hmax * pitch) != hipSuccess ||
+        hipMalloc((void **)&d_vac, nslot * hmax * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&d_jobs, nslot * 3 * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&d_res, nslot * 4 * sizeof(int)) != hipSuccess) {
+        cleanup();
+        return fail(h, BRKGA_ENOMEM, "greedy workspace allocation failed");
+    }
+    struct Batch { double pt = 0.0, pth = 0.0; int minrow = 0; std::vector<int> parts; };
+    std::vector<std::vector<Batch>> batches(M);
+    std::vector<double> mk(M, 0.0);
+    std::vector<int> jobs, res;
+    double worst = 0.0, best_ms = 0.0;
+    // one part; device errors return from here so the workspace is still freed
+    auto place = [&](int k) -> int {
+        const int ty = slot_type[order[k]], r0 = rot_base[ty];
+        const int h0 = rot_h[r0], w0 = rot_w[r0];
+        auto fits = [&](int m) { return !((h0 > ph[m] || w0 > pw[m]) && (w0 > ph[m] || h0 > pw[m])); };
+        // test the part against every open batch of every machine it fits
+        jobs.clear();
+        for (int m = 0; m < M; m++)
+            if (fits(m))
+                for (int b = 0; b < (int)batches[m].size(); b++) {
+                    jobs.push_back(m); jobs.push_back(b); jobs.push_back(ph[m] - batches[m][b].minrow);
+                }
+        const int nj = (int)jobs.size() / 3;
+        res.assign((size_t)nj * 4, 0);
+        if (nj) {
+            GreedyArgs A;
+            A.grids = d_grids; A.vac = d_vac; A.max_batches = n; A.ty = ty; A.jobs = d_jobs; A.res = d_res;
+            HIPCHK(h, hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(greedy_test_kernel, dim3(nj), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, A);
+            HIPCHK(h, hipGetLastError());
+            HIPCHK(h, hipMemcpyAsync(res.data(), d_res, res.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHK(h, hipStreamSynchronize(s));
+        }
+        // the choice (BRKGA_alg3.py:467-507): "<=" lets later candidates win ties,
+        // except a second equal existing batch of the same machine
+        best_ms = INFINITY;
+        int kind = 0, cm = -1, cb = -1, cx = 0, crow = 0, crot = 0, jo = 0;
+        double cmm = 0.0;
+        for (int m = 0; m < M; m++) {
+            if (!fits(m)) continue;
+            bool placed_exist = false;
+            const double ptm = pt[(size_t)m * T + ty], pthm = pth[(size_t)m * T + ty];
+            double mm = ((mk[m] + setup[m]) + ptm) + pthm;
+            double nm = worst > mm ? worst : mm;
+            if (nm <= best_ms) { best_ms = nm; kind = 1; cm = m; cmm = mm; }
+            for (int b = 0; b < (int)batches[m].size(); b++, jo++) {
+                const int *r = &res[(size_t)jo * 4];
+                if (!r[0]) continue;
+                const double bpth = batches[m][b].pth;
+                mm = bpth < pthm ? ((mk[m] - bpth) + pthm) + ptm : mk[m] + ptm;
+                nm = worst > mm ? worst : mm;
+                if (nm <= best_ms) {
+                    if (placed_exist && nm == best_ms) continue;
+                    placed_exist = true; best_ms = nm; kind = 2; cm = m; cb = b;
+                    cx = r[1]; crow = r[2]; crot = r[3]; cmm = mm;
+                }
+            }
+        }
+        if (kind == 0) return fail(h, BRKGA_EINVAL, "a part fits no machine");
+        const int H = ph[cm];
+        if (kind == 1) {
+            cb = (int)batches[cm].size();
+            batches[cm].emplace_back();
+            batches[cm][cb].minrow = H;
+            crot = best_rot[ty]; cx = 0; crow = H - 1;
+        }
+        const int rg = r0 + crot, hh = rot_h[rg];
+        GreedyInsert I;
+        const size_t slot = (size_t)cm * n + cb;
+        I.grid = d_grids + slot * hmax * pitch; I.vac = d_vac + slot * hmax;
+        I.rg = rg; I.H = H; I.W = pw[cm]; I.py = crow - hh + 1; I.px = cx; I.fresh = kind == 1;
+        hipLaunchKernelGGL(greedy_insert_kernel, dim3(1), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, I);
+        HIPCHK(h, hipGetLastError());
+        Batch &bt = batches[cm][cb];
+        bt.pt += pt[(size_t)cm * T + ty];
+        if (pth[(size_t)cm * T + ty] > bt.pth) bt.pth = pth[(size_t)cm * T + ty];
+        if (I.py < bt.minrow) bt.minrow = I.py;
+        bt.parts.push_back(type_id[ty]);
+        mk[cm] = cmm;
+        if (cmm > worst) worst = cmm;
+        if (out_log) {
+            int32_t *e = out_log + 6 * k;
+            e[0] = cm; e[1] = type_id[ty]; e[2] = cb; e[3] = cx; e[4] = crow; e[5] = crot;
+        }
+        return BRKGA_OK;
+    };
+    for (int k = 0; k < n && rc == BRKGA_OK; k++) rc = place(k);
+    if (rc == BRKGA_OK) rc = hipStreamSynchronize(s) == hipSuccess ? BRKGA_OK : fail(h, BRKGA_EDEVICE, "greedy kernels failed");
+    cleanup();
+    if (rc != BRKGA_OK) return rc;
+    // the chromosome (BRKGA_alg3.py:555-595)
+    std::vector<char> used(n, 0);
+    for (int j = 0; j < 2 * n; j++) out_chromosome[j] = 0.0f;
+    for (int m = 0; m < M; m++) {
+        std::vector<int> pos;
+        for (const Batch &bt : batches[m])
+            for (int id : bt.parts)
+                for (int j = 0; j < n; j++)
+                    if (!used[j] && type_id[slot_type[j]] == id) { used[j] = 1; pos.push_back(j); break; }
+        const double step = pos.empty() ? 0.0 : 1.0 / (double)pos.size();
+        for (size_t i = 0; i < pos.size(); i++) {
+            out_chromosome[n + pos[i]] = (float)machine_keys[m];
+            out_chromosome[pos[i]] = (float)((double)i * step);
+        }
+    }
+    *out_makespan = best_ms;
     return BRKGA_OK;
 }
 

@@ -59,7 +59,8 @@ class BrkgaGaParams(ctypes.Structure):
 TILE_CLASSES = 16   # BRKGA_TILE_CLASSES: mask width classes nc = ceil(w / 32)
 
 EXPORTED = ["brkga_create", "brkga_evaluate", "brkga_evaluate_device", "brkga_get_stats",
-            "brkga_reset_stats", "brkga_set_tiles", "brkga_get_tiles", "brkga_last_error",
+            "brkga_reset_stats", "brkga_set_tiles", "brkga_get_tiles", "brkga_initial_solution",
+            "brkga_last_error",
             "brkga_destroy",
             "brkga_ga_create", "brkga_ga_run", "brkga_ga_best", "brkga_ga_population",
             "brkga_ga_last_error", "brkga_ga_destroy", "brkga_sort_fitness", "brkga_breed_pool",
@@ -102,6 +103,8 @@ def load_library(path=LIB_PATH):
     lib.brkga_set_tiles.restype = ctypes.c_int
     lib.brkga_get_tiles.argtypes = [_P, _P, _P]
     lib.brkga_get_tiles.restype = ctypes.c_int
+    lib.brkga_initial_solution.argtypes = [_P, _P, _P, _P, _P, _P, _P]
+    lib.brkga_initial_solution.restype = ctypes.c_int
     lib.brkga_last_error.argtypes = [_P]
     lib.brkga_last_error.restype = ctypes.c_char_p
     lib.brkga_destroy.argtypes = [_P]
@@ -144,6 +147,7 @@ class Decoder:
         self._lib = load_library()
         self.n_parts = int(packed["n_parts"])
         self.n_machines = int(packed["n_machines"])
+        self.n_types = int(packed["n_types"])
         self.thresholds = np.asarray(packed["thresholds"], np.float64).reshape(-1)[:max(self.n_machines - 1, 0)]
         keep = []
         prob = BrkgaProblem(self.n_parts, self.n_machines, int(packed["n_types"]),
@@ -216,6 +220,22 @@ class Decoder:
 
     def reset_stats(self):
         self._check(self._lib.brkga_reset_stats(self._h), "brkga_reset_stats")
+
+    def initial_solution(self, type_height, machine_keys, with_log=False, stream=None):
+        """Greedy seed chromosome on the GPU: ``(keys float32[2n], makespan,
+        log int32[n, 6] or None)`` (see ``brkga_initial_solution``)."""
+        th = np.ascontiguousarray(type_height, np.float64)
+        mk = np.ascontiguousarray(machine_keys, np.float64)
+        if th.size != self.n_types or mk.size != self.n_machines:
+            raise ValueError("type_height needs n_types values and machine_keys n_machines values")
+        keys = np.zeros(2 * self.n_parts, np.float32)
+        ms = ctypes.c_double()
+        log = np.full((self.n_parts, 6), -1, np.int32) if with_log else None
+        self._check(self._lib.brkga_initial_solution(self._h, th.ctypes.data, mk.ctypes.data, keys.ctypes.data,
+                                                     ctypes.byref(ms), log.ctypes.data if with_log else None,
+                                                     ctypes.c_void_p(stream.cuda_stream) if stream is not None
+                                                     else None), "brkga_initial_solution")
+        return keys, ms.value, log
 
     def get_tiles(self):
         """Tile table ``(tiles_per_pass, window_tiles)``, int32 ``[n_machines, TILE_CLASSES]``."""

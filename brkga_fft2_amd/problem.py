@@ -41,6 +41,7 @@ class PartData:
     shapes: List[Tuple[int, int]]
     densities: List[np.ndarray]
     best_rotation: int
+    height: float = 0.0   # part height in mm (orders the greedy initial solution)
 
     @property
     def lengths(self) -> List[int]:
@@ -135,6 +136,8 @@ def build_problem(instance_parts: Sequence[int], masks: Dict[int, np.ndarray],
     parts = {int(p): prepare_part(int(p), masks[int(p)], areas[int(p)],
                                   None if nrot is None else nrot[int(p)])
              for p in uniq}
+    for p in uniq:
+        parts[int(p)].height = float(height[int(p)])
     mlist = []
     for m in machines:
         L, W = int(m["L"]), int(m["W"])
@@ -153,6 +156,49 @@ def build_problem(instance_parts: Sequence[int], masks: Dict[int, np.ndarray],
 def thresholds_for(nb_machines: int) -> List[float]:
     """Machine-assignment thresholds ``t / M`` (``BRKGA_alg3.py:442``)."""
     return [t / nb_machines for t in range(1, nb_machines)]
+
+
+def type_heights(problem, packed) -> np.ndarray:
+    """Part height of each packed part type: ``PartData.height`` when set,
+    else the height time on machine 0 (height x HT, the same order)."""
+    out = np.empty(int(packed["n_types"]), np.float64)
+    for t, pid in enumerate(np.asarray(packed["type_id"])):
+        part = problem.parts[int(pid)]
+        h = float(getattr(part, "height", 0.0) or 0.0)
+        out[t] = h if h > 0 else float(problem.machines[0].parts[int(pid)].proc_time_height)
+    return out
+
+
+def initial_solution_order(packed, type_height) -> np.ndarray:
+    """Slots in the greedy initial solution's order: decreasing height, then
+    decreasing polygon area, ties in instance order (a stable sort, as
+    ``sort_values(['height(mm)', 'Area'], ascending=False)`` of
+    ``BRKGA_alg3.py:450-452``)."""
+    st = np.asarray(packed["slot_type"])
+    h = np.asarray(type_height, np.float64)[st]
+    a = np.asarray(packed["area"], np.float64)[st]
+    return np.lexsort((np.arange(st.size), -a, -h)).astype(np.int32)
+
+
+def initial_machine_keys(thresholds, nb_machines, rng) -> np.ndarray:
+    """One machine key per machine inside its threshold bucket, drawn as
+    ``BRKGA_alg3.py:585-593`` draws them (``random.uniform`` there, the
+    generator ``rng`` here): machine 0 in (0, t0), the last in
+    (t_{M-2} + 1e-4, 1 - 1e-4), the others in (t_{m-1} + 1e-4, t_m - 1e-4).
+    Any key inside the bucket decodes to the same machine."""
+    t = list(thresholds)
+    out = np.empty(nb_machines, np.float64)
+    for m in range(nb_machines):
+        if nb_machines == 1:
+            lo, hi = 0.0, 1.0
+        elif m == 0:
+            lo, hi = 0.0, t[0]
+        elif m == nb_machines - 1:
+            lo, hi = t[m - 1] + 0.0001, 1 - 0.0001
+        else:
+            lo, hi = t[m - 1] + 0.0001, t[m] - 0.0001
+        out[m] = rng.uniform(lo, hi)
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -159,6 +159,23 @@ int brkga_reset_stats(brkga_handle *h);
 int brkga_set_tiles(brkga_handle *h, const int32_t *tiles_per_pass, const int32_t *window_tiles);
 int brkga_get_tiles(const brkga_handle *h, int32_t *tiles_per_pass, int32_t *window_tiles);
 
+/*
+ * Greedy initial solution (replaces BRKGA_alg3.py:445-603 and the batch class
+ * of binClassInitialSol.py:41-174).  Parts are taken by decreasing height
+ * (type_height [n_types]), then decreasing area, ties in instance order; each
+ * goes to the new or existing batch, on any machine whose plate fits it, that
+ * keeps the running makespan lowest (the reference's tie rules).  Writes the
+ * seed chromosome (float32 [2n]: per machine, its parts in batch order get
+ * sequence keys i/k and the machine key machine_keys[m], which must lie in
+ * machine m's threshold bucket -- the reference draws it with random.uniform),
+ * the greedy's makespan, and optionally the placements (int32 [n][6]:
+ * machine, part id, batch, col, bottom row, rotation) in greedy order.  All
+ * buffers are host memory; the call is synchronous on `stream` (nullable).
+ * BRKGA_EINVAL when a part fits no machine.
+ */
+int brkga_initial_solution(brkga_handle *h, const double *type_height, const double *machine_keys,
+                           float *out_chromosome, double *out_makespan, int32_t *out_log, void *stream);
+
 /* Message of the last failed call on h (or of the last failed brkga_create
  * when h is NULL).  Valid until the next call on the same handle. */
 const char *brkga_last_error(const brkga_handle *h);

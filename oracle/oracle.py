@@ -60,6 +60,10 @@ def _load():
             ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
             ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         lib.oracle_bottom_left_bruteforce.restype = ctypes.c_int
+        lib.oracle_initial_solution.argtypes = [ctypes.POINTER(_Problem), ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p]
+        lib.oracle_initial_solution.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -127,3 +131,24 @@ def bottom_left_bruteforce(grid, mask):
                                             m.shape[0], m.shape[1], ctypes.byref(col),
                                             ctypes.byref(row))
     return (True, col.value, row.value) if has else (False, None, None)
+
+
+def initial_solution(packed, order, machine_keys, with_log=False):
+    """Greedy initial solution (BRKGA_alg3.py:445-603): ``(keys float32[2n],
+    makespan, log [n, 6] or None)`` for the slot order ``order`` and the
+    per-machine machine keys ``machine_keys``."""
+    lib = _load()
+    s, keep = _problem(packed)
+    n = int(packed["n_parts"])
+    order = np.ascontiguousarray(order, np.int32)
+    mk = np.ascontiguousarray(machine_keys, np.float64)
+    keys = np.zeros(2 * n, np.float32)
+    ms = ctypes.c_double()
+    log = np.full((n, 6), -1, np.int32) if with_log else None
+    rc = lib.oracle_initial_solution(ctypes.byref(s), order.ctypes.data, mk.ctypes.data,
+                                     keys.ctypes.data, ctypes.byref(ms),
+                                     log.ctypes.data if with_log else None)
+    del keep
+    if rc != 0:
+        raise RuntimeError("oracle_initial_solution: a part fits no machine")
+    return keys, ms.value, log

@@ -388,3 +388,162 @@ int oracle_bottom_left_bruteforce(const uint8_t *grid, int H, int W, const uint8
             }
     return 0;
 }
+
+/* ---------------------------------------------------------------------------
+ * Greedy initial solution: BRKGA_alg3.py:445-603 with the batch class of
+ * binClassInitialSol.py:41-174.
+ *
+ * Parts are taken in `order` (slots sorted by decreasing height, then
+ * decreasing polygon area, ties in instance order).  For each part, every
+ * machine whose plate fits rotation 0 in either orientation is tried: a new
+ * batch (running machine makespan + setup + processing + height time, the
+ * part at (0, L-1) in its best rotation), then every existing batch of that
+ * machine in order with can_insert (vacancy filter, bottom-left collision
+ * test and density -> row -> col choice over the rotations, no area check).
+ * The candidate minimising max(current worst, machine makespan) wins, "<="
+ * letting later candidates win ties, except that a second existing batch of
+ * the same machine does not replace an equal one (placedInExist).
+ *
+ * The chromosome: the parts of machine m, in batch order then insertion
+ * order, are mapped back to the first unused slot holding that part id; their
+ * machine keys are machine_keys[m] (the reference draws one random.uniform
+ * per machine inside the machine's threshold bucket) and their sequence keys
+ * i * (1 / k) for the i-th of the machine's k parts (np.linspace(0, 1, k,
+ * endpoint=False)), all stored as float32.
+ *
+ * Returns 0, or -1 when a part fits no machine (the reference then fails).
+ * log (nullable): [n_parts][6] (machine, part id, batch, col, bottom row,
+ * rotation) in greedy order.
+ * ------------------------------------------------------------------------- */
+typedef struct { pp_bin b; int32_t *parts; int np; } greedy_bin;
+
+static int greedy_can_insert(const pp_problem *p, const rot_tables *t, const pp_bin *b, int m, int ty,
+                             int *out_x, int *out_row, int *out_rot) {
+    const int H = p->plate_h[m], W = p->plate_w[m], r0 = p->rot_base[ty];
+    int best_len = 0, best_row = -1, best_x = 0, best_rot = -1;
+    int64_t positions = 0;
+    for (int r = 0; r < p->nrot[ty]; r++) {
+        const int rg = r0 + r, h = p->rot_h[rg], w = p->rot_w[rg];
+        if (h > H || w > W) continue;
+        if (!vacancy_fit(b->vac, H, t->dens + t->dens_off[rg], h)) continue;
+        int y, x;
+        if (!bottom_left(b, H, W, t, rg, h, w, &y, &x, &positions)) continue;
+        const int row = y + h - 1;
+        const int new_len = b->enc_len > H - y ? b->enc_len : H - y;
+        if (best_rot < 0 || new_len < best_len || (new_len == best_len && row > best_row) ||
+            (new_len == best_len && row == best_row && x < best_x)) {
+            best_len = new_len; best_row = row; best_x = x; best_rot = r;
+        }
+    }
+    if (best_rot < 0) return 0;
+    *out_x = best_x; *out_row = best_row; *out_rot = best_rot;
+    return 1;
+}
+
+static void greedy_open(greedy_bin *g, int H, int W, int n) {
+    pp_bin *b = &g->b;
+    b->grid = (uint8_t *)calloc((size_t)H * W, 1);
+    b->freer = (uint16_t *)malloc(sizeof(uint16_t) * (size_t)H * W);
+    b->vac = (int32_t *)malloc(sizeof(int32_t) * (size_t)H);
+    for (int y = 0; y < H; y++) {
+        b->vac[y] = W;
+        for (int x = 0; x < W; x++) b->freer[(int64_t)y * W + x] = (uint16_t)(W - x);
+    }
+    b->area = 0.0; b->proc_time = 0.0; b->proc_time_height = 0.0;
+    b->min_row = H; b->max_row = -1; b->enc_len = 0;
+    g->parts = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    g->np = 0;
+}
+
+int oracle_initial_solution(const pp_problem *p, const int32_t *order, const double *machine_keys,
+                            float *out_keys, double *out_makespan, int32_t *log) {
+    const int n = p->n_parts, M = p->n_machines, P = p->n_types;
+    rot_tables t;
+    build_tables(p, &t);
+    greedy_bin **bins = (greedy_bin **)calloc((size_t)M, sizeof(greedy_bin *));
+    int *nbm = (int *)calloc((size_t)M, sizeof(int));
+    double *mk = (double *)calloc((size_t)M, sizeof(double));
+    for (int m = 0; m < M; m++) bins[m] = (greedy_bin *)calloc((size_t)n, sizeof(greedy_bin));
+    double worst = 0.0, best_ms = 0.0;
+    int rc = 0;
+    for (int k = 0; k < n && rc == 0; k++) {
+        const int ty = p->slot_type[order[k]];
+        const int r0 = p->rot_base[ty];
+        const int h0 = p->rot_h[r0], w0 = p->rot_w[r0];
+        best_ms = 1.0 / 0.0;
+        int kind = 0, cm = -1, cb = -1, cx = 0, crow = 0, crot = 0;   /* kind 1 new, 2 existing */
+        double cmm = 0.0;
+        for (int m = 0; m < M; m++) {
+            const int H = p->plate_h[m], W = p->plate_w[m];
+            int placed_exist = 0;
+            if ((h0 > H || w0 > W) && (w0 > H || h0 > W)) continue;
+            const int64_t mp = (int64_t)m * P + ty;
+            const double pt = p->proc_time[mp], pth = p->proc_time_height[mp];
+            double mm = ((mk[m] + p->setup[m]) + pt) + pth;
+            double nm = worst > mm ? worst : mm;
+            if (nm <= best_ms) { best_ms = nm; kind = 1; cm = m; cmm = mm; }
+            for (int b = 0; b < nbm[m]; b++) {
+                int x, row, rot;
+                if (!greedy_can_insert(p, &t, &bins[m][b].b, m, ty, &x, &row, &rot)) continue;
+                const double bpth = bins[m][b].b.proc_time_height;
+                mm = bpth < pth ? ((mk[m] - bpth) + pth) + pt : mk[m] + pt;
+                nm = worst > mm ? worst : mm;
+                if (nm <= best_ms) {
+                    if (placed_exist && nm == best_ms) continue;
+                    placed_exist = 1; best_ms = nm; kind = 2; cm = m; cb = b; cx = x; crow = row; crot = rot;
+                    cmm = mm;
+                }
+            }
+        }
+        if (kind == 0) { rc = -1; break; }
+        const int H = p->plate_h[cm], W = p->plate_w[cm];
+        if (kind == 1) {
+            cb = nbm[cm]++;
+            greedy_open(&bins[cm][cb], H, W, n);
+            crot = p->best_rot[ty]; cx = 0; crow = H - 1;
+        }
+        greedy_bin *g = &bins[cm][cb];
+        const int rg = r0 + crot, h = p->rot_h[rg], w = p->rot_w[rg];
+        insert_part(&g->b, H, W, crow - h + 1, cx, p->masks + p->mask_off[rg], h, w, p->area[ty]);
+        const int64_t mp = (int64_t)cm * P + ty;
+        g->b.proc_time += p->proc_time[mp];
+        if (p->proc_time_height[mp] > g->b.proc_time_height) g->b.proc_time_height = p->proc_time_height[mp];
+        g->parts[g->np++] = p->type_id[ty];
+        mk[cm] = cmm;
+        if (cmm > worst) worst = cmm;
+        if (log) {
+            int32_t *e = log + 6 * k;
+            e[0] = cm; e[1] = p->type_id[ty]; e[2] = cb; e[3] = cx; e[4] = crow; e[5] = crot;
+        }
+    }
+    if (rc == 0) {
+        char *used = (char *)calloc((size_t)n, 1);
+        int32_t *pos = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+        for (int j = 0; j < 2 * n; j++) out_keys[j] = 0.0f;
+        for (int m = 0; m < M; m++) {
+            int cnt = 0;
+            for (int b = 0; b < nbm[m]; b++)
+                for (int q = 0; q < bins[m][b].np; q++) {
+                    const int id = bins[m][b].parts[q];
+                    for (int j = 0; j < n; j++)
+                        if (!used[j] && p->type_id[p->slot_type[j]] == id) { used[j] = 1; pos[cnt++] = j; break; }
+                }
+            const double step = cnt ? 1.0 / (double)cnt : 0.0;
+            for (int i = 0; i < cnt; i++) {
+                out_keys[n + pos[i]] = (float)machine_keys[m];
+                out_keys[pos[i]] = (float)((double)i * step);
+            }
+        }
+        free(used); free(pos);
+        *out_makespan = best_ms;
+    }
+    for (int m = 0; m < M; m++) {
+        for (int b = 0; b < nbm[m]; b++) {
+            free(bins[m][b].b.grid); free(bins[m][b].b.freer); free(bins[m][b].b.vac); free(bins[m][b].parts);
+        }
+        free(bins[m]);
+    }
+    free(bins); free(nbm); free(mk);
+    free_tables(&t);
+    return rc;
+}
