@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--ttt-out", default="", help="write the full time-to-target record (JSON) here")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default tile table instead of tuning it before the timed steps")
+    ap.add_argument("--no-init", action="store_true",
+                    help="skip timing the greedy initial solution")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the C3 batched vs single-chromosome comparison")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -267,6 +269,23 @@ def main():
         single = len(one) / (time.perf_counter() - t0)
         ds1.close()
 
+    # ---- greedy initial solution (the seed chromosome) on the same instance ----
+    init = None
+    if not args.no_init:
+        from brkga_fft2_amd.problem import initial_machine_keys, type_heights
+        th = type_heights(prob, packed)
+        mk = initial_machine_keys(thresholds_for(M), M, np.random.default_rng(args.seed))
+        dg = Decoder(packed, device=0)
+        dg.initial_solution(th, mk, stream=stream)
+        t0 = time.perf_counter()
+        g_keys, g_ms, _ = dg.initial_solution(th, mk, stream=stream)
+        g_s = time.perf_counter() - t0
+        dg.close()
+        init = {"workload": "%s instance, %d parts, %d machines, one run" % (args.config, n, M),
+                "gpu_s": round(g_s, 4), "makespan": g_ms}
+        print("bench: greedy initial solution %.4f s, makespan %.6g" % (g_s, g_ms),
+              file=sys.stderr, flush=True)
+
     # ---- CPU baseline: the C oracle on this host's cores, bounded sample ----
     cpu = None
     if not args.no_cpu:
@@ -293,6 +312,13 @@ def main():
                "sample": "%d chromosomes (the first %d timed steps' populations), same instance, "
                          "C oracle with %d OpenMP threads, %.1f s" % (done, done // pop, cores, cpu_t),
                "value_1_thread": round(cpu1, 2), "cpu_model": _cpu_model()}
+        if init is not None:   # the greedy's C restatement, one thread
+            from brkga_fft2_amd.problem import initial_solution_order
+            t0 = time.perf_counter()
+            w_keys, w_ms, _ = oracle.initial_solution(packed, initial_solution_order(packed, th), mk)
+            init["cpu_oracle_s"] = round(time.perf_counter() - t0, 4)
+            if w_ms != g_ms or not np.array_equal(w_keys, g_keys):
+                raise RuntimeError("GPU greedy initial solution differs from the CPU oracle")
 
     # ---- C3: batched persistent decode vs the single-chromosome path ----
     c3 = None
@@ -352,6 +378,7 @@ def main():
         line["batched_over_single"] = round(value / single, 2)
     line["host_buffers_per_s"] = round(host_rate, 2)
     line["autotune"] = tuned
+    line["initial_solution"] = init
     line["c3_batched_vs_single"] = c3
     line["time_to_target"] = ttt
     print(json.dumps(line), flush=True)
