@@ -699,10 +699,35 @@ __device__ __forceinline__ void vacancy_filter(const DevProblem &P, int rb, int 
     }
 }
 
+// Number of leading candidates (top rows, descending) of a rotation of height
+// h that could still beat the choice (bl, brow) made so far: new enclosure
+// length max(enc, H - y) smaller, or equal with a bottom row y + h - 1 not
+// above brow.  Along the list the length never falls and the bottom row
+// always falls, so the losers form a suffix and the first one ends the
+// prefix.  Every wave computes the same count.
+__device__ __forceinline__ int winning_prefix(const short *cand, int ncand, int enc, int H, int h, int bl,
+                                              int brow) {
+    const int lane = threadIdx.x & 63;
+    for (int k0 = 0; k0 < ncand; k0 += 64) {
+        const int k = k0 + lane;
+        bool lose = false;
+        if (k < ncand) {
+            const int y = cand[k];
+            const int nl = enc > H - y ? enc : H - y;
+            lose = nl > bl || (nl == bl && y + h - 1 < brow);
+        }
+        const unsigned long long b = __ballot(lose);
+        if (b) return k0 + (int)__builtin_ctzll(b);
+    }
+    return ncand;
+}
+
 // Collision tests of a part's rotations against one bin (one rotation at a
 // time, whole workgroup) and the choice among them: smallest new enclosure
 // length (the density), then larger bottom row, then smaller column, the
-// first rotation on ties (binClassNew.py:101-117).  Returns the rotation, or
+// first rotation on ties (binClassNew.py:101-117).  Once a rotation has a
+// placement, later rotations test only the candidates that could still win
+// (winning_prefix; the dense mode tests them all).  Returns the rotation, or
 // -1, with its top-left (by, bx); uniform across the workgroup.
 template <bool DENSE>
 __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork &Wk, int mach, int rb, int nr,
@@ -713,13 +738,17 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
                                               unsigned long long &c_mfma, unsigned long long *ph, int &by, int &bx) {
     int best = -1, bl = 0, brow = 0;
     for (int r = 0; r < nr; r++) {
-        const int ncand = uni(s_ncand[r]);
+        int ncand = uni(s_ncand[r]);
         if (ncand == 0) continue;
         const int rg = rb + r;
         const int h = P.rot_h[rg], w = P.rot_w[rg];
+        const short *cand = s_cand + r * hmax;
+        if (!DENSE && best >= 0) {
+            ncand = uni(winning_prefix(cand, ncand, enc, H, h, bl, brow));
+            if (ncand == 0) continue;
+        }
         c_tests++;
         c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
-        const short *cand = s_cand + r * hmax;
         const int ncr = P.rot_nc[rg];
         const unsigned char *tc = P.tiles + ((long long)mach * TUNE_NC + min(ncr, TUNE_NC - 1)) * 2;
         const int key = test_rotation<DENSE>(grid, Wk.pitch, W, h, w, ncr,
