@@ -343,11 +343,13 @@ __device__ __forceinline__ int one_pass(lds_cu8 *band, int stride, int rowoff, i
 
 // All passes over one staged x-window (tiles [0, tl) of the window), run by
 // the whole workgroup.  The window is cut into passes of 8/4/2/1 tiles (no
-// MFMA is spent on tiles past the window).  Returns the window's minimum key.
+// MFMA is spent on tiles past the window).  Returns the window's minimum key;
+// with `early`, stops after the first pass that finds a zero in candidate 0
+// (keys below XKEY): no later tile can give a smaller key.
 template <int NC>
 __device__ __attribute__((always_inline)) int window_passes(lds_cu8 *band, int stride, int rowoff, int h,
                              lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
-                             lds_u64 *s_zw, unsigned long long &mfma, int tpcap) {
+                             lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
     constexpr int TPM = pass_tiles(NC);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int a0 = (h * wave) >> 2, a1 = (h * (wave + 1)) >> 2;
@@ -371,6 +373,7 @@ __device__ __attribute__((always_inline)) int window_passes(lds_cu8 *band, int s
         mfma += (unsigned long long)tp * (a1 - a0) * NC;
         key = k < key ? k : key;
         t0 += tp;
+        if (early && key < XKEY) break;
     }
     return key;
 }
@@ -379,7 +382,7 @@ __device__ __attribute__((always_inline)) int window_passes(lds_cu8 *band, int s
 // chunk loop, one LDS fragment per MFMA.
 __device__ int window_passes_generic(lds_cu8 *band, int stride, int rowoff, int h, int nc,
                                      lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
-                                     lds_u64 *s_zw, unsigned long long &mfma) {
+                                     lds_u64 *s_zw, unsigned long long &mfma, bool early) {
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int shift = 32 + ((lane >> 5) << 4) - (lane & 31);
     const int a0 = (h * wave) >> 2, a1 = (h * (wave + 1)) >> 2;
@@ -401,18 +404,19 @@ __device__ int window_passes_generic(lds_cu8 *band, int stride, int rowoff, int 
         parity ^= 1;
         const int k = decode_zero<1>(zp, xbase + (t0 << 5), xmax, nrows);
         key = k < key ? k : key;
+        if (early && key < XKEY) break;
     }
     return key;
 }
 
 __device__ __forceinline__ int window_dispatch(int nc, lds_cu8 *band, int stride, int rowoff, int h,
                                lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
-                               lds_u64 *s_zw, unsigned long long &mfma, int tpcap) {
-#define CASE(K) case K: return window_passes<K>(band, stride, rowoff, h, qbits, tl, xbase, xmax, nrows, s_zw, mfma, tpcap);
+                               lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
+#define CASE(K) case K: return window_passes<K>(band, stride, rowoff, h, qbits, tl, xbase, xmax, nrows, s_zw, mfma, tpcap, early);
     switch (nc) {
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
         default:
-            return window_passes_generic(band, stride, rowoff, h, nc, qbits, tl, xbase, xmax, nrows, s_zw, mfma);
+            return window_passes_generic(band, stride, rowoff, h, nc, qbits, tl, xbase, xmax, nrows, s_zw, mfma, early);
     }
 #undef CASE
 }
@@ -513,9 +517,10 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
             unsigned long long t1 = tid == 0 ? tick() : 0;
             if (tid == 0) ph[2] += t1 - t0;
             const int k = window_dispatch(nc, band, stride, rowoff, h, s_q, tl, tw0 << 5, W - w,
-                                          nrows, s_zw, mfma, tpcap);
+                                          nrows, s_zw, mfma, tpcap, !DENSE);
             gkey = k < gkey ? k : gkey;
             if (tid == 0) ph[3] += tick() - t1;
+            if (!DENSE && gkey < XKEY) break;   // candidate 0 has a zero: nothing further left can win
         }
         if (gkey != KEY_NONE) {
             if (found == KEY_NONE) found = (g0 + gkey / XKEY) * XKEY + gkey % XKEY;

@@ -56,7 +56,7 @@ class BrkgaGaParams(ctypes.Structure):
                 ("elite_prob", ctypes.c_double), ("seed", ctypes.c_uint64)]
 
 
-TILE_CLASSES = 16   # BRKGA_TILE_CLASSES: mask width classes nc = ceil(w / 32)
+TILE_CLASSES = 16   # BRKGA_TILE_CLASSES: mask width classes nc = ceil((w + 31) / 32)
 
 EXPORTED = ["brkga_create", "brkga_evaluate", "brkga_evaluate_device", "brkga_get_stats",
             "brkga_reset_stats", "brkga_set_tiles", "brkga_get_tiles", "brkga_initial_solution",
