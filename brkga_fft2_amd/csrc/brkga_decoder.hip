@@ -598,59 +598,65 @@ __global__ void reset_counters_kernel(unsigned int *queue) { *queue = 0u; }
 // the mask row's longest 1-run (numba_utils.py:131-157 applies the same test
 // to decide whether the bin is tried).  For mask row a, lane l tests grid row
 // 64k + a + l, so the ballot is already aligned with the candidate word.
-// Four mask rows per step keep 4 x NK reads in flight (clamped indices, the
-// guard folded into the predicate); NK = ceil((H - h + 1) / 64) words.
+// Four mask rows per step keep 4 x NK reads in flight; NK = ceil((H - h + 1) /
+// 64) words, all indexed by constants so that they stay in scalar registers
+// and the loop exit stays uniform.  Lanes whose top row is past H - h read
+// rows up to H + 62 (inside the LDS allocation: the layout pads the vacancy
+// arrays) and are cleared by the initial mask.  The passing top rows go to
+// cand in descending order; returns their count.
 template <int NK>
-__device__ __forceinline__ void vacancy_words(unsigned long long (&cw)[8], lds_cs16 *vrow,
-                                              lds_ci32 *den, int h, int H, int ny) {
+__device__ __forceinline__ int vacancy_rotation(lds_cs16 *vrow, lds_ci32 *den, int h, int ny, short *cand) {
     const int lane = threadIdx.x & 63;
+    unsigned long long cw[NK];
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int left = ny - (k << 6);
-        cw[k] = k >= NK ? 0ull : (left >= 64 ? ~0ull : ((1ull << left) - 1ull));
+    for (int k = 0; k < NK; k++) {
+        const int left = ny - (k << 6);   // >= 1 for k < NK
+        cw[k] = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
     }
+    lds_cs16 *vl = vrow + lane;
     for (int a = 0; a < h; a += 4) {
-        unsigned long long any = 0;
+        int aj[4], d[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const int aj = min(a + j, h - 1);
-            const int d = den[aj];
-#pragma unroll
-            for (int k = 0; k < NK; k++) {
-                const int r = (k << 6) + aj + lane;
-                const int v = vrow[min(r, H - 1)];
-                cw[k] &= __ballot(r >= H || v >= d);
-            }
+            aj[j] = min(a + j, h - 1);
+            d[j] = __builtin_amdgcn_readfirstlane(den[aj[j]]);
         }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int k = 0; k < NK; k++) cw[k] &= __ballot(vl[aj[j] + (k << 6)] >= d[j]);
+        unsigned long long any = 0;
 #pragma unroll
         for (int k = 0; k < NK; k++) any |= cw[k];
         if (!any) break;
     }
+    // compaction in scalar form: bit-reversed, word k gives lane l the row
+    // y = 64k + 63 - l (inverse ballot) and its rank among the passing rows
+    // above it (mbcnt)
+    int nc_ = 0;
+#pragma unroll
+    for (int k = NK - 1; k >= 0; k--) {   // descending y
+        if (cw[k]) {
+            const unsigned long long rv = __builtin_bitreverse64(cw[k]);
+            const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(rv >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)rv, 0));
+            if (__builtin_amdgcn_inverse_ballot_w64(rv)) cand[nc_ + pos] = (short)((k << 6) + 63 - lane);
+            nc_ += __popcll(cw[k]);
+        }
+    }
+    return nc_;
 }
 
 // Densities (longest 1-run per mask row) of rotations rb..rb+nr-1 into
-// s_den[r * hrot + a]: one batched pass of the whole workgroup (the caller
-// synchronises before the vacancy filter reads them).
-__device__ __forceinline__ void load_densities(const DevProblem &P, int rb, int nr, int hrot, int *s_den) {
-    int hr[MAX_ROT];
-    long long dof[MAX_ROT];
-#pragma unroll
-    for (int r = 0; r < MAX_ROT; r++) {
-        hr[r] = r < nr ? P.rot_h[rb + r] : 0;
-        dof[r] = r < nr ? P.dens_off[rb + r] : 0;
-    }
-    const DivBy byh(hrot);
-    wg_copy<int>(nr * hrot,
-                 [&](int k) {
-                     const int r = byh(k), a = k - r * hrot;
-                     int h = hr[0];
-                     long long o = dof[0];
-#pragma unroll
-                     for (int q = 1; q < MAX_ROT; q++)
-                         if (r == q) { h = hr[q]; o = dof[q]; }
-                     return a < h ? P.dens[o + a] : 0;
-                 },
-                 [&](int k, int v) { s_den[k] = v; });
+// s_den, rotation after rotation (rotation r at the sum of the heights of
+// rotations rb..rb+r-1).  brkga_create appends the densities in rotation
+// order and a type's rotations are consecutive, so this is one contiguous
+// copy by the whole workgroup (the caller synchronises before the vacancy
+// filter reads them).
+__device__ __forceinline__ void load_densities(const DevProblem &P, int rb, int nr, int *s_den) {
+    int tot = 0;
+    for (int r = 0; r < nr; r++) tot += P.rot_h[rb + r];
+    const int *src = P.dens + P.dens_off[rb];
+    wg_copy<int>(tot, [&](int k) { return src[k]; }, [&](int k, int v) { s_den[k] = v; });
 }
 
 // Vacancy filter of a part's rotations against one bin, wave r: rotation r.
@@ -661,39 +667,30 @@ __device__ __forceinline__ void load_densities(const DevProblem &P, int rb, int 
 // synchronises.  Dense mode lists every row of the valid region (same results).
 template <bool DENSE>
 __device__ __forceinline__ void vacancy_filter(const DevProblem &P, int rb, int nr, int H, int W, const short *vrow,
-                                               const int *s_den, int hrot, short *s_cand, int hmax, int *s_ncand) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+                                               const int *s_den, short *s_cand, int hmax, int *s_ncand) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (wave < nr) {
         const int rg = rb + wave;
         const int h = P.rot_h[rg], w = P.rot_w[rg];
         short *cand = s_cand + wave * hmax;
-        const int *den = s_den + wave * hrot;
+        int doff = 0;
+        for (int q = 0; q < wave; q++) doff += P.rot_h[rb + q];
+        const int *den = s_den + doff;
         int nc_ = 0;
         if (h <= H && w <= W) {
-            // Bitset form: word k holds top rows y in [64k, 64k+64); for
-            // mask row a, lane l tests grid row 64k + a + l, so the ballot
-            // is already aligned with the candidate word.  Four mask rows
-            // per step keep many LDS reads in flight.
             const int ny = H - h + 1;
             const int nk = (ny + 63) >> 6;   // <= 8 for plates <= 512 rows
-            unsigned long long cw[8];
+            lds_cs16 *vr = (lds_cs16 *)vrow;
+            lds_ci32 *dn = (lds_ci32 *)den;
             switch (nk) {
-                case 1: vacancy_words<1>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                case 2: vacancy_words<2>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                case 3: vacancy_words<3>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                case 4: vacancy_words<4>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                case 5: vacancy_words<5>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                case 6: vacancy_words<6>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                case 7: vacancy_words<7>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-                default: vacancy_words<8>(cw, (lds_cs16 *)vrow, (lds_ci32 *)den, h, H, ny); break;
-            }
-            for (int k = nk - 1; k >= 0; k--) {   // descending y
-                const int y = (k << 6) + 63 - lane;
-                const bool ok = (cw[k] >> (63 - lane)) & 1ull;
-                const unsigned long long bal = __ballot(ok);
-                const int pos = __popcll(bal & ((1ull << lane) - 1ull));
-                if (ok) cand[nc_ + pos] = (short)y;
-                nc_ += __popcll(bal);
+                case 1: nc_ = vacancy_rotation<1>(vr, dn, h, ny, cand); break;
+                case 2: nc_ = vacancy_rotation<2>(vr, dn, h, ny, cand); break;
+                case 3: nc_ = vacancy_rotation<3>(vr, dn, h, ny, cand); break;
+                case 4: nc_ = vacancy_rotation<4>(vr, dn, h, ny, cand); break;
+                case 5: nc_ = vacancy_rotation<5>(vr, dn, h, ny, cand); break;
+                case 6: nc_ = vacancy_rotation<6>(vr, dn, h, ny, cand); break;
+                case 7: nc_ = vacancy_rotation<7>(vr, dn, h, ny, cand); break;
+                default: nc_ = vacancy_rotation<8>(vr, dn, h, ny, cand); break;
             }
             if (DENSE && nc_ > 0) {   // full valid region, bottom-up
                 nc_ = H - h + 1;
@@ -835,7 +832,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
     double *s_sv = (double *)smem;                          // [n]
     int *s_mach = (int *)(s_sv + n);                        // [n]
     int *s_ty = s_mach + n;                                 // [n] part type at sequence position i
-    int *s_den = s_ty + n;                                  // [MAX_ROT][hrot]
+    int *s_den = s_ty + n;                                  // [<= MAX_ROT * hrot] one type's densities
     short *s_cand = (short *)(s_den + MAX_ROT * Wk.hrot);   // [MAX_ROT][hmax]
     short *s_vac = s_cand + MAX_ROT * hmax;                 // [hmax] vacancy of an uncached bin
     short *s_vcache = s_vac + hmax;                         // [vc_bins][hmax] vacancy of bins < vc_bins
@@ -912,7 +909,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
             const int nr = P.nrot[ty];
             const int rb = P.rot_base[ty];
             const int nb = uni(s_nb);
-            load_densities(P, rb, nr, Wk.hrot, s_den);
+            load_densities(P, rb, nr, s_den);
             __syncthreads();   // s_den is read by every wave's vacancy filter
             bool placed = false;
             int pb = 0, py = 0, px = 0, prot = 0;
@@ -929,7 +926,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                     __syncthreads();
                 }
                 // ---- vacancy filter, per candidate top row (wave r: rotation r) ----
-                vacancy_filter<DENSE>(P, rb, nr, H, W, vrow, s_den, Wk.hrot, s_cand, hmax, s_ncand);
+                vacancy_filter<DENSE>(P, rb, nr, H, W, vrow, s_den, s_cand, hmax, s_ncand);
                 __syncthreads();
                 if (tid == 0) ph[1] += tick() - tp;
                 // ---- collision tests, one rotation at a time (whole workgroup) ----
@@ -1056,10 +1053,10 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P
     const long long slot = (long long)m * A.max_batches + b;
     const unsigned char *grid = A.grids + slot * hmax * Wk.pitch;
     const int *vac = A.vac + slot * hmax;
-    load_densities(P, rb, nr, Wk.hrot, s_den);
+    load_densities(P, rb, nr, s_den);
     for (int y = tid; y < H; y += WG_THREADS) s_vac[y] = (short)vac[y];
     __syncthreads();
-    vacancy_filter<false>(P, rb, nr, H, W, s_vac, s_den, Wk.hrot, s_cand, hmax, s_ncand);
+    vacancy_filter<false>(P, rb, nr, H, W, s_vac, s_den, s_cand, hmax, s_ncand);
     __syncthreads();
     unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int bx = 0, by = 0;
@@ -1388,8 +1385,9 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     for (int r = 0; r < R; r++) hrot = std::max(hrot, p->rot_h[r]);
     w.hrot = hrot;
     w.vc_bins = std::min(mb, 16);
+    // (+64 rows: the vacancy filter reads up to 63 rows past a vacancy vector)
     size_t fixed = (size_t)p->n_parts * (8 + 4 + 4) + (size_t)MAX_ROT * hrot * 4 +
-                   (size_t)MAX_ROT * hmax * 2 + (size_t)hmax * 2 + (size_t)w.vc_bins * hmax * 2;
+                   (size_t)MAX_ROT * hmax * 2 + (size_t)hmax * 2 + (size_t)w.vc_bins * hmax * 2 + 64 * 2;
     fixed = (fixed + 15) & ~(size_t)15;
     size_t qwords = 1;
     for (int r = 0; r < R; r++) qwords = std::max(qwords, (size_t)p->rot_h[r] * (nc[r] + 1));
