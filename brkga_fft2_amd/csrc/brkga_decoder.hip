@@ -531,16 +531,14 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
     return found;
 }
 
-// Longest free run (longest run of zero cells) of one grid row in LDS,
-// computed by one wave (numba_utils.py:161-188 computes the same value
-// sequentially).  Blocks of 512 cells: lane l reads cells 8l..8l+7 with one
-// 8-byte LDS read (cells past W count as occupied) and keeps their free cells
-// at the low end (pre) and high end (suf) and their longest free run (a
-// 16-pattern table per 4 cells and the run across the two halves).  A run
-// crossing lanes starts in the suffix of a lane, spans the all-free lanes
-// after it (one ballot) and ends in the prefix of the next lane that is not
-// all free (one shuffle); the run at the block start adds the free cells
-// carried from the previous block.
+// Longest free run (longest run of zero cells) of `rows` grid rows in LDS
+// (row r at band + r * rstride), numba_utils.py:161-188 computes the same
+// value sequentially per row.  Eight lanes per row, eight rows per wave
+// pass: lane j of a row scans the cells [j*S, j*S + S), S = 8 * ceil(W / 64),
+// with one 8-byte LDS read per 8 cells (cells past W count as occupied), and
+// keeps its free cells at the segment start (pre) and end (suf) and its
+// longest run (best); three ordered butterfly steps join the eight segments
+// of the row.  put(r, v) receives each row's value from one lane.
 __device__ __forceinline__ unsigned occupied4(unsigned v) {   // bit j: byte j non-zero
     v |= v >> 4;
     v |= v >> 2;
@@ -552,41 +550,56 @@ __device__ __forceinline__ int free_run4(unsigned o4) {   // longest free run of
     return (int)(((o4 < 8 ? 0x11122234u : 0x01121123u) >> ((o4 & 7u) << 2)) & 0xFu);
 }
 
-__device__ __forceinline__ int row_free_run(lds_cu8 *rp, int W) {
-    const int lane = threadIdx.x & 63;
-    int best = 0, carry = 0;
-    for (int c0 = 0; c0 < W; c0 += 512) {
-        const int x = c0 + (lane << 3);
-        unsigned o = 0xFFu;   // bit j: cell x + j occupied
-        if (x < W) {
-            // x + 7 stays inside the row's 16-byte padded span
-            const unsigned long long v = *(LDSP const unsigned long long *)(rp + x);
-            o = occupied4((unsigned)v) | (occupied4((unsigned)(v >> 32)) << 4);
-            const int valid = W - x;
-            if (valid < 8) o |= 0xFFu & (0xFFu << valid);
+template <typename Put>
+__device__ __forceinline__ void rows_free_run(lds_cu8 *band, int rstride, int rows, int W, Put put) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int sub = lane & 7;
+    const int S = ((W + 63) >> 6) << 3;   // cells per segment, a multiple of 8
+    const int nch = S >> 3;
+    for (int r0 = wave << 3; r0 < rows; r0 += 32) {
+        const int r = r0 + (lane >> 3);
+        lds_cu8 *rp = band + min(r, rows - 1) * rstride;
+        const int x0 = sub * S;
+        int pre = 0, run = 0, best = 0;
+        bool first = true;
+        for (int i = 0; i < nch; i++) {
+            const int x = x0 + (i << 3);
+            unsigned o = 0xFFu;   // bit t: cell x + t occupied
+            if (x < W) {
+                // x + 7 stays inside the row's 16-byte padded span
+                const unsigned long long v = *(LDSP const unsigned long long *)(rp + x);
+                o = occupied4((unsigned)v) | (occupied4((unsigned)(v >> 32)) << 4);
+                const int valid = W - x;
+                if (valid < 8) o |= 0xFFu & (0xFFu << valid);
+            }
+            const bool z = o == 0;
+            const int lead = __builtin_ctz(o | 0x100u);
+            const int trail = o ? __builtin_clz(o) - 24 : 8;
+            const unsigned lo4 = o & 0xFu, hi4 = o >> 4;
+            const int across = (lo4 ? __builtin_clz(lo4) - 28 : 4) + __builtin_ctz(hi4 | 0x10u);
+            const int inner = max(max(free_run4(lo4), free_run4(hi4)), across);
+            if (first && !z) pre = run + lead;
+            if (!z) best = max(best, max(run + lead, inner));
+            run = z ? run + 8 : trail;
+            first = first && z;
         }
-        const unsigned lo4 = o & 0xFu, hi4 = o >> 4;
-        const int pre = __builtin_ctz(o | 0x100u);
-        const int suf = o ? __builtin_clz(o) - 24 : 8;
-        const int across = (lo4 ? __builtin_clz(lo4) - 28 : 4) + __builtin_ctz(hi4 | 0x10u);
-        int b = max(max(free_run4(lo4), free_run4(hi4)), across);
-        const unsigned long long F = __ballot(o == 0);   // all-free lanes
-        const unsigned long long rest = lane < 63 ? (F >> (lane + 1)) : 0ull;
-        const int c = min((int)__builtin_ctzll(~rest), 63 - lane);
-        const int nxt = lane + 1 + c;
-        const int pre_n = __shfl(pre, min(nxt, 63), 64);
-        if (o != 0 && nxt <= 63) b = max(b, suf + 8 * c + pre_n);
-        best = max(best, wave_max_bits<10>(b));   // b <= 8 + 8 * 62 + 8 = 512
-        if (F == ~0ull) {
-            carry += 512;
-            continue;
+        if (first) pre = run;   // all S cells free: pre = suf = S
+        int suf = run, len = S;
+#pragma unroll
+        for (int d = 1; d < 8; d <<= 1) {   // join segments [.. d) and [d .. 2d)
+            const int ps = __shfl_xor(pre | (suf << 16), d, 64);
+            const int pb = __shfl_xor(best, d, 64);
+            const int opre = ps & 0xFFFF, osuf = ps >> 16;
+            const bool left = (sub & d) == 0;
+            const int lpre = left ? pre : opre, lsuf = left ? suf : osuf;
+            const int rpre = left ? opre : pre, rsuf = left ? osuf : suf;
+            best = max(max(best, pb), lsuf + rpre);
+            pre = lpre == len ? len + rpre : lpre;
+            suf = rsuf == len ? len + lsuf : rsuf;
+            len <<= 1;
         }
-        const int f = __builtin_ctzll(~F);            // first lane that is not all free
-        best = max(best, carry + 8 * f + __builtin_amdgcn_readlane(pre, f));
-        const int last = 63 - __builtin_clzll(~F);     // last lane that is not all free
-        carry = __builtin_amdgcn_readlane(suf, last) + 8 * (63 - last);
+        if (sub == 0 && r < rows) put(r, max(best, max(pre, suf)));
     }
-    return max(best, carry);
 }
 
 struct __align__(8) PlaceRes { int has, y, x, pad; };
@@ -770,54 +783,64 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
 
 // Stamp rotation rg at top-left (py, px) into a bin (binClassNew.py:155-160,
 // cuda_batch_update.py:90-126), whole workgroup: rows [py, py+h) pass through
-// the LDS band in chunks -- load (a fresh bin's rows are zero), stamp the mask
-// (a wave per row, a lane per cell), write back, recompute the rows' free runs
-// into vv (HBM) and into vcache when it is not null.
+// the LDS band in chunks -- the grid rows (a fresh bin's rows are zero) and
+// the mask bits come in one batched copy, each 4-byte band word that the
+// part covers takes its four mask bits as bytes, the rows go back to HBM,
+// and their free runs go to vv (HBM) and to vcache when it is not null.
 __device__ __forceinline__ void stamp_part(const DevProblem &P, const DevWork &Wk, int rg, int W, unsigned char *grid,
                                            int *vv, short *vcache, bool fresh, int py, int px, lds_u8 *band,
                                            lds_u32 *s_q) {
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x;
     const int h = P.rot_h[rg], w = P.rot_w[rg];
-    // rows [py, py+h) pass through LDS in chunks: load (a fresh bin's
-    // rows are zero), stamp the mask, write back, recompute free runs
     const int wq = (W + 15) >> 4;               // 16-byte columns per row
+    const int rb16 = wq << 4;                   // band row bytes
     const DivBy bywq(wq);
-    const int chunk = min(h, Wk.band_bytes / (wq << 4));
+    const int chunk = min(h, Wk.band_bytes / rb16);
+    const int nq = P.rot_nc[rg] + 1;            // mask bit words per row
+    const int q0 = px >> 2, nw = ((px + w - 1) >> 2) - q0 + 1;   // band words per row
+    const DivBy bynw(nw);
     for (int a0 = 0; a0 < h; a0 += chunk) {
         const int rows = min(chunk, h - a0);
-        if (fresh) {
-            for (int k = tid; k < rows * wq; k += WG_THREADS)
-                *(lds_v4i *)(band + (k << 4)) = (v4i){0, 0, 0, 0};
-        } else {
-            const unsigned char *g0 = grid + (long long)(py + a0) * Wk.pitch;
-            wg_copy(rows * wq,
-                    [&](int k) { const int r = bywq(k), q = k - r * wq;
-                                 return *(const v4i *)(g0 + (long long)r * Wk.pitch + (q << 4)); },
-                    [&](int k, v4i v) { *(lds_v4i *)(band + (k << 4)) = v; });
-        }
-        const int nq = P.rot_nc[rg] + 1;   // mask bits of these rows -> LDS
+        const unsigned char *g0 = grid + (long long)(py + a0) * Wk.pitch;
         const unsigned *qg = P.qbits + P.qbits_off[rg] + (long long)a0 * nq;
-        wg_copy<unsigned>(rows * nq, [&](int k) { return qg[k]; },
-                          [&](int k, unsigned v) { s_q[k] = v; });
+        if (fresh)
+            for (int k = tid; k < rows * wq; k += WG_THREADS) *(lds_v4i *)(band + (k << 4)) = (v4i){0, 0, 0, 0};
+        const int ng = fresh ? 0 : rows * wq;
+        wg_copy(ng + rows * nq,
+                [&](int k) -> v4i {
+                    if (k < ng) {
+                        const int r = bywq(k), q = k - r * wq;
+                        return *(const v4i *)(g0 + (long long)r * Wk.pitch + (q << 4));
+                    }
+                    return (v4i){(int)qg[k - ng], 0, 0, 0};
+                },
+                [&](int k, v4i v) {
+                    if (k < ng) *(lds_v4i *)(band + (k << 4)) = v;
+                    else s_q[k - ng] = (unsigned)v[0];
+                });
         __syncthreads();
-        for (int r = wave; r < rows; r += 4)   // wave per row, lane per cell
-            for (int bcol = lane; bcol < w; bcol += 64) {
-                const unsigned bits = s_q[r * nq + ((bcol + 32) >> 5)];
-                if ((bits >> ((bcol + 32) & 31)) & 1u) band[((r * wq) << 4) + px + bcol] = 1;
-            }
+        // mask row bits Q[i] = P[a][i - 32] (32 zero bits in front, zero past w):
+        // the word at band bytes 4q..4q+3 takes bits 4q - px + 32 .. + 3
+        for (int k = tid; k < rows * nw; k += WG_THREADS) {
+            const int r = bynw(k), q = q0 + (k - r * nw);
+            const int i = (q << 2) - px + 32;   // >= 29
+            const int wi = i >> 5;
+            const unsigned lo = s_q[r * nq + wi];
+            const unsigned hi = wi + 1 < nq ? s_q[r * nq + wi + 1] : 0u;
+            const unsigned b4 = (unsigned)((((unsigned long long)hi << 32) | lo) >> (i & 31)) & 0xFu;
+            // bit t -> byte t (the four shifted copies never overlap)
+            if (b4) *(lds_u32 *)(band + r * rb16 + (q << 2)) |= (b4 * 0x00204081u) & 0x01010101u;
+        }
         __syncthreads();
         for (int k = tid; k < rows * wq; k += WG_THREADS) {
             const int r = bywq(k), q = k - r * wq;
             *(v4i *)(grid + (long long)(py + a0 + r) * Wk.pitch + (q << 4)) =
                 *(lds_cv4i *)(band + ((r * wq + q) << 4));
         }
-        for (int r = wave; r < rows; r += 4) {   // numba_utils.py:161-188
-            const int v = row_free_run(band + ((r * wq) << 4), W);
-            if (lane == 0) {
-                vv[py + a0 + r] = v;
-                if (vcache) vcache[py + a0 + r] = (short)v;
-            }
-        }
+        rows_free_run(band, rb16, rows, W, [&](int r, int v) {
+            vv[py + a0 + r] = v;
+            if (vcache) vcache[py + a0 + r] = (short)v;
+        });
         __syncthreads();
     }
 }

@@ -1,14 +1,11 @@
-"""The decoder's free-run scan (row_free_run in csrc/brkga_decoder.hip) works
-on blocks of 512 cells: each of 64 lanes summarises 8 cells (free cells at
-the low and high end, longest free run from a 16-entry table per 4 cells and
-the run across them, all-free flag),
-a ballot of the all-free lanes joins runs across lanes, and a carry joins
-blocks.  This restates those steps in Python and checks them against the
-plain longest-zero-run scan (numba_utils.py:161-188) on random rows,
-including rows that need several blocks."""
+"""The decoder's free-run scan (rows_free_run in csrc/brkga_decoder.hip) gives
+each grid row eight lanes: lane j scans the cells [j*S, j*S + S), S = 8 *
+ceil(W / 64), eight cells at a time (free cells at the segment start and
+end, longest run from a 16-entry table per 4 cells and the run across them),
+and three ordered butterfly steps join the eight segments.  This restates
+those steps in Python and checks them against the plain longest-zero-run scan
+(numba_utils.py:161-188) on random rows of many widths."""
 import random
-
-M64 = (1 << 64) - 1
 
 
 def _maxrun4(o):
@@ -26,48 +23,57 @@ def test_pattern_table_constants():
     assert (lo, hi) == (0x11122234, 0x01121123)
 
 
+def test_mask_bits_to_bytes_multiplier():
+    """The stamp spreads four mask bits over the four bytes of a band word with
+    one multiply: bit t lands in byte t and the shifted copies never overlap."""
+    for b in range(16):
+        assert (b * 0x00204081) & 0x01010101 == sum(((b >> t) & 1) << (8 * t) for t in range(4))
+
+
 def _ctz(v):
     return (v & -v).bit_length() - 1
 
 
-def lane_block_scan(row):
-    W, best, carry = len(row), 0, 0
-    table = (0x11122234, 0x01121123)
+def _lut(o4):
+    return ((0x11122234 if o4 < 8 else 0x01121123) >> (4 * (o4 & 7))) & 0xF
 
-    def lut(o4):
-        return (table[o4 >> 3] >> (4 * (o4 & 7))) & 0xF
 
-    for c0 in range(0, W, 512):
-        pre, suf, b, full = [], [], [], []
-        for lane in range(64):
-            x = c0 + 8 * lane
-            o = 0xFF
-            if x < W:
-                o = sum(1 << j for j in range(8) if x + j >= W or row[x + j])
-            lo4, hi4 = o & 0xF, o >> 4
-            pre.append(_ctz(o | 0x100))
-            suf.append(8 - o.bit_length() if o else 8)
-            across = (4 - lo4.bit_length() if lo4 else 4) + _ctz(hi4 | 0x10)
-            b.append(max(lut(lo4), lut(hi4), across))
-            full.append(o == 0)
-        F = sum(1 << lane for lane in range(64) if full[lane])
-        for lane in range(64):
-            rest = F >> (lane + 1) if lane < 63 else 0
-            c = min(_ctz(~rest & M64), 63 - lane)
-            nxt = lane + 1 + c
-            if not full[lane] and nxt <= 63:
-                b[lane] = max(b[lane], suf[lane] + 8 * c + pre[nxt])
-        assert max(b) < 1024   # the kernel's 10-bit ballot search
-        best = max(best, max(b))
-        if F == M64:
-            carry += 512
-            continue
-        nf = ~F & M64
-        f = _ctz(nf)
-        best = max(best, carry + 8 * f + pre[f])
-        last = nf.bit_length() - 1
-        carry = suf[last] + 8 * (63 - last)
-    return max(best, carry)
+def segment_scan(row):
+    W = len(row)
+    S = 8 * ((W + 63) // 64)
+    lanes = []
+    for j in range(8):
+        first, pre, best, run = True, 0, 0, 0
+        for i in range(S // 8):
+            x = j * S + 8 * i
+            o = sum(1 << t for t in range(8) if x + t >= W or row[x + t])
+            z = o == 0
+            lead = _ctz(o | 0x100)
+            trail = (8 - o.bit_length()) if o else 8
+            lo4, hi4 = o & 15, o >> 4
+            across = ((4 - lo4.bit_length()) if lo4 else 4) + _ctz(hi4 | 0x10)
+            inner = max(_lut(lo4), _lut(hi4), across)
+            if first and not z:
+                pre = run + lead
+            if not z:
+                best = max(best, run + lead, inner)
+            run = run + 8 if z else trail
+            first = first and z
+        if first:
+            pre = run
+        lanes.append((pre, run, best))
+    length = S
+    for d in (1, 2, 4):
+        joined = []
+        for i in range(8):
+            a, b = (lanes[i], lanes[i ^ d]) if (i & d) == 0 else (lanes[i ^ d], lanes[i])
+            pre = length + b[0] if a[0] == length else a[0]
+            suf = length + a[1] if b[0] == length else b[1]
+            joined.append((pre, suf, max(a[2], b[2], a[1] + b[0])))
+        lanes = joined
+        length *= 2
+    assert len(set(lanes)) == 1   # every lane of the row ends with the same value
+    return max(lanes[0])
 
 
 def plain_scan(row):
@@ -78,14 +84,16 @@ def plain_scan(row):
     return best
 
 
-def test_lane_block_scan_matches_plain_scan():
+def test_segment_scan_matches_plain_scan():
     rng = random.Random(2)
-    for _ in range(1500):
-        W = rng.choice([1, 2, 3, 7, 8, 9, 31, 63, 64, 65, 250, 300, 500, 511, 512, 513, 700, 1100])
+    for _ in range(2000):
+        W = rng.choice([1, 2, 3, 7, 8, 9, 31, 63, 64, 65, 250, 300, 350, 400, 500, 511, 512, 513, 700, 1100])
         p = rng.random() ** 2
         row = [1 if rng.random() < p else 0 for _ in range(W)]
         if rng.random() < 0.3:
             a = rng.randrange(W)
             b = rng.randrange(a, W + 1)
             row[a:b] = [0] * (b - a)
-        assert lane_block_scan(row) == plain_scan(row), (W, row)
+        assert segment_scan(row) == plain_scan(row), (W, row)
+    for W in (1, 64, 300, 512):   # all free, all occupied
+        assert segment_scan([0] * W) == W and segment_scan([1] * W) == 0
