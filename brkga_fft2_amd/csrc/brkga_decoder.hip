@@ -237,7 +237,7 @@ __device__ __forceinline__ void store_zero_masks(const v16i (&acc)[TP], int tcnt
 }
 
 template <int TP, int NC>
-__device__ __forceinline__ void corr_pass(lds_cu8 *band, int stride, int rowoff, int a0,
+__device__ __forceinline__ void corr_pass(lds_cu8 *band, int stride, int loff, int a0,
                                           int a1, lds_cu32 *qbits, int tloc, int tcnt,
                                           lds_u64 *zw) {
     constexpr int NA = TP + NC - 1;
@@ -246,7 +246,7 @@ __device__ __forceinline__ void corr_pass(lds_cu8 *band, int stride, int rowoff,
     v16i acc[TP];
 #pragma unroll
     for (int t = 0; t < TP; t++) acc[t] = (v16i){};
-    lds_cu8 *base = band + rowoff * stride + (tloc << 5) + ((lane >> 5) << 4);
+    lds_cu8 *base = band + loff + (tloc << 5) + ((lane >> 5) << 4);
     if (a0 < a1) {
         // A fragments in order j = 0..NA-1 per mask row, each feeding the
         // pairs (t = j - c, c); fragment j + D is loaded while j is used (the
@@ -290,12 +290,14 @@ __device__ __forceinline__ void corr_pass(lds_cu8 *band, int stride, int rowoff,
 }
 
 // Bottom-most then left-most zero in the combined masks of one pass, as
-// m*XKEY + x (m < nrows, x <= xmax), or KEY_NONE.  C layout of the 32x32
-// MFMA: lane holds column n = lane&31 and rows m = 8*(i/4) + 4*(lane/32) + i%4
-// of accumulator i, so row m lives in ballot word i = 4*(m/8) + m%4, half
-// (m/4)%2.  Lanes 0..31 each scan one row m; every wave computes the result.
+// c*XKEY + x (candidate c < nrows, x <= xmax), or KEY_NONE.  C layout of the
+// 32x32 MFMA: lane holds column n = lane&31 and rows m = 8*(i/4) + 4*(lane/32)
+// + i%4 of accumulator i, so row m lives in ballot word i = 4*(m/8) + m%4,
+// half (m/4)%2.  Lanes 0..31 each scan one row m: candidate m & (2^rlog - 1)
+// at x offset segx (the lane's segment of a folded group, 0 unfolded); every
+// wave computes the result.
 template <int TP>
-__device__ __forceinline__ int decode_zero(lds_u64 *zp, int xbase, int xmax, int nrows) {
+__device__ __forceinline__ int decode_zero(lds_u64 *zp, int xbase, int xmax, int nrows, int rlog, int segx) {
     const int lane = threadIdx.x & 63;
     const int m = lane & 31;
     const int i = ((m >> 3) << 2) + (m & 3);
@@ -308,16 +310,28 @@ __device__ __forceinline__ int decode_zero(lds_u64 *zp, int xbase, int xmax, int
         bits[t] = (unsigned)(z >> half);
     }
     int key = KEY_NONE;
-    if (lane < 32 && m < nrows) {
+    const int c = m & ((1 << rlog) - 1);
+    if (lane < 32 && c < nrows) {
+        const int x0 = xbase + segx;
 #pragma unroll
         for (int t = TP - 1; t >= 0; t--) {   // the smallest tile with a zero wins
-            const int lim = xmax - (xbase + (t << 5));   // last valid column offset in this tile
+            const int lim = xmax - (x0 + (t << 5));   // last valid column offset in this tile
             unsigned b = lim < 0 ? 0u : (lim < 31 ? bits[t] & ((2u << lim) - 1u) : bits[t]);
-            if (b) key = m * XKEY + xbase + (t << 5) + __builtin_ctz(b);
+            if (b) key = c * XKEY + x0 + (t << 5) + __builtin_ctz(b);
         }
     }
     return wave_min_key(key);
 }
+
+// A group's candidates across the 32 accumulator rows.  With R = 2^rlog >=
+// nrows, row m holds candidate m & (R-1) in segment m >> rlog of the
+// window's tiles: F = 32 / R segments of ceil(tl / F) tiles each, the last
+// ones moved left to end at tile tl (no A read leaves the staged window; the
+// overlap only repeats positions, and the decode keeps the smallest key).
+// One MFMA then covers F times the x-range of an unfolded group.  loff: the
+// lane's A-row byte offset in the band (row of its candidate * stride +
+// segment column); segx: the segment's first column.
+struct Fold { int loff, rlog, segx; };
 
 // Tiles per pass shrink as the chunk count grows so that the accumulators,
 // the B fragments and the A fragments in flight fit the register budget.
@@ -330,24 +344,25 @@ __host__ __device__ constexpr int pass_tiles(int nc) { return nc <= 5 ? 8 : (nc 
 // enough: a wave can only write buffer p%2 again (pass p+2) after every wave
 // has passed the barrier of pass p+1, i.e. has finished decoding pass p.
 template <int TP, int NC>
-__device__ __forceinline__ int one_pass(lds_cu8 *band, int stride, int rowoff, int a0, int a1,
+__device__ __forceinline__ int one_pass(lds_cu8 *band, int stride, const Fold &f, int a0, int a1,
                                         lds_cu32 *qbits, int t0, int xbase, int xmax, int nrows,
                                         lds_u64 *s_zw, int &parity) {
     lds_u64 *zp = s_zw + parity * ZW_PASS;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    corr_pass<TP, NC>(band, stride, rowoff, a0, a1, qbits, t0, TP, zp + wave * ZW_WAVE);
+    corr_pass<TP, NC>(band, stride, f.loff, a0, a1, qbits, t0, TP, zp + wave * ZW_WAVE);
     __syncthreads();
     parity ^= 1;
-    return decode_zero<TP>(zp, xbase + (t0 << 5), xmax, nrows);
+    return decode_zero<TP>(zp, xbase + (t0 << 5), xmax, nrows, f.rlog, f.segx);
 }
 
 // All passes over one staged x-window (tiles [0, tl) of the window), run by
 // the whole workgroup.  The window is cut into passes of 8/4/2/1 tiles (no
 // MFMA is spent on tiles past the window).  Returns the window's minimum key;
-// with `early`, stops after the first pass that finds a zero in candidate 0
-// (keys below XKEY): no later tile can give a smaller key.
+// with `early`, stops once candidate 0 has a zero left of every tile still to
+// come (a key below xbase + 32 * t0; in a folded group the later passes of
+// segment 0 come before the current ones of the other segments).
 template <int NC>
-__device__ __attribute__((always_inline)) int window_passes(lds_cu8 *band, int stride, int rowoff, int h,
+__device__ __attribute__((always_inline)) int window_passes(lds_cu8 *band, int stride, const Fold &f, int h,
                              lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
                              lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
     constexpr int TPM = pass_tiles(NC);
@@ -358,29 +373,29 @@ __device__ __attribute__((always_inline)) int window_passes(lds_cu8 *band, int s
         const int rem = tl - t0;
         int k, tp;
         if (TPM >= 8 && tpcap >= 8 && rem >= 8) {
-            k = one_pass<(TPM >= 8 ? 8 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
+            k = one_pass<(TPM >= 8 ? 8 : 1), NC>(band, stride, f, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 8;
         } else if (TPM >= 4 && tpcap >= 4 && rem >= 4) {
-            k = one_pass<(TPM >= 4 ? 4 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
+            k = one_pass<(TPM >= 4 ? 4 : 1), NC>(band, stride, f, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 4;
         } else if (TPM >= 2 && tpcap >= 2 && rem >= 2) {
-            k = one_pass<(TPM >= 2 ? 2 : 1), NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
+            k = one_pass<(TPM >= 2 ? 2 : 1), NC>(band, stride, f, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 2;
         } else {
-            k = one_pass<1, NC>(band, stride, rowoff, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
+            k = one_pass<1, NC>(band, stride, f, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 1;
         }
         mfma += (unsigned long long)tp * (a1 - a0) * NC;
         key = k < key ? k : key;
         t0 += tp;
-        if (early && key < XKEY) break;
+        if (early && key < xbase + (t0 << 5)) break;
     }
     return key;
 }
 
 // Generic chunk count (NC above the compiled set): one tile per pass, runtime
 // chunk loop, one LDS fragment per MFMA.
-__device__ int window_passes_generic(lds_cu8 *band, int stride, int rowoff, int h, int nc,
+__device__ int window_passes_generic(lds_cu8 *band, int stride, const Fold &f, int h, int nc,
                                      lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
                                      lds_u64 *s_zw, unsigned long long &mfma, bool early) {
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -389,7 +404,7 @@ __device__ int window_passes_generic(lds_cu8 *band, int stride, int rowoff, int 
     int key = KEY_NONE, parity = 0;
     for (int t0 = 0; t0 < tl; t0++) {
         v16i acc[1] = {(v16i){}};
-        lds_cu8 *base = band + rowoff * stride + (t0 << 5) + ((lane >> 5) << 4);
+        lds_cu8 *base = band + f.loff + (t0 << 5) + ((lane >> 5) << 4);
         for (int a = a0; a < a1; a++) {
             lds_cu8 *rp = base + a * stride;
             lds_cu32 *q = qbits + a * (nc + 1);
@@ -402,21 +417,21 @@ __device__ int window_passes_generic(lds_cu8 *band, int stride, int rowoff, int 
         store_zero_masks<1>(acc, 1, zp + wave * ZW_WAVE);
         __syncthreads();
         parity ^= 1;
-        const int k = decode_zero<1>(zp, xbase + (t0 << 5), xmax, nrows);
+        const int k = decode_zero<1>(zp, xbase + (t0 << 5), xmax, nrows, f.rlog, f.segx);
         key = k < key ? k : key;
-        if (early && key < XKEY) break;
+        if (early && key < xbase + ((t0 + 1) << 5)) break;
     }
     return key;
 }
 
-__device__ __forceinline__ int window_dispatch(int nc, lds_cu8 *band, int stride, int rowoff, int h,
+__device__ __forceinline__ int window_dispatch(int nc, lds_cu8 *band, int stride, const Fold &f, int h,
                                lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
                                lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
-#define CASE(K) case K: return window_passes<K>(band, stride, rowoff, h, qbits, tl, xbase, xmax, nrows, s_zw, mfma, tpcap, early);
+#define CASE(K) case K: return window_passes<K>(band, stride, f, h, qbits, tl, xbase, xmax, nrows, s_zw, mfma, tpcap, early);
     switch (nc) {
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
         default:
-            return window_passes_generic(band, stride, rowoff, h, nc, qbits, tl, xbase, xmax, nrows, s_zw, mfma, early);
+            return window_passes_generic(band, stride, f, h, nc, qbits, tl, xbase, xmax, nrows, s_zw, mfma, early);
     }
 #undef CASE
 }
@@ -499,7 +514,8 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
         const int ylo = uni(cand[g1 - 1]);
         const int rows = ytop + h - ylo;
         const int nrows = g1 - g0;
-        const int rowoff = cand[g0 + min(lane & 31, nrows - 1)] - ylo;
+        int rl = 5;   // smallest R = 2^rl >= nrows
+        while (rl > 0 && (1 << (rl - 1)) >= nrows) rl--;
         int gkey = KEY_NONE;
         groups++;
         for (int tw0 = 0; tw0 < T; tw0 += TW) {
@@ -516,7 +532,14 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
             __syncthreads();
             unsigned long long t1 = tid == 0 ? tick() : 0;
             if (tid == 0) ph[2] += t1 - t0;
-            const int k = window_dispatch(nc, band, stride, rowoff, h, s_q, tl, tw0 << 5, W - w,
+            Fold f;
+            f.rlog = rl;
+            while ((32 >> f.rlog) > tl) f.rlog++;   // no more segments than tiles
+            const int tf = (tl + (32 >> f.rlog) - 1) >> (5 - f.rlog);   // tiles per segment
+            const int m = lane & 31;
+            f.segx = min((m >> f.rlog) * tf, tl - tf) << 5;
+            f.loff = (cand[g0 + min(m & ((1 << f.rlog) - 1), nrows - 1)] - ylo) * stride + f.segx;
+            const int k = window_dispatch(nc, band, stride, f, h, s_q, tf, tw0 << 5, W - w,
                                           nrows, s_zw, mfma, tpcap, !DENSE);
             gkey = k < gkey ? k : gkey;
             if (tid == 0) ph[3] += tick() - t1;

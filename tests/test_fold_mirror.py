@@ -1,0 +1,79 @@
+"""Folded correlation groups (Fold, window_passes and decode_zero in
+csrc/brkga_decoder.hip): a group of nrows <= R = 2^rlog candidates fills the
+32 accumulator rows with F = 32 / R segments of a window's tiles; passes of
+8/4/2/1 tiles run over every segment at once and the pass loop may stop early
+once candidate 0 has a zero left of every tile still to come.  This restates
+that schedule in Python on random zero patterns and checks it against the
+plain answer -- the smallest (candidate, x) with zero overlap -- so that a
+wrong segment layout, a gap between segments or a premature exit fails."""
+import random
+
+XKEY = 8192
+NONE = 0x7FFFFFFF
+
+
+def fold_params(nrows, tl):
+    rl = 5
+    while rl > 0 and (1 << (rl - 1)) >= nrows:
+        rl -= 1
+    while (32 >> rl) > tl:
+        rl += 1
+    tf = (tl + (32 >> rl) - 1) >> (5 - rl)
+    return rl, tf
+
+
+def window_keys(zero, nrows, tl, xbase, xmax, tpm, tpcap, early):
+    """zero[c][x]: candidate c has zero overlap at window column x."""
+    rl, tf = fold_params(nrows, tl)
+    segx = [min((m >> rl) * tf, tl - tf) * 32 for m in range(32)]
+    key, t0 = NONE, 0
+    while t0 < tf:
+        rem = tf - t0
+        tp = next(t for t in (8, 4, 2, 1) if t <= tpm and t <= tpcap and t <= rem or t == 1)
+        k = NONE
+        for m in range(32):   # lanes 0..31, one accumulator row each
+            c = m & ((1 << rl) - 1)
+            if c >= nrows:
+                continue
+            for t in range(tp):
+                for n in range(32):
+                    x = xbase + segx[m] + 32 * (t0 + t) + n
+                    if x <= xmax and zero[c][x - xbase]:
+                        k = min(k, c * XKEY + x)
+        key = min(key, k)
+        t0 += tp
+        if early and key < xbase + 32 * t0:
+            break
+    return key
+
+
+def test_folded_schedule_finds_the_smallest_key():
+    rng = random.Random(7)
+    for _ in range(3000):
+        tl = rng.randint(1, 12)
+        nrows = rng.randint(1, 32)
+        xbase = 32 * rng.randint(0, 3)
+        width = 32 * tl
+        xmax = xbase + rng.randint(0, width - 1)
+        p = rng.choice([0.0, 0.002, 0.01, 0.05, 0.3])
+        zero = [[rng.random() < p for _ in range(width)] for _ in range(nrows)]
+        want = min([c * XKEY + xbase + x for c in range(nrows) for x in range(width)
+                    if zero[c][x] and xbase + x <= xmax], default=NONE)
+        tpm = rng.choice([8, 4, 2, 1])
+        tpcap = rng.choice([8, 4, 2, 1])
+        for early in (False, True):
+            assert window_keys(zero, nrows, tl, xbase, xmax, tpm, tpcap, early) == want, \
+                (tl, nrows, xbase, xmax, tpm, tpcap, early)
+
+
+def test_segments_cover_the_window_without_leaving_it():
+    for tl in range(1, 65):
+        for nrows in range(1, 33):
+            rl, tf = fold_params(nrows, tl)
+            assert (1 << rl) >= nrows and (32 >> rl) <= tl
+            starts = {min(q * tf, tl - tf) for q in range(32 >> rl)}
+            covered = set()
+            for s in starts:
+                assert 0 <= s and s + tf <= tl
+                covered.update(range(s, s + tf))
+            assert covered == set(range(tl))
