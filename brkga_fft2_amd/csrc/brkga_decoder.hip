@@ -330,8 +330,10 @@ __device__ __forceinline__ int decode_zero(lds_u64 *zp, int xbase, int xmax, int
 // overlap only repeats positions, and the decode keeps the smallest key).
 // One MFMA then covers F times the x-range of an unfolded group.  loff: the
 // lane's A-row byte offset in the band (row of its candidate * stride +
-// segment column); segx: the segment's first column.
-struct Fold { int loff, rlog, segx; };
+// segment column); segx: the segment's first column; skip: leading mask
+// rows that meet only empty grid rows for every candidate of the group (they
+// add nothing to any overlap count, so the waves split rows [skip, h)).
+struct Fold { int loff, rlog, segx, skip; };
 
 // Tiles per pass shrink as the chunk count grows so that the accumulators,
 // the B fragments and the A fragments in flight fit the register budget.
@@ -367,7 +369,7 @@ __device__ __attribute__((always_inline)) int window_passes(lds_cu8 *band, int s
                              lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
     constexpr int TPM = pass_tiles(NC);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int a0 = (h * wave) >> 2, a1 = (h * (wave + 1)) >> 2;
+    const int a0 = f.skip + (((h - f.skip) * wave) >> 2), a1 = f.skip + (((h - f.skip) * (wave + 1)) >> 2);
     int key = KEY_NONE, parity = 0;
     for (int t0 = 0; t0 < tl;) {
         const int rem = tl - t0;
@@ -400,7 +402,7 @@ __device__ int window_passes_generic(lds_cu8 *band, int stride, const Fold &f, i
                                      lds_u64 *s_zw, unsigned long long &mfma, bool early) {
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int shift = 32 + ((lane >> 5) << 4) - (lane & 31);
-    const int a0 = (h * wave) >> 2, a1 = (h * (wave + 1)) >> 2;
+    const int a0 = f.skip + (((h - f.skip) * wave) >> 2), a1 = f.skip + (((h - f.skip) * (wave + 1)) >> 2);
     int key = KEY_NONE, parity = 0;
     for (int t0 = 0; t0 < tl; t0++) {
         v16i acc[1] = {(v16i){}};
@@ -487,10 +489,12 @@ __device__ __forceinline__ void wg_copy(int n, Load load, Store store) {
 // staged into the LDS band (rows ylo..ytop+h-1, one x-window at a time) and
 // correlated.  Returns (candidate index)*XKEY + x of the bottom-most, then
 // left-most, placement with zero overlap, or KEY_NONE.  Uniform across the
-// workgroup.
+// workgroup.  Grid rows above r0 are empty (no part reaches above the bin's
+// top-most placement), so mask rows that only meet them are neither staged
+// nor correlated, except in the dense mode.
 template <bool DENSE>
 __device__ __attribute__((always_inline)) int test_rotation(const unsigned char *grid, int gpitch, int W, int h, int w, int nc,
-                             const unsigned *qbits_g, const short *cand, int ncand, lds_u8 *band,
+                             const unsigned *qbits_g, const short *cand, int ncand, int r0, lds_u8 *band,
                              int band_bytes, lds_u32 *s_q, lds_u64 *s_zw, unsigned long long &groups,
                              unsigned long long &mfma, unsigned long long *ph, int tpcap, int wcap) {
     const int tid = threadIdx.x, lane = tid & 63;
@@ -516,19 +520,22 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
         const int nrows = g1 - g0;
         int rl = 5;   // smallest R = 2^rl >= nrows
         while (rl > 0 && (1 << (rl - 1)) >= nrows) rl--;
-        int gkey = KEY_NONE;
+        // mask rows a < r0 - ytop meet empty rows for every candidate
+        const int skip = DENSE ? 0 : min(h, max(0, r0 - ytop));
+        int gkey = skip == h ? 0 : KEY_NONE;   // the part lies in empty rows: candidate 0 at x = 0
         groups++;
-        for (int tw0 = 0; tw0 < T; tw0 += TW) {
+        for (int tw0 = 0; tw0 < T && skip < h; tw0 += TW) {
             unsigned long long t0 = tid == 0 ? tick() : 0;
             const int tl = min(TW, T - tw0);
             const int q16 = (tl + nc - 1) * 2;
             const DivBy byq(q16);
-            const unsigned char *src = grid + (long long)ylo * gpitch + (tw0 << 5);
-            wg_copy(rows * q16,
+            const unsigned char *src = grid + (long long)(ylo + skip) * gpitch + (tw0 << 5);
+            lds_u8 *dst = band + skip * stride;
+            wg_copy((rows - skip) * q16,
                     [&](int k) { const int r = byq(k), q = k - r * q16;
                                  return *(const v4i *)(src + (long long)r * gpitch + (q << 4)); },
                     [&](int k, v4i v) { const int r = byq(k), q = k - r * q16;
-                                        *(lds_v4i *)(band + r * stride + (q << 4)) = v; });
+                                        *(lds_v4i *)(dst + r * stride + (q << 4)) = v; });
             __syncthreads();
             unsigned long long t1 = tid == 0 ? tick() : 0;
             if (tid == 0) ph[2] += t1 - t0;
@@ -539,6 +546,7 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
             const int m = lane & 31;
             f.segx = min((m >> f.rlog) * tf, tl - tf) << 5;
             f.loff = (cand[g0 + min(m & ((1 << f.rlog) - 1), nrows - 1)] - ylo) * stride + f.segx;
+            f.skip = skip;
             const int k = window_dispatch(nc, band, stride, f, h, s_q, tf, tw0 << 5, W - w,
                                           nrows, s_zw, mfma, tpcap, !DENSE);
             gkey = k < gkey ? k : gkey;
@@ -790,7 +798,7 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
         const int ncr = P.rot_nc[rg];
         const unsigned char *tc = P.tiles + ((long long)mach * TUNE_NC + min(ncr, TUNE_NC - 1)) * 2;
         const int key = test_rotation<DENSE>(grid, Wk.pitch, W, h, w, ncr,
-                                             P.qbits + P.qbits_off[rg], cand, ncand,
+                                             P.qbits + P.qbits_off[rg], cand, ncand, H - enc,
                                              band, Wk.band_bytes, s_q, s_zw, c_groups, c_mfma, ph,
                                              uni(tc[0]), uni(tc[1]));
         if (key == KEY_NONE) continue;
