@@ -771,19 +771,23 @@ __device__ __forceinline__ int winning_prefix(const short *cand, int ncand, int 
 // Collision tests of a part's rotations against one bin (one rotation at a
 // time, whole workgroup) and the choice among them: smallest new enclosure
 // length (the density), then larger bottom row, then smaller column, the
-// first rotation on ties (binClassNew.py:101-117).  Once a rotation has a
-// placement, later rotations test only the candidates that could still win
-// (winning_prefix; the dense mode tests them all).  Returns the rotation, or
-// -1, with its top-left (by, bx); uniform across the workgroup.
+// first rotation on ties (binClassNew.py:101-117).  The rotation rf (the
+// type's minimum-height rotation, the likeliest winner) is tested first and
+// the others follow in order; once a rotation has a placement, the next ones
+// test only the candidates that could still win (winning_prefix; the dense
+// mode tests them all), and equal placements go to the smaller rotation
+// index, so the order changes no choice.  Returns the rotation, or -1, with
+// its top-left (by, bx); uniform across the workgroup.
 template <bool DENSE>
-__device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork &Wk, int mach, int rb, int nr,
+__device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork &Wk, int mach, int rb, int nr, int rf,
                                               int H, int W, int enc, const unsigned char *grid,
                                               const short *s_cand, const int *s_ncand, int hmax, lds_u8 *band,
                                               lds_u32 *s_q, lds_u64 *s_zw, unsigned long long &c_tests,
                                               unsigned long long &c_algo, unsigned long long &c_groups,
                                               unsigned long long &c_mfma, unsigned long long *ph, int &by, int &bx) {
     int best = -1, bl = 0, brow = 0;
-    for (int r = 0; r < nr; r++) {
+    for (int k = 0; k < nr; k++) {
+        const int r = k == 0 ? rf : (k <= rf ? k - 1 : k);
         int ncand = uni(s_ncand[r]);
         if (ncand == 0) continue;
         const int rg = rb + r;
@@ -805,7 +809,8 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
         // density (new enclosure length) -> row -> col (binClassNew.py:101-117)
         const int y = uni(cand[key / XKEY]), x = key % XKEY, row = y + h - 1;
         const int nl = enc > H - y ? enc : H - y;
-        if (best < 0 || nl < bl || (nl == bl && row > brow) || (nl == bl && row == brow && x < bx)) {
+        if (best < 0 || nl < bl || (nl == bl && row > brow) ||
+            (nl == bl && row == brow && (x < bx || (x == bx && r < best)))) {
             best = r; bl = nl; brow = row; bx = x; by = y;
         }
     }
@@ -985,7 +990,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                 if (tid == 0) ph[1] += tick() - tp;
                 // ---- collision tests, one rotation at a time (whole workgroup) ----
                 int bx = 0, by = 0;
-                const int best = best_placement<DENSE>(P, Wk, mach, rb, nr, H, W, H - b_minrow[b], grid, s_cand,
+                const int best = best_placement<DENSE>(P, Wk, mach, rb, nr, P.best_rot[ty], H, W, H - b_minrow[b], grid, s_cand,
                                                        s_ncand, hmax, band, s_q, s_zw, c_tests, c_algo, c_groups,
                                                        c_mfma, ph, by, bx);
                 if (best >= 0) {
@@ -1114,7 +1119,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P
     __syncthreads();
     unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int bx = 0, by = 0;
-    const int best = best_placement<false>(P, Wk, m, rb, nr, H, W, enc, grid, s_cand, s_ncand, hmax, band, s_q,
+    const int best = best_placement<false>(P, Wk, m, rb, nr, P.best_rot[ty], H, W, enc, grid, s_cand, s_ncand, hmax, band, s_q,
                                            s_zw, c0, c1, c2, c3, ph, by, bx);
     if (tid == 0) {
         int *r = A.res + 4 * blockIdx.x;
