@@ -74,6 +74,9 @@ typedef LDSP const int lds_ci32;
 #define WG_THREADS 256
 #define MAX_ROT 4
 #define GROUP_ROWS 32
+#ifndef GROUP_CAP
+#define GROUP_CAP GROUP_ROWS
+#endif
 #define LDS_RESERVE 2048    // static LDS of the kernel + slack
 #define XKEY 8192           // key = m * XKEY + x ; plates must have W < XKEY
 #define ZW_WAVE (8 * 16)    // zero masks per wave and pass: up to 8 tiles x 16 accumulators
@@ -158,32 +161,9 @@ struct DivBy {
     __device__ __forceinline__ int operator()(int k) const { return d > 1 ? (int)__umulhi((unsigned)k, m) : k; }
 };
 
-// Wave reductions by a bit-by-bit ballot search: each step is one compare
-// and one ballot, with no cross-lane data movement (a shuffle tree costs six
-// dependent LDS round trips).
-//   wave_max_bits<B>: maximum of values in [0, 2^B);
-//   wave_min_key: minimum of decode keys m * XKEY + x (< 32 * XKEY = 2^18) or KEY_NONE.
-template <int B>
-__device__ __forceinline__ int wave_max_bits(int v) {
-    int r = 0;
-#pragma unroll
-    for (int b = B - 1; b >= 0; b--) {
-        const int c = r | (1 << b);
-        if (__ballot(v >= c)) r = c;
-    }
-    return r;
-}
-
-__device__ __forceinline__ int wave_min_key(int v) {
-    if (!__ballot(v != KEY_NONE)) return KEY_NONE;
-    int r = 0;
-#pragma unroll
-    for (int b = 17; b >= 0; b--) {
-        const int c = r | (1 << b);
-        if (!__ballot(v < c)) r = c;
-    }
-    return r;
-}
+// v_writelane_b32: `old` with lane `lane` (a constant) set to the scalar
+// `val`.  The compiler exposes it as an intrinsic only, bound here by name.
+extern "C" __device__ int lane_write(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
 
 // B fragment of the 32x32x32 MFMA for Toeplitz chunk c of one mask row,
 // built in registers from the row's bits.  Q is the mask row padded with 32
@@ -225,13 +205,15 @@ __device__ __forceinline__ void store_zero_masks(const v16i (&acc)[TP], int tcnt
 #pragma unroll
     for (int t = 0; t < TP; t++) {
         if (t < tcnt) {
-            unsigned long long keep = 0;
+            // ballot i (a scalar register pair) goes into lane i of (lo, hi)
+            int lo = 0, hi = 0;
 #pragma unroll
             for (int i = 0; i < 16; i++) {
                 const unsigned long long z = __ballot(acc[t][i] == 0);
-                keep = lane == i ? z : keep;
+                lo = lane_write((int)(unsigned)z, i, lo);
+                hi = lane_write((int)(unsigned)(z >> 32), i, hi);
             }
-            if (lane < 16) zw[t * 16 + lane] = keep;
+            if (lane < 16) zw[t * 16 + lane] = ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
         }
     }
 }
@@ -320,7 +302,20 @@ __device__ __forceinline__ int decode_zero(lds_u64 *zp, int xbase, int xmax, int
             if (b) key = c * XKEY + x0 + (t << 5) + __builtin_ctz(b);
         }
     }
-    return wave_min_key(key);
+    // Minimum key in scalar form.  Row m = s * R + c (segment s, candidate c,
+    // R = 2^rlog): the smallest candidate with a zero wins, and for it the
+    // lowest segment with a zero holds the smallest x (segments start in
+    // non-decreasing order and each covers every position between its start
+    // and the next segment's, so a zero of a later segment left of an earlier
+    // segment's minimum would lie in the earlier segment's range).
+    const unsigned b = (unsigned)__ballot(key != KEY_NONE);
+    if (!b) return KEY_NONE;
+    unsigned bc = b;
+    for (int sh = 16; sh >= (1 << rlog); sh >>= 1) bc |= bc >> sh;   // OR over segments
+    const unsigned rmask = rlog >= 5 ? 0xFFFFFFFFu : ((1u << (1 << rlog)) - 1u);
+    const int cmin = __builtin_ctz(bc & rmask);
+    const unsigned pat = rlog >= 5 ? 1u : 0xFFFFFFFFu / rmask;   // bit s * R of every segment s
+    return __builtin_amdgcn_readlane(key, __builtin_ctz(b & (pat << cmin)));
 }
 
 // A group's candidates across the 32 accumulator rows.  With R = 2^rlog >=
@@ -504,7 +499,7 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
     const int T = (W - w + 1 + 31) >> 5;
     int stride, rcap;
     const int TW = band_window(T, nc, h, band_bytes, wcap, &stride, &rcap);
-    const int maxrows = min(GROUP_ROWS, rcap - h + 1);
+    const int maxrows = min(GROUP_CAP, rcap - h + 1);
     int found = KEY_NONE;
     for (int g0 = 0; g0 < ncand;) {
         const int ytop = uni(cand[g0]);

@@ -2,8 +2,9 @@
 csrc/brkga_decoder.hip): a group of nrows <= R = 2^rlog candidates fills the
 32 accumulator rows with F = 32 / R segments of a window's tiles; passes of
 8/4/2/1 tiles run over every segment at once and the pass loop may stop early
-once candidate 0 has a zero left of every tile still to come.  This restates
-that schedule in Python on random zero patterns and checks it against the
+once candidate 0 has a zero left of every tile still to come; each pass takes
+its minimum key by a scalar selection over the lanes' keys (pass_min_key).
+This restates that schedule in Python on random zero patterns and checks it against the
 plain answer -- the smallest (candidate, x) with zero overlap -- so that a
 wrong segment layout, a gap between segments or a premature exit fails."""
 import random
@@ -22,6 +23,25 @@ def fold_params(nrows, tl):
     return rl, tf
 
 
+def pass_min_key(lane_key, rl):
+    """The decode's scalar selection of a pass's minimum key: the smallest
+    candidate with a zero (lane bits folded over segments), then the lowest
+    segment of that candidate with a zero, whose key is read from its lane."""
+    b = sum(1 << m for m in range(32) if lane_key[m] != NONE)
+    if not b:
+        return NONE
+    bc, sh = b, 16
+    while sh >= (1 << rl):
+        bc |= bc >> sh
+        sh >>= 1
+    rmask = 0xFFFFFFFF if rl >= 5 else (1 << (1 << rl)) - 1
+    low = bc & rmask
+    cmin = (low & -low).bit_length() - 1
+    pat = 1 if rl >= 5 else 0xFFFFFFFF // rmask
+    sel = b & ((pat << cmin) & 0xFFFFFFFF)
+    return lane_key[(sel & -sel).bit_length() - 1]
+
+
 def window_keys(zero, nrows, tl, xbase, xmax, tpm, tpcap, early):
     """zero[c][x]: candidate c has zero overlap at window column x."""
     rl, tf = fold_params(nrows, tl)
@@ -30,7 +50,7 @@ def window_keys(zero, nrows, tl, xbase, xmax, tpm, tpcap, early):
     while t0 < tf:
         rem = tf - t0
         tp = next(t for t in (8, 4, 2, 1) if t <= tpm and t <= tpcap and t <= rem or t == 1)
-        k = NONE
+        lane_key = [NONE] * 32
         for m in range(32):   # lanes 0..31, one accumulator row each
             c = m & ((1 << rl) - 1)
             if c >= nrows:
@@ -39,7 +59,9 @@ def window_keys(zero, nrows, tl, xbase, xmax, tpm, tpcap, early):
                 for n in range(32):
                     x = xbase + segx[m] + 32 * (t0 + t) + n
                     if x <= xmax and zero[c][x - xbase]:
-                        k = min(k, c * XKEY + x)
+                        lane_key[m] = min(lane_key[m], c * XKEY + x)
+        k = pass_min_key(lane_key, rl)
+        assert k == min(lane_key)
         key = min(key, k)
         t0 += tp
         if early and key < xbase + 32 * t0:
