@@ -74,9 +74,6 @@ typedef LDSP const int lds_ci32;
 #define WG_THREADS 256
 #define MAX_ROT 4
 #define GROUP_ROWS 32
-#ifndef GROUP_CAP
-#define GROUP_CAP GROUP_ROWS
-#endif
 #define LDS_RESERVE 2048    // static LDS of the kernel + slack
 #define XKEY 8192           // key = m * XKEY + x ; plates must have W < XKEY
 #define ZW_WAVE (8 * 16)    // zero masks per wave and pass: up to 8 tiles x 16 accumulators
@@ -352,12 +349,18 @@ __device__ __forceinline__ int one_pass(lds_cu8 *band, int stride, const Fold &f
     return decode_zero<TP>(zp, xbase + (t0 << 5), xmax, nrows, f.rlog, f.segx);
 }
 
-// All passes over one staged x-window (tiles [0, tl) of the window), run by
-// the whole workgroup.  The window is cut into passes of 8/4/2/1 tiles (no
-// MFMA is spent on tiles past the window).  Returns the window's minimum key;
-// with `early`, stops once candidate 0 has a zero left of every tile still to
-// come (a key below xbase + 32 * t0; in a folded group the later passes of
-// segment 0 come before the current ones of the other segments).
+// All passes over one staged x-window (tiles [0, tl) of each segment), run
+// by the whole workgroup.  The window is cut into passes of 8/4/2/1 tiles; a
+// remainder of 6 or 7 tiles takes one 8-tile pass and a remainder of 3 one
+// 4-tile pass (one or two tiles of MFMAs cost less than the extra pass's
+// zero-mask exchange and barrier).  Tiles past the staged window read
+// whatever the band holds there (at most BAND_SLACK bytes past the band's
+// last row stay inside the allocation); the caller's xmax ends at the
+// window's last column, so their counts are never decoded.  Returns the
+// window's minimum key; with `early`, stops once candidate 0 has a zero left
+// of every tile still to come (a key below xbase + 32 * t0; in a folded group
+// the later passes of segment 0 come before the current ones of the other
+// segments).
 template <int NC>
 __device__ __attribute__((always_inline)) int window_passes(lds_cu8 *band, int stride, const Fold &f, int h,
                              lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
@@ -369,10 +372,10 @@ __device__ __attribute__((always_inline)) int window_passes(lds_cu8 *band, int s
     for (int t0 = 0; t0 < tl;) {
         const int rem = tl - t0;
         int k, tp;
-        if (TPM >= 8 && tpcap >= 8 && rem >= 8) {
+        if (TPM >= 8 && tpcap >= 8 && rem >= 6) {
             k = one_pass<(TPM >= 8 ? 8 : 1), NC>(band, stride, f, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 8;
-        } else if (TPM >= 4 && tpcap >= 4 && rem >= 4) {
+        } else if (TPM >= 4 && tpcap >= 4 && rem >= 3) {
             k = one_pass<(TPM >= 4 ? 4 : 1), NC>(band, stride, f, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 4;
         } else if (TPM >= 2 && tpcap >= 2 && rem >= 2) {
@@ -439,15 +442,19 @@ __host__ __device__ __forceinline__ int band_stride(int tw, int nc) {
     return ((((tw + nc - 1) << 5) + 127) & ~127) + 16;
 }
 
-// Largest x-window (in tiles, <= cap) whose 32-row band fits; shrinks the
-// window rather than the row group.
+// Bytes kept free after the band's last row: a pass rounded up past its
+// window reads at most two tiles (64 bytes) beyond a row's staged columns.
+#define BAND_SLACK 64
+
+// Largest x-window (in tiles, <= cap) whose 32-row band fits (with
+// BAND_SLACK bytes to spare); shrinks the window rather than the row group.
 __host__ __device__ __forceinline__ int band_window(int T, int nc, int h, int band_bytes, int cap,
                                                     int *stride_out, int *rcap_out) {
     int tw = T < cap ? T : cap;
     int st, rc;
     for (;;) {
         st = band_stride(tw, nc);
-        rc = band_bytes / st;
+        rc = (band_bytes - BAND_SLACK) / st;
         if (rc >= h + GROUP_ROWS - 1 || tw == 1) break;
         tw = (tw + 1) >> 1;
     }
@@ -499,7 +506,7 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
     const int T = (W - w + 1 + 31) >> 5;
     int stride, rcap;
     const int TW = band_window(T, nc, h, band_bytes, wcap, &stride, &rcap);
-    const int maxrows = min(GROUP_CAP, rcap - h + 1);
+    const int maxrows = min(GROUP_ROWS, rcap - h + 1);
     int found = KEY_NONE;
     for (int g0 = 0; g0 < ncand;) {
         const int ytop = uni(cand[g0]);
@@ -542,8 +549,8 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
             f.segx = min((m >> f.rlog) * tf, tl - tf) << 5;
             f.loff = (cand[g0 + min(m & ((1 << f.rlog) - 1), nrows - 1)] - ylo) * stride + f.segx;
             f.skip = skip;
-            const int k = window_dispatch(nc, band, stride, f, h, s_q, tf, tw0 << 5, W - w,
-                                          nrows, s_zw, mfma, tpcap, !DENSE);
+            const int k = window_dispatch(nc, band, stride, f, h, s_q, tf, tw0 << 5,
+                                          min(W - w, ((tw0 + tl) << 5) - 1), nrows, s_zw, mfma, tpcap, !DENSE);
             gkey = k < gkey ? k : gkey;
             if (tid == 0) ph[3] += tick() - t1;
             if (!DENSE && gkey < XKEY) break;   // candidate 0 has a zero: nothing further left can win
@@ -1459,7 +1466,7 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
             const int Tt = (W - rw + 1 + 31) / 32;
             need_full = std::max(need_full, (size_t)(rh + GROUP_ROWS - 1) *
                                                 band_stride(std::min(Tt, WINDOW_CAP), nc[r]));
-            need_min = std::max(need_min, (size_t)rh * band_stride(1, nc[r]));
+            need_min = std::max(need_min, (size_t)rh * band_stride(1, nc[r]) + BAND_SLACK);
         }
     size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (size_t)prop.maxSharedMemoryPerMultiProcessor : 65536;
     // Two workgroups per CU when a 32-row group of every rotation fits with at
@@ -1470,7 +1477,7 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
             const int H = p->plate_h[m], W = p->plate_w[m], rh = p->rot_h[r], rw = p->rot_w[r];
             if (rh > H || rw > W) continue;
             const int Tt = (W - rw + 1 + 31) / 32;
-            need_4 = std::max(need_4, (size_t)(rh + GROUP_ROWS - 1) * band_stride(std::min(Tt, 4), nc[r]));
+            need_4 = std::max(need_4, (size_t)(rh + GROUP_ROWS - 1) * band_stride(std::min(Tt, 4), nc[r]) + BAND_SLACK);
         }
     (void)need_full;
     int wpc2 = opt.wgs_per_cu > 0 ? opt.wgs_per_cu : (fixed + need_4 + LDS_RESERVE <= lds_cu / 2 ? 2 : 1);

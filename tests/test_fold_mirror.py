@@ -1,7 +1,8 @@
 """Folded correlation groups (Fold, window_passes and decode_zero in
 csrc/brkga_decoder.hip): a group of nrows <= R = 2^rlog candidates fills the
 32 accumulator rows with F = 32 / R segments of a window's tiles; passes of
-8/4/2/1 tiles run over every segment at once and the pass loop may stop early
+8/4/2/1 tiles (a remainder of 6-7 tiles rounded up to 8, of 3 to 4) run over
+every segment at once and the pass loop may stop early
 once candidate 0 has a zero left of every tile still to come; each pass takes
 its minimum key by a scalar selection over the lanes' keys (pass_min_key).
 This restates that schedule in Python on random zero patterns and checks it against the
@@ -42,14 +43,26 @@ def pass_min_key(lane_key, rl):
     return lane_key[(sel & -sel).bit_length() - 1]
 
 
+def pass_tiles(rem, tpm, tpcap):
+    """Tiles of the next pass: 8 for a remainder of 6 or more, 4 for 3 or
+    more, 2 for 2, within the register limit tpm and the tuned cap tpcap."""
+    for t, need in ((8, 6), (4, 3), (2, 2)):
+        if tpm >= t and tpcap >= t and rem >= need:
+            return t
+    return 1
+
+
 def window_keys(zero, nrows, tl, xbase, xmax, tpm, tpcap, early):
-    """zero[c][x]: candidate c has zero overlap at window column x."""
+    """zero[c][x]: candidate c has zero overlap at window column x < 32 * tl;
+    columns past the window (read by a rounded-up pass) hold garbage, which
+    is modelled as zeros that the window-end limit must discard."""
     rl, tf = fold_params(nrows, tl)
     segx = [min((m >> rl) * tf, tl - tf) * 32 for m in range(32)]
+    xend = min(xmax, xbase + 32 * tl - 1)
     key, t0 = NONE, 0
     while t0 < tf:
         rem = tf - t0
-        tp = next(t for t in (8, 4, 2, 1) if t <= tpm and t <= tpcap and t <= rem or t == 1)
+        tp = pass_tiles(rem, tpm, tpcap)
         lane_key = [NONE] * 32
         for m in range(32):   # lanes 0..31, one accumulator row each
             c = m & ((1 << rl) - 1)
@@ -58,7 +71,8 @@ def window_keys(zero, nrows, tl, xbase, xmax, tpm, tpcap, early):
             for t in range(tp):
                 for n in range(32):
                     x = xbase + segx[m] + 32 * (t0 + t) + n
-                    if x <= xmax and zero[c][x - xbase]:
+                    garbage = x - xbase >= 32 * tl
+                    if x <= xend and (garbage or zero[c][x - xbase]):
                         lane_key[m] = min(lane_key[m], c * XKEY + x)
         k = pass_min_key(lane_key, rl)
         assert k == min(lane_key)
@@ -76,7 +90,7 @@ def test_folded_schedule_finds_the_smallest_key():
         nrows = rng.randint(1, 32)
         xbase = 32 * rng.randint(0, 3)
         width = 32 * tl
-        xmax = xbase + rng.randint(0, width - 1)
+        xmax = xbase + rng.randint(0, width + 64)   # may lie past the window
         p = rng.choice([0.0, 0.002, 0.01, 0.05, 0.3])
         zero = [[rng.random() < p for _ in range(width)] for _ in range(nrows)]
         want = min([c * XKEY + xbase + x for c in range(nrows) for x in range(width)
