@@ -104,6 +104,7 @@ struct DevProblem {
     // per pass, cap on x-tiles per staged LDS window}.  Any values give the
     // same results; only the speed changes.
     const unsigned char *tiles;
+    const int *type_cost;   // [n_types] expected work of placing one part: its mask's bounding-box area
 };
 
 struct DevWork {
@@ -126,6 +127,7 @@ struct LaunchArgs {
     long long n_chrom;
     double *ctx_ms;          // [n_chrom * M]
     int *log;                // nullable [n_chrom][n_parts][6]
+    const int *order;        // nullable [n_chrom * M]: contexts in decode order
 };
 
 // Constant-rate (100 MHz on MI300-class parts) wall clock, read by thread 0
@@ -637,7 +639,65 @@ __device__ __forceinline__ void rows_free_run(lds_cu8 *band, int rstride, int ro
 
 struct __align__(8) PlaceRes { int has, y, x, pad; };
 
-__global__ void reset_counters_kernel(unsigned int *queue) { *queue = 0u; }
+// Context order.  A launch decodes n_chrom * M contexts on a fixed set of
+// persistent workgroups; contexts of one population differ several-fold in
+// work, and taking the heaviest first (longest-processing-time-first list
+// scheduling) keeps a long context from starting last.  The expected work of
+// a context is the summed bounding-box area of the masks it places, counted
+// into ORDER_BUCKETS buckets over [0, twice the mean); a counting sort from
+// the heaviest bucket down gives the order.  Any order gives the same
+// results: a context's outputs do not depend on the workgroup or the time it
+// runs.
+#define ORDER_BUCKETS 1024
+
+__global__ void reset_counters_kernel(unsigned int *queue, int *hist) {
+    if (threadIdx.x == 0) *queue = 0u;
+    for (int b = threadIdx.x; b < ORDER_BUCKETS; b += blockDim.x) hist[b] = 0;
+}
+
+// One thread per context: bucket = cost * ORDER_BUCKETS * M / (2 * cost_all),
+// cost_all = the summed cost of every slot (so M contexts average cost_all / M).
+__global__ void order_cost_kernel(DevProblem P, const void *keys, int key_f64, long long n_chrom,
+                                  long long cost_all, int *bucket, int *hist) {
+    const int M = P.n_machines, n = P.n_parts;
+    const long long ctx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (ctx >= n_chrom * M) return;
+    const long long chrom = ctx / M;
+    const int m = (int)(ctx % M);
+    long long cost = 0;
+    for (int j = 0; j < n; j++)
+        if (machine_of(load_key(keys, key_f64, chrom * 2LL * n + n + j), P.thresholds, M) == m)
+            cost += P.type_cost[P.slot_type[j]];
+    const int b = (int)min((long long)ORDER_BUCKETS - 1, cost * ORDER_BUCKETS * M / (2 * cost_all));
+    bucket[ctx] = b;
+    atomicAdd(&hist[b], 1);
+}
+
+// One workgroup of ORDER_BUCKETS threads: start of each bucket in the order
+// (heaviest bucket first, an exclusive scan), then every context takes the
+// next slot of its bucket.
+__global__ __launch_bounds__(ORDER_BUCKETS) void order_scatter_kernel(const int *bucket, const int *hist,
+                                                                      long long n_ctx, int *order) {
+    __shared__ int start[ORDER_BUCKETS];
+    const int t = threadIdx.x;
+    int v = hist[ORDER_BUCKETS - 1 - t];   // position t in descending bucket order
+    start[t] = v;
+    __syncthreads();
+    for (int d = 1; d < ORDER_BUCKETS; d <<= 1) {   // inclusive scan (Hillis-Steele)
+        const int o = t >= d ? start[t - d] : 0;
+        __syncthreads();
+        start[t] += o;
+        __syncthreads();
+    }
+    const int excl = start[t] - v;
+    __syncthreads();
+    start[ORDER_BUCKETS - 1 - t] = excl;   // indexed by bucket again
+    __syncthreads();
+    for (long long c = t; c < n_ctx; c += ORDER_BUCKETS) {
+        const int pos = atomicAdd(&start[bucket[c]], 1);
+        if (pos < n_ctx) order[pos] = (int)c;   // always true: the buckets count every context once
+    }
+}
 
 // Vacancy filter of one rotation, run by one wave: bit l of word k says top
 // row y = 64k + l passes, i.e. every grid row y+a (a < h) has a free run >=
@@ -920,12 +980,16 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
     unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ctx_max = 0, ctx_sum = 0;
     const long long n_ctx = L.n_chrom * M;
 
-    for (;;) {
+    for (bool first = true;; first = false) {
         unsigned long long tc0 = tid == 0 ? tick() : 0, tp = tc0;
-        if (tid == 0) s_ctx = (int)atomicAdd(Wk.queue, 1u);
+        // workgroup b starts with position b of the order (the launch has at
+        // most n_ctx workgroups), then takes positions from the queue
+        if (tid == 0) s_ctx = first ? (int)blockIdx.x : (int)gridDim.x + (int)atomicAdd(Wk.queue, 1u);
         __syncthreads();
-        const long long ctx = uni(s_ctx);
-        if (ctx >= n_ctx) break;
+        const long long q = uni(s_ctx);
+        if (q >= n_ctx) break;
+        const long long ctx = L.order ? uni(L.order[q]) : q;
+        if (ctx < 0 || ctx >= n_ctx) continue;   // never: the order is a permutation of [0, n_ctx)
         const long long chrom = ctx / M;
         const int mach = (int)(ctx % M);
         const int H = P.plate_h[mach], W = P.plate_w[mach];
@@ -1192,6 +1256,9 @@ struct brkga_handle {
     double *d_out = nullptr; size_t d_out_n = 0;
     int *d_log = nullptr; size_t d_log_bytes = 0;
     double *d_ctx = nullptr; size_t d_ctx_n = 0;
+    int *d_order = nullptr, *d_bucket = nullptr;   // [d_ctx_n] context order and cost buckets
+    int *d_hist = nullptr;                          // [ORDER_BUCKETS]
+    long long cost_all = 1;                         // summed context cost of all slots
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool ev_pending = false;
     double kernel_ms = 0.0, last_ms = 0.0;
@@ -1428,6 +1495,13 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
             h->tiles[((size_t)m * TUNE_NC + c) * 2 + 1] = (unsigned char)WINDOW_CAP;
         }
     if ((rc = upload(h, h->tiles.data(), h->tiles.size(), &d.tiles))) return bail(rc);
+    // context cost for the decode order: bounding-box area of rotation 0
+    std::vector<int> type_cost(T);
+    for (int t = 0; t < T; t++) type_cost[t] = p->rot_h[p->rot_base[t]] * p->rot_w[p->rot_base[t]];
+    h->cost_all = 0;
+    for (int j = 0; j < p->n_parts; j++) h->cost_all += type_cost[p->slot_type[j]];
+    h->cost_all = std::max(h->cost_all, 1LL);
+    if ((rc = upload(h, type_cost.data(), type_cost.size(), &d.type_cost))) return bail(rc);
     h->d_tiles = (unsigned char *)d.tiles;
     h->prob.thresholds = nullptr;   // host pointers are not kept
     DevWork &w = h->dw;
@@ -1521,9 +1595,15 @@ int brkga_internal_reserve(brkga_handle *h, int64_t n_chrom) {
     const int M = h->prob.n_machines;
     if ((size_t)(n_chrom * M) > h->d_ctx_n) {
         if (h->d_ctx) { (void)hipFree(h->d_ctx); h->d_ctx = nullptr; }
+        if (h->d_order) { (void)hipFree(h->d_order); h->d_order = nullptr; }
+        if (h->d_bucket) { (void)hipFree(h->d_bucket); h->d_bucket = nullptr; }
+        h->d_ctx_n = 0;
         HIPCHK(h, hipMalloc(&h->d_ctx, (size_t)n_chrom * M * sizeof(double)));
+        HIPCHK(h, hipMalloc(&h->d_order, (size_t)n_chrom * M * sizeof(int)));
+        HIPCHK(h, hipMalloc(&h->d_bucket, (size_t)n_chrom * M * sizeof(int)));
         h->d_ctx_n = (size_t)n_chrom * M;
     }
+    if (!h->d_hist) HIPCHK(h, hipMalloc(&h->d_hist, ORDER_BUCKETS * sizeof(int)));
     return 0;
 }
 
@@ -1535,10 +1615,16 @@ int brkga_internal_launch(brkga_handle *h, const void *d_keys, int key_dtype, in
     if (timed) settle_events(h);
     LaunchArgs L;
     L.keys = d_keys; L.key_f64 = key_dtype == BRKGA_KEYS_F64; L.n_chrom = n_chrom;
-    L.ctx_ms = h->d_ctx; L.log = d_log;
+    L.ctx_ms = h->d_ctx; L.log = d_log; L.order = h->d_order;
     const long long n_ctx = n_chrom * M;
     const int grid = (int)std::min<long long>(h->n_wg, std::max<long long>(n_ctx, 1));
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(1), 0, s, h->dw.queue);
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(256), 0, s, h->dw.queue, h->d_hist);
+    HIPCHK(h, hipGetLastError());
+    hipLaunchKernelGGL(order_cost_kernel, dim3((unsigned)((n_ctx + 255) / 256)), dim3(256), 0, s, h->dp, d_keys,
+                       L.key_f64, (long long)n_chrom, h->cost_all, h->d_bucket, h->d_hist);
+    HIPCHK(h, hipGetLastError());
+    hipLaunchKernelGGL(order_scatter_kernel, dim3(1), dim3(ORDER_BUCKETS), 0, s, h->d_bucket, h->d_hist, n_ctx,
+                       h->d_order);
     HIPCHK(h, hipGetLastError());
     if (timed) HIPCHK(h, hipEventRecord(h->ev0, s));
     auto kern = h->dense ? decode_kernel<true> : decode_kernel<false>;
@@ -1879,6 +1965,9 @@ extern "C" void brkga_destroy(brkga_handle *h) {
     if (h->d_out) (void)hipFree(h->d_out);
     if (h->d_log) (void)hipFree(h->d_log);
     if (h->d_ctx) (void)hipFree(h->d_ctx);
+    if (h->d_order) (void)hipFree(h->d_order);
+    if (h->d_bucket) (void)hipFree(h->d_bucket);
+    if (h->d_hist) (void)hipFree(h->d_hist);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     delete h;
