@@ -982,9 +982,15 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
 
     for (bool first = true;; first = false) {
         unsigned long long tc0 = tid == 0 ? tick() : 0, tp = tc0;
-        // workgroup b starts with position b of the order (the launch has at
-        // most n_ctx workgroups), then takes positions from the queue
-        if (tid == 0) s_ctx = first ? (int)blockIdx.x : (int)gridDim.x + (int)atomicAdd(Wk.queue, 1u);
+        // The first G = gridDim.x positions (G <= n_ctx) go one to each
+        // workgroup, later ones come from the queue.  Workgroups b and b + G/2
+        // of a full launch share a compute unit, so b < G/2 takes position b
+        // and b + G/2 position G - 1 - b: the heaviest context shares its CU
+        // with the lightest of the first round, which finishes early.
+        if (tid == 0) {
+            const int G = (int)gridDim.x, b = (int)blockIdx.x;
+            s_ctx = first ? (b < G / 2 ? b : G / 2 + (G - 1 - b)) : G + (int)atomicAdd(Wk.queue, 1u);
+        }
         __syncthreads();
         const long long q = uni(s_ctx);
         if (q >= n_ctx) break;
