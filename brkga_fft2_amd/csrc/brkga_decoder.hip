@@ -650,37 +650,45 @@ struct __align__(8) PlaceRes { int has, y, x, pad; };
 // runs.
 #define ORDER_BUCKETS 1024
 
-__global__ void reset_counters_kernel(unsigned int *queue, int *hist) {
-    if (threadIdx.x == 0) *queue = 0u;
-    for (int b = threadIdx.x; b < ORDER_BUCKETS; b += blockDim.x) hist[b] = 0;
-}
-
-// One thread per context: bucket = cost * ORDER_BUCKETS * M / (2 * cost_all),
-// cost_all = the summed cost of every slot (so M contexts average cost_all / M).
-__global__ void order_cost_kernel(DevProblem P, const void *keys, int key_f64, long long n_chrom,
-                                  long long cost_all, int *bucket, int *hist) {
-    const int M = P.n_machines, n = P.n_parts;
-    const long long ctx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (ctx >= n_chrom * M) return;
+// One wave per context, four per workgroup: the lanes take the slots and a
+// butterfly adds their costs; bucket = cost * ORDER_BUCKETS * M / (2 *
+// cost_all), cost_all = the summed cost of every slot (so the M contexts of a
+// chromosome average cost_all / M).  hist starts at zero (order_scatter_kernel
+// clears it after use; brkga_create before the first launch).
+__global__ __launch_bounds__(256) void order_cost_kernel(DevProblem P, const void *keys, int key_f64,
+                                                         long long n_chrom, long long cost_all, int *bucket,
+                                                         int *hist) {
+    const int M = P.n_machines, n = P.n_parts, lane = threadIdx.x & 63;
+    const long long ctx = blockIdx.x * 4LL + (threadIdx.x >> 6);
+    if (ctx >= n_chrom * M) return;   // the whole wave
     const long long chrom = ctx / M;
     const int m = (int)(ctx % M);
     long long cost = 0;
-    for (int j = 0; j < n; j++)
+    for (int j = lane; j < n; j += 64)
         if (machine_of(load_key(keys, key_f64, chrom * 2LL * n + n + j), P.thresholds, M) == m)
             cost += P.type_cost[P.slot_type[j]];
-    const int b = (int)min((long long)ORDER_BUCKETS - 1, cost * ORDER_BUCKETS * M / (2 * cost_all));
-    bucket[ctx] = b;
-    atomicAdd(&hist[b], 1);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) cost += __shfl_xor(cost, d, 64);
+    if (lane == 0) {
+        const int b = (int)min((long long)ORDER_BUCKETS - 1, cost * ORDER_BUCKETS * M / (2 * cost_all));
+        bucket[ctx] = b;
+        atomicAdd(&hist[b], 1);
+    }
 }
 
 // One workgroup of ORDER_BUCKETS threads: start of each bucket in the order
 // (heaviest bucket first, an exclusive scan), then every context takes the
-// next slot of its bucket.
-__global__ __launch_bounds__(ORDER_BUCKETS) void order_scatter_kernel(const int *bucket, const int *hist,
-                                                                      long long n_ctx, int *order) {
+// next slot of its bucket.  Clears the histogram and the decode's queue
+// counter for the next launch.  Should the histogram not count n_ctx contexts
+// (it is left dirty only if an earlier launch failed between the two order
+// kernels), the order is the identity.
+__global__ __launch_bounds__(ORDER_BUCKETS) void order_scatter_kernel(const int *bucket, int *hist, long long n_ctx,
+                                                                      int *order, unsigned int *queue) {
     __shared__ int start[ORDER_BUCKETS];
     const int t = threadIdx.x;
-    int v = hist[ORDER_BUCKETS - 1 - t];   // position t in descending bucket order
+    const int v = hist[ORDER_BUCKETS - 1 - t];   // position t in descending bucket order
+    hist[ORDER_BUCKETS - 1 - t] = 0;
+    if (t == 0) *queue = 0u;
     start[t] = v;
     __syncthreads();
     for (int d = 1; d < ORDER_BUCKETS; d <<= 1) {   // inclusive scan (Hillis-Steele)
@@ -690,10 +698,12 @@ __global__ __launch_bounds__(ORDER_BUCKETS) void order_scatter_kernel(const int 
         __syncthreads();
     }
     const int excl = start[t] - v;
+    const bool counted = start[ORDER_BUCKETS - 1] == n_ctx;
     __syncthreads();
     start[ORDER_BUCKETS - 1 - t] = excl;   // indexed by bucket again
     __syncthreads();
     for (long long c = t; c < n_ctx; c += ORDER_BUCKETS) {
+        if (!counted) { order[c] = (int)c; continue; }
         const int pos = atomicAdd(&start[bucket[c]], 1);
         if (pos < n_ctx) order[pos] = (int)c;   // always true: the buckets count every context once
     }
@@ -1609,7 +1619,10 @@ int brkga_internal_reserve(brkga_handle *h, int64_t n_chrom) {
         HIPCHK(h, hipMalloc(&h->d_bucket, (size_t)n_chrom * M * sizeof(int)));
         h->d_ctx_n = (size_t)n_chrom * M;
     }
-    if (!h->d_hist) HIPCHK(h, hipMalloc(&h->d_hist, ORDER_BUCKETS * sizeof(int)));
+    if (!h->d_hist) {
+        HIPCHK(h, hipMalloc(&h->d_hist, ORDER_BUCKETS * sizeof(int)));
+        HIPCHK(h, hipMemset(h->d_hist, 0, ORDER_BUCKETS * sizeof(int)));   // cleared by each launch after use
+    }
     return 0;
 }
 
@@ -1624,13 +1637,11 @@ int brkga_internal_launch(brkga_handle *h, const void *d_keys, int key_dtype, in
     L.ctx_ms = h->d_ctx; L.log = d_log; L.order = h->d_order;
     const long long n_ctx = n_chrom * M;
     const int grid = (int)std::min<long long>(h->n_wg, std::max<long long>(n_ctx, 1));
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(256), 0, s, h->dw.queue, h->d_hist);
-    HIPCHK(h, hipGetLastError());
-    hipLaunchKernelGGL(order_cost_kernel, dim3((unsigned)((n_ctx + 255) / 256)), dim3(256), 0, s, h->dp, d_keys,
+    hipLaunchKernelGGL(order_cost_kernel, dim3((unsigned)((n_ctx + 3) / 4)), dim3(256), 0, s, h->dp, d_keys,
                        L.key_f64, (long long)n_chrom, h->cost_all, h->d_bucket, h->d_hist);
     HIPCHK(h, hipGetLastError());
     hipLaunchKernelGGL(order_scatter_kernel, dim3(1), dim3(ORDER_BUCKETS), 0, s, h->d_bucket, h->d_hist, n_ctx,
-                       h->d_order);
+                       h->d_order, h->dw.queue);
     HIPCHK(h, hipGetLastError());
     if (timed) HIPCHK(h, hipEventRecord(h->ev0, s));
     auto kern = h->dense ? decode_kernel<true> : decode_kernel<false>;
