@@ -196,6 +196,43 @@ def test_determinism_and_batch_invariance():
     dec.close()
 
 
+def test_chromosome_462_batch_sweep_tiling_and_poison_rows():
+    """verify_native_vs_pp.py:120-201 on the seed-42 P50M2-0 population: chromosome
+    462 decodes to the same makespan (the golden 448166.8336) at the end of batches
+    of 1-500 rows (A), at its own row or swapped to row 0, and in a batch of 500
+    copies of itself (B), twice in a row (C), and in that tiled batch with one row
+    replaced by another chromosome (D).  Batch sizes span fewer, as many and more
+    contexts than the persistent workgroups, so both the first-round assignment and
+    the queue decode it."""
+    _say("chromosome 462 sweep / tiling / poison rows")
+    prob, n, M, packed = ref_instance("P50M2-0")
+    np.random.seed(42)
+    pop = np.random.uniform(0.0, 1.0, size=(500, 100)).astype(np.float32)
+    chosen = pop[462].copy()
+    want = oracle.decode(packed, pop[462:463])[0][0]
+    assert abs(want - 448166.8336) < 1e-3
+    dec = _decoder(packed)
+    for N in (1, 100, 200, 250, 256, 257, 300, 400, 463, 500):   # A
+        batch = pop[:N].copy()
+        batch[N - 1] = chosen
+        assert dec.evaluate(batch)[0][N - 1] == want, N
+    full = dec.evaluate(pop)[0]                                  # B
+    assert full[462] == want
+    swapped = pop.copy()
+    swapped[[0, 462]] = swapped[[462, 0]]
+    assert dec.evaluate(swapped)[0][0] == want
+    tiled = np.tile(chosen, (500, 1))
+    t = dec.evaluate(tiled)[0]
+    assert np.all(t == want)
+    np.testing.assert_array_equal(dec.evaluate(pop)[0], full)    # C
+    for k in (0, 1, 50, 100, 200, 300, 400, 461, 463):           # D
+        batch = tiled.copy()
+        batch[k] = pop[k]
+        got = dec.evaluate(batch)[0]
+        assert got[462] == want and got[k] == full[k], k
+    dec.close()
+
+
 def test_bin_capacity_growth():
     """A workspace of one bin per context overflows and is grown transparently."""
     _say("capacity growth")
