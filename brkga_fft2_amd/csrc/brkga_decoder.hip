@@ -654,7 +654,7 @@ struct __align__(8) PlaceRes { int has, y, x, pad; };
 // butterfly adds their costs; bucket = cost * ORDER_BUCKETS * M / (2 *
 // cost_all), cost_all = the summed cost of every slot (so the M contexts of a
 // chromosome average cost_all / M).  hist starts at zero (order_scatter_kernel
-// clears it after use; brkga_create before the first launch).
+// clears it after use; it is zeroed when first allocated).
 __global__ __launch_bounds__(256) void order_cost_kernel(DevProblem P, const void *keys, int key_f64,
                                                          long long n_chrom, long long cost_all, int *bucket,
                                                          int *hist) {
