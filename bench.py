@@ -91,6 +91,59 @@ def _batched_vs_single(config, seed, dev, stream, pop=1024, launches=3, singles=
             "batched_over_single": round(batched / single, 2)}
 
 
+# The reference's published decoder timings (NVIDIA RTX A4000, its native
+# C++/CUDA decoder with the cuFFT collision test): one evaluate_batch call on a
+# population of 10 n chromosomes drawn by np.random.default_rng(127), mean of 5
+# calls, the faster of its 1- and 2-worker runs.  Source:
+# benchmarks/20260422_163051_NVIDIA_RTX_A4000_w1-2_stage3-a4000-baseline.json
+# (mean_s at :29,58 P25M2-2; :87,116 P25M2-4; :145,174 P50M2-1; :203,232
+# P50M2-3; :261,290 P75M2-0; :319,348 P75M2-4; :377,406 P100M4-0; :435,464
+# P100M4-3).
+REFERENCE_S_PER_CALL = {"P25M2-2": 0.21187, "P25M2-4": 0.19037, "P50M2-1": 0.75892, "P50M2-3": 0.68457,
+                        "P75M2-0": 1.60600, "P75M2-4": 1.78770, "P100M4-0": 5.47184, "P100M4-3": 5.40797}
+
+
+def _reference_instances(reps=5):
+    """The same workloads on this GPU: each instance's fingerprint population
+    (host float32 keys in, host makespans out, as evaluate_batch takes and
+    returns them), mean of ``reps`` calls after one warm-up call; the first five
+    feasible makespans and the feasible count are checked against the values
+    the reference recorded for the same population (reported; the GPU tests
+    are the parity gate)."""
+    import numpy as np
+    from brkga_fft2_amd.native import Decoder
+    from brkga_fft2_amd.problem import load_reference_instance, pack_problem, thresholds_for
+    golden = os.path.join(ROOT, "tests", "golden")
+    with open(os.path.join(golden, "fingerprints.json")) as f:
+        fps = json.load(f)
+    out = []
+    for name, ref_s in REFERENCE_S_PER_CALL.items():
+        fp = fps[name]
+        prob, n, M = load_reference_instance(golden, name)
+        packed = pack_problem(prob, n, M, thresholds_for(M), prob.instance_parts)
+        rng = np.random.default_rng(fp["seed"])
+        keys = rng.random((fp["num_individuals"], 2 * n), dtype=np.float64).astype(np.float32)
+        d = Decoder(packed, device=0)
+        ms, _ = d.evaluate(keys)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ms, _ = d.evaluate(keys)
+        sec = (time.perf_counter() - t0) / reps
+        d.close()
+        feas = [float(m) for m in ms if m < 1e15]
+        ok = ["%.6f" % m for m in feas[:5]] == fp["fingerprint"] and len(feas) == fp["n_feasible"]
+        pop = len(keys)
+        out.append({"instance": name, "population": pop, "s_per_call": round(sec, 5),
+                    "chromosomes_per_s": round(pop / sec, 1),
+                    "reference_a4000_s_per_call": ref_s,
+                    "reference_chromosomes_per_s": round(pop / ref_s, 1),
+                    "speedup": round(ref_s / sec, 1), "fingerprint_ok": bool(ok)})
+        print("bench: %s population %d: %.5f s/call (%.0f chrom/s) vs reference A4000 %.5f s/call, "
+              "%.0fx, fingerprint %s" % (name, pop, sec, pop / sec, ref_s, ref_s / sec, "ok" if ok else "MISMATCH"),
+              file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +167,8 @@ def main():
                     help="skip timing the greedy initial solution")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the C3 batched vs single-chromosome comparison")
+    ap.add_argument("--no-ref", action="store_true",
+                    help="skip timing the reference's own instances and populations")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB ten times (calibrates the "
                          "profiler's HBM byte counters against a known size) and exit")
@@ -328,6 +383,8 @@ def main():
               (c3["batched_per_s"], c3["single_per_s"], c3["batched_over_single"]),
               file=sys.stderr, flush=True)
 
+    ref_inst = None if args.no_ref else _reference_instances()
+
     # ---- wall time to a target makespan: surrogate-screened vs plain BRKGA ----
     ttt = None
     if not args.no_ttt:
@@ -380,6 +437,7 @@ def main():
     line["autotune"] = tuned
     line["initial_solution"] = init
     line["c3_batched_vs_single"] = c3
+    line["reference_instances"] = ref_inst
     line["time_to_target"] = ttt
     print(json.dumps(line), flush=True)
 
