@@ -201,6 +201,10 @@ class Decoder:
             raise ValueError("keys must be contiguous; out must be contiguous float64")
         if keys.dtype not in (torch.float32, torch.float64):
             raise ValueError("keys must be float32 or float64")
+        if keys.dim() != 2 or out.dim() != 1 or out.shape[0] != keys.shape[0]:
+            raise ValueError("keys must be [N, 2*n_parts] and out [N]")
+        if keys.shape[0] == 0:
+            return out
         dt = BRKGA_KEYS_F64 if keys.dtype == torch.float64 else BRKGA_KEYS_F32
         if stream is None:
             stream = torch.cuda.current_stream(keys.device)

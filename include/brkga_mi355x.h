@@ -23,7 +23,9 @@
  * Conventions: every call returns 0 on success or a negative BRKGA_E* code;
  * no C++ exception crosses the ABI; host buffers belong to the caller; device
  * workspaces belong to the handle.  A handle is not re-entrant: calls on one
- * handle must not overlap (one handle per thread).
+ * handle must not overlap (one handle per thread), and the device work of its
+ * calls must not overlap either: every launch through a handle uses the same
+ * workspace, so issue them on one stream or order the streams with events.
  */
 #ifndef BRKGA_MI355X_H
 #define BRKGA_MI355X_H

@@ -261,3 +261,31 @@ def test_float64_keys_and_device_entry_point():
     torch.cuda.synchronize()
     np.testing.assert_array_equal(out.cpu().numpy(), ref_ms)
     dec.close()
+
+
+def test_empty_population_and_shape_checks():
+    """An empty population decodes to nothing on both entry points; a mismatched
+    output buffer is refused before any launch; the decoder still agrees with the
+    oracle afterwards."""
+    _say("empty population + shape checks")
+    import torch
+    from brkga_fft2_amd.native import DecoderError
+    prob, n, M, packed = syn_instance("C1", seed=2)
+    dec = _decoder(packed)
+    ms, log = dec.evaluate(np.zeros((0, 2 * n), np.float32), with_log=True)
+    assert ms.shape == (0,) and log.shape == (0, n, 6)
+    dk = torch.zeros((0, 2 * n), dtype=torch.float32, device="cuda")
+    out = torch.empty(0, dtype=torch.float64, device="cuda")
+    assert dec.evaluate_device(dk, out) is out
+    keys = syn_keys("C1", 16, n, seed=4)
+    dk = torch.from_numpy(keys).cuda()
+    with pytest.raises(ValueError):
+        dec.evaluate_device(dk, torch.empty(8, dtype=torch.float64, device="cuda"))
+    with pytest.raises(DecoderError):
+        dec.evaluate(np.zeros((4, 2 * n + 1), np.float32))
+    ref_ms, _, _ = oracle.decode(packed, keys)
+    out = torch.empty(16, dtype=torch.float64, device="cuda")
+    dec.evaluate_device(dk, out)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), ref_ms)
+    dec.close()
