@@ -35,38 +35,40 @@ __device__ __forceinline__ bf16_t f2bf(float f) {
 }
 
 #define GEMM_TILE 64
-#define GEMM_KC 32
-#define GEMM_LDS_K 40   // padded row length (bf16) of the LDS tiles
+#define GEMM_KC 64
+#define GEMM_LDS_K 72   // padded row length (bf16) of the LDS tiles
+#define GEMM_SHARES (GEMM_TILE * GEMM_KC / 8 / 256)   // 16-byte shares per thread per operand tile
 #define GEMM_SPLIT_MAX 8
 
 enum { EPI_RELU = 1, EPI_MASK = 2 };
 
-// One thread's 8-element share (16 bytes) of a 64-row x 32-k operand tile.
-// KCONT: element (r, k) sits at P[r*ld + k] (k contiguous); otherwise at
-// P[k*ld + r] (rows contiguous).  Rows >= rows and k >= kend read as zero; the
-// host guarantees whole vectors (K % 8 == 0 for KCONT, rows % 8 == 0 otherwise,
-// ld % 8 == 0), so a vector is either wholly inside or wholly outside.
+// Share q (0 <= q < 256 * GEMM_SHARES, 8 elements = 16 bytes) of a 64-row x
+// GEMM_KC-k operand tile.  KCONT: element (r, k) sits at P[r*ld + k] (k
+// contiguous); otherwise at P[k*ld + r] (rows contiguous).  Rows >= rows and
+// k >= kend read as zero; the host guarantees whole vectors (K % 8 == 0 for
+// KCONT, rows % 8 == 0 otherwise, ld % 8 == 0), so a vector is either wholly
+// inside or wholly outside.
 template <bool KCONT>
 __device__ __forceinline__ uint4 stage_load(const bf16_t *P, int ld, int rows, int kend, int row0, int k0,
-                                            int tid) {
+                                            int q) {
     uint4 v = {0u, 0u, 0u, 0u};
     if (KCONT) {
-        const int r = tid >> 2, kk = (tid & 3) << 3;
+        const int r = q / (GEMM_KC / 8), kk = (q % (GEMM_KC / 8)) << 3;
         if (row0 + r < rows && k0 + kk < kend) v = *(const uint4 *)(P + (long long)(row0 + r) * ld + k0 + kk);
     } else {
-        const int kk = tid >> 3, r = (tid & 7) << 3;
+        const int kk = q >> 3, r = (q & 7) << 3;
         if (k0 + kk < kend && row0 + r < rows) v = *(const uint4 *)(P + (long long)(k0 + kk) * ld + row0 + r);
     }
     return v;
 }
 
-// Store the share into the LDS tile S[row][k] (k contiguous, the MFMA operand order).
+// Store share q into the LDS tile S[row][k] (k contiguous, the MFMA operand order).
 template <bool KCONT>
-__device__ __forceinline__ void stage_store(bf16_t (*S)[GEMM_LDS_K], uint4 v, int tid) {
+__device__ __forceinline__ void stage_store(bf16_t (*S)[GEMM_LDS_K], uint4 v, int q) {
     if (KCONT) {
-        *(uint4 *)&S[tid >> 2][(tid & 3) << 3] = v;
+        *(uint4 *)&S[q / (GEMM_KC / 8)][(q % (GEMM_KC / 8)) << 3] = v;
     } else {
-        const int kk = tid >> 3, r = (tid & 7) << 3;
+        const int kk = q >> 3, r = (q & 7) << 3;
         const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -94,16 +96,26 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, int
     const int wm = (wave >> 1) << 5, wn = (wave & 1) << 5;
     const int kb = blockIdx.z * kper, ke = min(K, kb + kper);
     f32x16 acc = {};
-    uint4 ra = stage_load<!TA>(A, lda, M, ke, m0, kb, tid);
-    uint4 rb = stage_load<TB>(B, ldb, N, ke, n0, kb, tid);
+    uint4 ra[GEMM_SHARES], rb[GEMM_SHARES];
+#pragma unroll
+    for (int q = 0; q < GEMM_SHARES; q++) {
+        ra[q] = stage_load<!TA>(A, lda, M, ke, m0, kb, tid + 256 * q);
+        rb[q] = stage_load<TB>(B, ldb, N, ke, n0, kb, tid + 256 * q);
+    }
     for (int k0 = kb; k0 < ke; k0 += GEMM_KC) {
         __syncthreads();
-        stage_store<!TA>(As, ra, tid);
-        stage_store<TB>(Bs, rb, tid);
+#pragma unroll
+        for (int q = 0; q < GEMM_SHARES; q++) {
+            stage_store<!TA>(As, ra[q], tid + 256 * q);
+            stage_store<TB>(Bs, rb[q], tid + 256 * q);
+        }
         __syncthreads();
         if (k0 + GEMM_KC < ke) {
-            ra = stage_load<!TA>(A, lda, M, ke, m0, k0 + GEMM_KC, tid);
-            rb = stage_load<TB>(B, ldb, N, ke, n0, k0 + GEMM_KC, tid);
+#pragma unroll
+            for (int q = 0; q < GEMM_SHARES; q++) {
+                ra[q] = stage_load<!TA>(A, lda, M, ke, m0, k0 + GEMM_KC, tid + 256 * q);
+                rb[q] = stage_load<TB>(B, ldb, N, ke, n0, k0 + GEMM_KC, tid + 256 * q);
+            }
         }
 #pragma unroll
         for (int ks = 0; ks < GEMM_KC; ks += 16) {
@@ -179,41 +191,32 @@ __global__ __launch_bounds__(256) void featurize_kernel(const float *keys, int B
     (void)B;
 }
 
-// y[b] = H2[b] . w3 + b3 ; one wave per row.
+// y[b] = H2[b] . w3 + b3 ; one wave per row.  With targets t (training) also
+// dy[b] = 2 (y[b] - t[b]) / B and sq[b] = (y[b] - t[b])^2 / B.
 __global__ __launch_bounds__(256) void output_kernel(const bf16_t *H, int B, int Hd, const float *w3,
-                                                     const float *b3, float *y) {
+                                                     const float *b3, float *y, const float *t, float *dy,
+                                                     float *sq) {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (row >= B) return;
     float s = 0.0f;
     for (int j = lane; j < Hd; j += 64) s += bf2f(H[(long long)row * Hd + j]) * w3[j];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) y[row] = s + b3[0];
+    if (lane == 0) {
+        const float yr = s + b3[0];
+        y[row] = yr;
+        if (t) {
+            const float d = yr - t[row];
+            dy[row] = 2.0f * d / (float)B;
+            sq[row] = d * d / (float)B;
+        }
+    }
 }
 
-// dy = 2 (y - t) / B ; loss partial sums
-__global__ void loss_grad_kernel(const float *y, const float *t, int B, float *dy, float *loss) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
-    const float d = y[b] - t[b];
-    dy[b] = 2.0f * d / (float)B;
-    if (loss) atomicAdd(loss, d * d / (float)B);
-}
-
-// dH2[b][j] = dy[b] * w3[j] * [H2[b][j] > 0]  (bf16)
-__global__ void dh2_kernel(const float *dy, const float *w3, const bf16_t *H2, int B, int Hd, bf16_t *dH2) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)B * Hd) return;
-    const int b = (int)(t / Hd), j = (int)(t % Hd);
-    dH2[t] = f2bf(bf2f(H2[t]) > 0.0f ? dy[b] * w3[j] : 0.0f);
-}
-
-// g[j] (+)= sum_b M[b][j] * (w ? w[b] : 1)
-// Column sums g[j] = sum_b Mx[b][j] * (w ? w[b] : 1) in two fixed-order stages:
-// partial sums over 64-row blocks (one wave per 16 rows, 64 columns per
-// workgroup), then the row blocks in order.
+// Column sums part[rb][j] = sum of Mx[b][j] over the 64-row block rb (one wave per
+// 16 rows, 64 columns per workgroup); head_final_kernel adds the blocks in order.
 #define COLSUM_ROWS 64
-__global__ __launch_bounds__(256) void colsum_partial_kernel(const bf16_t *Mx, int B, int Hd, const float *w,
-                                                             float *part) {
+static_assert(COLSUM_ROWS == 64, "head_kernel reduces a block's rows with one 64-lane butterfly");
+__global__ __launch_bounds__(256) void colsum_partial_kernel(const bf16_t *Mx, int B, int Hd, float *part) {
     __shared__ float sh[4][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int j = blockIdx.x * 64 + lane, rb = blockIdx.y;
@@ -222,7 +225,7 @@ __global__ __launch_bounds__(256) void colsum_partial_kernel(const bf16_t *Mx, i
 #pragma unroll
         for (int i = 0; i < COLSUM_ROWS / 4; i++) {
             const int b = rb * COLSUM_ROWS + wv * (COLSUM_ROWS / 4) + i;
-            if (b < B) s += bf2f(Mx[(long long)b * Hd + j]) * (w ? w[b] : 1.0f);
+            if (b < B) s += bf2f(Mx[(long long)b * Hd + j]);
         }
     }
     sh[wv][lane] = s;
@@ -230,25 +233,94 @@ __global__ __launch_bounds__(256) void colsum_partial_kernel(const bf16_t *Mx, i
     if (wv == 0 && j < Hd) part[(long long)rb * Hd + j] = (sh[0][lane] + sh[1][lane]) + (sh[2][lane] + sh[3][lane]);
 }
 
-__global__ void colsum_final_kernel(const float *part, int R, int Hd, float *g) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= Hd) return;
-    float s = 0.0f;
-    for (int r = 0; r < R; r++) s += part[(long long)r * Hd + j];
-    g[j] = s;
+// Backward through the output layer for one 64-row block and 64 columns
+// (grid R x ceil(Hd / 64), one wave per 16 rows):
+//   dH2[b][j] = dy[b] w3[j] [H2[b][j] > 0]   (bf16)
+//   dw3 partial[j] = sum_b H2[b][j] dy[b],   db2 partial[j] = sum_b dH2[b][j]
+// into part3 / part2 [block][Hd]; the first column slice also writes
+// scal[block] = (sum_b sq[b], sum_b dy[b]) -- the loss and db3 partials (COLSUM_ROWS
+// is one wave's width, so each is one butterfly).
+// head_final_kernel adds the blocks in order, so every sum has a fixed order.
+__global__ __launch_bounds__(256) void head_kernel(const bf16_t *H2, int B, int Hd, const float *w3,
+                                                   const float *dy, const float *sq, bf16_t *dH2,
+                                                   float *part3, float *part2, float *scal) {
+    __shared__ float sdy[COLSUM_ROWS];
+    __shared__ float s3[4][64], s2[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, blk = blockIdx.x;
+    const int r0 = blk * COLSUM_ROWS;
+    const int nr = min(COLSUM_ROWS, B - r0);
+    if (threadIdx.x < COLSUM_ROWS) {
+        const bool in = (int)threadIdx.x < nr;
+        float g = in ? dy[r0 + threadIdx.x] : 0.0f;
+        sdy[threadIdx.x] = g;
+        if (blockIdx.y == 0) {   // loss and db3 partials: one butterfly over the block's rows
+            float l = in ? sq[r0 + threadIdx.x] : 0.0f;
+            for (int off = 32; off > 0; off >>= 1) {
+                l += __shfl_xor(l, off, 64);
+                g += __shfl_xor(g, off, 64);
+            }
+            if (threadIdx.x == 0) {
+                scal[2 * blk] = l;
+                scal[2 * blk + 1] = g;
+            }
+        }
+    }
+    __syncthreads();
+    const int j = blockIdx.y * 64 + lane;
+    float a3 = 0.0f, a2 = 0.0f;
+    if (j < Hd) {
+        const float wj = w3[j];
+        float h[COLSUM_ROWS / 4];
+#pragma unroll
+        for (int i = 0; i < COLSUM_ROWS / 4; i++) {
+            const int r = wv * (COLSUM_ROWS / 4) + i;
+            h[i] = r < nr ? bf2f(H2[(long long)(r0 + r) * Hd + j]) : 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < COLSUM_ROWS / 4; i++) {
+            const int r = wv * (COLSUM_ROWS / 4) + i;
+            if (r < nr) {
+                const bf16_t gh = f2bf(h[i] > 0.0f ? sdy[r] * wj : 0.0f);
+                dH2[(long long)(r0 + r) * Hd + j] = gh;
+                a3 += h[i] * sdy[r];
+                a2 += bf2f(gh);
+            }
+        }
+    }
+    s3[wv][lane] = a3;
+    s2[wv][lane] = a2;
+    __syncthreads();
+    if (wv == 0 && j < Hd) {
+        part3[(long long)blk * Hd + j] = (s3[0][lane] + s3[1][lane]) + (s3[2][lane] + s3[3][lane]);
+        part2[(long long)blk * Hd + j] = (s2[0][lane] + s2[1][lane]) + (s2[2][lane] + s2[3][lane]);
+    }
 }
 
-__global__ void sum_kernel(const float *v, int B, float *out) {
-    __shared__ float sh[256];
-    float s = 0.0f;
-    for (int b = threadIdx.x; b < B; b += 256) s += v[b];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
-        __syncthreads();
+// Sums the R row blocks in order: db1 (part1), db2 (part2), dw3 (part3), and
+// thread 0 the loss and db3 (scal).
+__global__ void head_final_kernel(const float *part1, const float *part2, const float *part3,
+                                  const float *scal, int R, int Hd, float *db1, float *db2, float *dw3,
+                                  float *db3, float *loss) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < Hd) {
+        float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+#pragma unroll 8
+        for (int r = 0; r < R; r++) {
+            s1 += part1[(long long)r * Hd + j];
+            s2 += part2[(long long)r * Hd + j];
+            s3 += part3[(long long)r * Hd + j];
+        }
+        db1[j] = s1;
+        db2[j] = s2;
+        dw3[j] = s3;
     }
-    if (threadIdx.x == 0) out[0] = sh[0];
+    if (j == 0) {
+        float l = 0.0f, d = 0.0f;
+#pragma unroll 8
+        for (int r = 0; r < R; r++) { l += scal[2 * r]; d += scal[2 * r + 1]; }
+        loss[0] = l;
+        db3[0] = d;
+    }
 }
 
 __global__ void adam_kernel(float *p, const float *g, float *m, float *v, long long n, float lr, float b1,
@@ -284,8 +356,10 @@ struct brkga_mlp {
     // activations (grown on demand)
     long long cap = 0;
     bf16_t *x = nullptr, *H1 = nullptr, *H2 = nullptr, *dH2 = nullptr, *dH1 = nullptr;
-    float *y = nullptr, *dy = nullptr, *loss = nullptr;
-    float *part = nullptr;   // column-sum partials [cap / 64][Hd]
+    float *y = nullptr, *dy = nullptr, *sq = nullptr, *loss = nullptr;
+    float *part = nullptr;   // column-sum partials [cap / 64][Hd]: db1
+    float *part2 = nullptr, *part3 = nullptr;   // the same for db2, dw3
+    float *scal = nullptr;   // [cap / 64][2]: loss and db3 partials
     float *ws = nullptr;     // split-k partials [GEMM_SPLIT_MAX][max(Fp, Hd)][Hd]
     long long step = 0;
     float lr = 1e-3f, wd = 0.0f;
@@ -313,7 +387,8 @@ static int mlp_alloc(brkga_mlp *net, size_t bytes, void **q) {
 static int mlp_reserve(brkga_mlp *net, long long B) {
     if (B <= net->cap) return 0;
     for (void *q : {(void *)net->x, (void *)net->H1, (void *)net->H2, (void *)net->dH2, (void *)net->dH1,
-                    (void *)net->y, (void *)net->dy, (void *)net->part}) {
+                    (void *)net->y, (void *)net->dy, (void *)net->sq, (void *)net->part, (void *)net->part2,
+                    (void *)net->part3, (void *)net->scal}) {
         if (!q) continue;
         auto it = std::find(net->allocs.begin(), net->allocs.end(), q);
         if (it != net->allocs.end()) { (void)hipFree(q); net->allocs.erase(it); }
@@ -327,8 +402,12 @@ static int mlp_reserve(brkga_mlp *net, long long B) {
     if ((rc = mlp_alloc(net, (size_t)B * net->Hd * 2, &q))) return rc; net->dH1 = (bf16_t *)q;
     if ((rc = mlp_alloc(net, (size_t)B * 4, &q))) return rc; net->y = (float *)q;
     if ((rc = mlp_alloc(net, (size_t)B * 4, &q))) return rc; net->dy = (float *)q;
-    if ((rc = mlp_alloc(net, (size_t)((B + COLSUM_ROWS - 1) / COLSUM_ROWS) * net->Hd * 4, &q))) return rc;
-    net->part = (float *)q;
+    if ((rc = mlp_alloc(net, (size_t)B * 4, &q))) return rc; net->sq = (float *)q;
+    const size_t R = (size_t)((B + COLSUM_ROWS - 1) / COLSUM_ROWS);
+    if ((rc = mlp_alloc(net, R * net->Hd * 4, &q))) return rc; net->part = (float *)q;
+    if ((rc = mlp_alloc(net, R * net->Hd * 4, &q))) return rc; net->part2 = (float *)q;
+    if ((rc = mlp_alloc(net, R * net->Hd * 4, &q))) return rc; net->part3 = (float *)q;
+    if ((rc = mlp_alloc(net, R * 2 * 4, &q))) return rc; net->scal = (float *)q;
     net->cap = B;
     return 0;
 }
@@ -366,13 +445,9 @@ static int gemm(hipStream_t s, int M, int N, int K, const bf16_t *A, int lda, co
     return 0;
 }
 
-static void colsum(brkga_mlp *net, hipStream_t s, const bf16_t *Mx, int B, const float *w, float *g) {
-    const int Hd = net->Hd, R = (B + COLSUM_ROWS - 1) / COLSUM_ROWS;
-    hipLaunchKernelGGL(colsum_partial_kernel, dim3((Hd + 63) / 64, R), dim3(256), 0, s, Mx, B, Hd, w, net->part);
-    hipLaunchKernelGGL(colsum_final_kernel, dim3((Hd + 255) / 256), dim3(256), 0, s, net->part, R, Hd, g);
-}
-
-static int forward(brkga_mlp *net, long long B, hipStream_t s) {
+// Forward pass from net->x: H1, H2 and y; with targets t also the loss
+// gradient dy and the squared errors.
+static int forward(brkga_mlp *net, long long B, hipStream_t s, const float *t) {
     const int Hd = net->Hd;
     int rc;
     if ((rc = gemm<false, false>(s, (int)B, Hd, net->Fp, net->x, net->Fp, net->pb + net->oW1, Hd,
@@ -381,21 +456,18 @@ static int forward(brkga_mlp *net, long long B, hipStream_t s) {
                                  nullptr, 0, EPI_RELU, nullptr, net->H2, Hd)))
         return rc;
     hipLaunchKernelGGL(output_kernel, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, s, net->H2, (int)B, Hd,
-                       net->p + net->ow3, net->p + net->ob3, net->y);
+                       net->p + net->ow3, net->p + net->ob3, net->y, t, net->dy, net->sq);
     return 0;
 }
 
-// gradients of the mean squared error into net->g (the inputs are already in net->x)
-static int backward(brkga_mlp *net, const float *t, long long B, hipStream_t s) {
-    const int Hd = net->Hd;
+// Gradients of the mean squared error into net->g and the loss into net->loss
+// (after forward with the targets; the inputs are already in net->x).
+static int backward(brkga_mlp *net, long long B, hipStream_t s) {
+    const int Hd = net->Hd, R = (int)((B + COLSUM_ROWS - 1) / COLSUM_ROWS);
     int rc;
-    hipLaunchKernelGGL(loss_grad_kernel, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, s, net->y, t, (int)B,
-                       net->dy, net->loss);
-    colsum(net, s, net->H2, (int)B, net->dy, net->g + net->ow3);                  // dw3 = H2^T dy
-    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, s, net->dy, (int)B, net->g + net->ob3);
-    hipLaunchKernelGGL(dh2_kernel, dim3((unsigned)((B * Hd + 255) / 256)), dim3(256), 0, s, net->dy,
-                       net->p + net->ow3, net->H2, (int)B, Hd, net->dH2);
-    colsum(net, s, net->dH2, (int)B, nullptr, net->g + net->ob2);                 // db2
+    // dH2 and the partial sums of dw3, db2, db3 and the loss
+    hipLaunchKernelGGL(head_kernel, dim3((unsigned)R, (unsigned)((Hd + 63) / 64)), dim3(256), 0, s, net->H2, (int)B,
+                       Hd, net->p + net->ow3, net->dy, net->sq, net->dH2, net->part3, net->part2, net->scal);
     // dW2 = H1^T dH2
     if ((rc = gemm<true, false>(s, Hd, Hd, (int)B, net->H1, Hd, net->dH2, Hd, nullptr, nullptr, 0, 0,
                                 net->g + net->oW2, nullptr, Hd, net->ws)))
@@ -404,7 +476,12 @@ static int backward(brkga_mlp *net, const float *t, long long B, hipStream_t s) 
     if ((rc = gemm<false, true>(s, (int)B, Hd, Hd, net->dH2, Hd, net->pb + net->oW2, Hd, nullptr, net->H1, Hd,
                                 EPI_MASK, nullptr, net->dH1, Hd)))
         return rc;
-    colsum(net, s, net->dH1, (int)B, nullptr, net->g + net->ob1);                 // db1
+    hipLaunchKernelGGL(colsum_partial_kernel, dim3((Hd + 63) / 64, R), dim3(256), 0, s, net->dH1, (int)B, Hd,
+                       net->part);
+    // db1, db2, dw3, db3 and the loss, each summed over the row blocks in order
+    hipLaunchKernelGGL(head_final_kernel, dim3((Hd + 255) / 256), dim3(256), 0, s, net->part, net->part2,
+                       net->part3, net->scal, R, Hd, net->g + net->ob1, net->g + net->ob2, net->g + net->ow3,
+                       net->g + net->ob3, net->loss);
     // dW1 = X^T dH1
     return gemm<true, false>(s, net->Fp, Hd, (int)B, net->x, net->Fp, net->dH1, Hd, nullptr, nullptr, 0, 0,
                              net->g + net->oW1, nullptr, Hd, net->ws);
@@ -497,7 +574,7 @@ extern "C" int brkga_mlp_predict(brkga_mlp *net, const float *d_keys, int64_t B,
     MLPCHK(hipSetDevice(net->device));
     hipStream_t s = (hipStream_t)stream;
     int rc = mlp_features(net, d_keys, B, s);
-    if (rc || (rc = forward(net, B, s))) return rc;
+    if (rc || (rc = forward(net, B, s, nullptr))) return rc;
     MLPCHK(hipMemcpyAsync(d_pred, net->y, (size_t)B * 4, hipMemcpyDeviceToDevice, s));
     MLPCHK(hipGetLastError());
     return 0;
@@ -509,9 +586,8 @@ extern "C" int brkga_mlp_gradients(brkga_mlp *net, const float *d_keys, const fl
     MLPCHK(hipSetDevice(net->device));
     hipStream_t s = (hipStream_t)stream;
     int rc = mlp_features(net, d_keys, B, s);
-    if (rc || (rc = forward(net, B, s))) return rc;
-    MLPCHK(hipMemsetAsync(net->loss, 0, 4, s));
-    if ((rc = backward(net, d_target, B, s))) return rc;
+    if (rc || (rc = forward(net, B, s, d_target))) return rc;
+    if ((rc = backward(net, B, s))) return rc;
     if (d_grad) MLPCHK(hipMemcpyAsync(d_grad, net->g, (size_t)net->np_ * 4, hipMemcpyDeviceToDevice, s));
     MLPCHK(hipGetLastError());
     return 0;

@@ -165,13 +165,22 @@ class MLPSurrogate:
             pass
 
 
+def _average_ranks(v):
+    """0-based ranks, ties given their average rank (float64)."""
+    srt = np.sort(v)
+    return (np.searchsorted(srt, v, "left") + np.searchsorted(srt, v, "right") - 1) * 0.5
+
+
 def spearman(a, b):
-    """Spearman rank correlation (average ranks on ties)."""
-    from scipy.stats import spearmanr
+    """Spearman rank correlation (average ranks on ties); NaN for fewer than 3
+    values or a constant input."""
     a, b = np.asarray(a), np.asarray(b)
     if a.size < 3 or np.all(a == a[0]) or np.all(b == b[0]):
         return float("nan")
-    return float(spearmanr(a, b).correlation)
+    ra, rb = _average_ranks(a), _average_ranks(b)
+    ra -= ra.mean()
+    rb -= rb.mean()
+    return float(ra @ rb / np.sqrt((ra @ ra) * (rb @ rb)))
 
 
 class SurrogateBRKGA:
@@ -260,12 +269,13 @@ class SurrogateBRKGA:
             tt = ((y - y.mean()) / y.std().clamp_min(1e-12)).float()
         b = min(self.batch, n)
         idx_all = torch.from_numpy(self.rng.integers(0, n, size=(self.train_steps, b))).to(self.dev)
+        # every step's batch gathered at once; step k trains on rows k*b .. k*b+b-1
+        xs = self.buf_x[idx_all.reshape(-1)]
+        ts = tt[idx_all.reshape(-1)]
         loss = float("nan")
         for k in range(self.train_steps):
-            idx = idx_all[k]
-            x = self.buf_x[idx].contiguous()
-            t = tt[idx].contiguous()
-            loss = self.net.train_step(x, t, stream=self.stream, want_loss=(k == self.train_steps - 1))
+            loss = self.net.train_step(xs[k * b:(k + 1) * b], ts[k * b:(k + 1) * b], stream=self.stream,
+                                       want_loss=(k == self.train_steps - 1))
         return loss
 
     def step(self):
@@ -300,9 +310,9 @@ class SurrogateBRKGA:
         self.fit = torch.cat([self.fit[elite], fit_c])
         rho_s = float("nan")
         if chosen_pred is not None:
-            f = fit_c.cpu().numpy()
-            ok = f < 1e15
-            rho_s = spearman(chosen_pred.cpu().numpy()[ok], f[ok])
+            pf = torch.stack([chosen_pred.double(), fit_c]).cpu().numpy()   # one copy to the host
+            ok = pf[1] < 1e15
+            rho_s = spearman(pf[0][ok], pf[1][ok])
         if self.use_surrogate:
             self.log["spearman"].append(rho_s)
             self._add_training(chosen, fit_c)

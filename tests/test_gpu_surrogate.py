@@ -74,6 +74,10 @@ def test_forward_and_gradients_match_oracle(B):
         if sz > 1:
             cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300)
             assert cos >= 0.999, name
+    # the loss a training step reports is the mean squared error before its update
+    loss = net.train_step(dk, torch.from_numpy(t).cuda())
+    loss_ref = float(np.mean((y_ref - t.astype(np.float64)) ** 2))
+    assert abs(loss - loss_ref) <= 2e-2 * max(1.0, loss_ref)
     net.close()
 
 
