@@ -50,7 +50,7 @@ def test_host_loop_matches_oracle_driven_loop():
     assert np.all(np.diff(mins) <= 0)
 
 
-@pytest.mark.parametrize("cfg", ["C2"])
+@pytest.mark.parametrize("cfg", ["C1", "C2", "C3"])   # one, two and four machines
 def test_device_loop_graph_equals_eager_and_oracle(cfg):
     _say("device GA %s" % cfg)
     from brkga_fft2_amd.native import Decoder, DeviceGA
