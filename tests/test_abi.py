@@ -66,3 +66,22 @@ def test_create_null_arguments():
     h = ctypes.c_void_p()
     assert lib.brkga_create(None, None, ctypes.byref(h)) == -1
     assert "NULL" in lib.brkga_last_error(None).decode()
+
+
+@pytest.mark.parametrize("compiler,flags", [("gcc", ["-std=c99", "-x", "c"]), ("g++", ["-std=c++11", "-x", "c++"])])
+def test_header_compiles_as_plain_c_and_cpp(tmp_path, compiler, flags):
+    """The boundary header is plain C (what a cgo / JNI / ctypes binding includes)
+    and also valid C++; every declared entry point is callable by name."""
+    import shutil
+    import subprocess
+    if shutil.which(compiler) is None:
+        pytest.skip("%s not installed" % compiler)
+    names = sorted(_declared_functions())
+    src = tmp_path / "use_header.c"
+    src.write_text('#include "brkga_mi355x.h"\n'
+                   "typedef void (*entry_fn)(void);\n"
+                   "entry_fn entry_points[] = {\n" + "".join("    (entry_fn)&%s,\n" % n for n in names) + "};\n")
+    cmd = [compiler] + flags + ["-Wall", "-Wextra", "-Werror", "-pedantic", "-I", os.path.join(ROOT, "include"),
+                                "-c", str(src), "-o", str(tmp_path / "use_header.o")]
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert res.returncode == 0, res.stdout
