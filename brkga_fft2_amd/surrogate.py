@@ -240,10 +240,18 @@ class SurrogateBRKGA:
         self.dec.evaluate_device(keys, out, stream=self.stream, check_overflow=False)
         self.decoder_calls += keys.shape[0]
 
-    def _add_training(self, keys, fit):
+    def _add_training(self, keys, fit, feasible=None):
+        """Append the feasible (keys, fitness) pairs to the ring buffer.  With
+        ``feasible`` (a host mask already at hand) no device-to-host read is needed."""
         torch = self.torch
-        ok = fit < 1e15
-        k, f = keys[ok], fit[ok]
+        if feasible is None:
+            ok = fit < 1e15
+            k, f = keys[ok], fit[ok]
+        elif feasible.all():
+            k, f = keys, fit
+        else:
+            sel = torch.from_numpy(np.nonzero(feasible)[0]).to(self.dev)
+            k, f = keys.index_select(0, sel), fit.index_select(0, sel)
         m = k.shape[0]
         if m == 0:
             return
@@ -309,13 +317,14 @@ class SurrogateBRKGA:
         self.pop = torch.cat([self.pop[elite], chosen])
         self.fit = torch.cat([self.fit[elite], fit_c])
         rho_s = float("nan")
+        ok = None
         if chosen_pred is not None:
             pf = torch.stack([chosen_pred.double(), fit_c]).cpu().numpy()   # one copy to the host
             ok = pf[1] < 1e15
             rho_s = spearman(pf[0][ok], pf[1][ok])
         if self.use_surrogate:
             self.log["spearman"].append(rho_s)
-            self._add_training(chosen, fit_c)
+            self._add_training(chosen, fit_c, ok)
             self.log["loss"].append(self._train())
         self.log["screened"].append(bool(screen))
         self.gen += 1
