@@ -709,19 +709,21 @@ __global__ __launch_bounds__(ORDER_BUCKETS) void order_scatter_kernel(const int 
     }
 }
 
-// Vacancy filter of one rotation, run by one wave: bit l of word k says top
-// row y = 64k + l passes, i.e. every grid row y+a (a < h) has a free run >=
-// the mask row's longest 1-run (numba_utils.py:131-157 applies the same test
-// to decide whether the bin is tried).  For mask row a, lane l tests grid row
-// 64k + a + l, so the ballot is already aligned with the candidate word.
-// Four mask rows per step keep 4 x NK reads in flight; NK = ceil((H - h + 1) /
-// 64) words, all indexed by constants so that they stay in scalar registers
-// and the loop exit stays uniform.  Lanes whose top row is past H - h read
-// rows up to H + 62 (inside the LDS allocation: the layout pads the vacancy
-// arrays) and are cleared by the initial mask.  The passing top rows go to
-// cand in descending order; returns their count.
+// Vacancy filter of one rotation over top rows [y0, y0 + 64 NK), run by one
+// wave: bit l of word k says top row y = y0 + 64k + l passes, i.e. every grid
+// row y+a (a < h) has a free run >= the mask row's longest 1-run
+// (numba_utils.py:131-157 applies the same test to decide whether the bin is
+// tried).  For mask row a, lane l tests grid row y0 + 64k + a + l, so the
+// ballot is already aligned with the candidate word.  Four mask rows per step
+// keep 4 x NK reads in flight; the NK <= 8 words are all indexed by constants
+// so that they stay in scalar registers and the loop exit stays uniform.
+// ny: valid top rows from y0 on (H - h + 1 - y0).  Lanes whose top row is
+// past H - h read rows up to H + 62 (inside the LDS allocation: the layout
+// pads the vacancy arrays) and are cleared by the initial mask.  The passing
+// top rows go to cand in descending order; returns their count.
 template <int NK>
-__device__ __forceinline__ int vacancy_rotation(lds_cs16 *vrow, lds_ci32 *den, int h, int ny, short *cand) {
+__device__ __forceinline__ int vacancy_rotation(lds_cs16 *vrow, lds_ci32 *den, int h, int ny, short *cand,
+                                                int y0) {
     const int lane = threadIdx.x & 63;
     unsigned long long cw[NK];
 #pragma unroll
@@ -729,7 +731,7 @@ __device__ __forceinline__ int vacancy_rotation(lds_cs16 *vrow, lds_ci32 *den, i
         const int left = ny - (k << 6);   // >= 1 for k < NK
         cw[k] = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
     }
-    lds_cs16 *vl = vrow + lane;
+    lds_cs16 *vl = vrow + y0 + lane;
     for (int a = 0; a < h; a += 4) {
         int aj[4], d[4];
 #pragma unroll
@@ -755,7 +757,7 @@ __device__ __forceinline__ int vacancy_rotation(lds_cs16 *vrow, lds_ci32 *den, i
         if (cw[k]) {
             const unsigned long long rv = __builtin_bitreverse64(cw[k]);
             const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(rv >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)rv, 0));
-            if (__builtin_amdgcn_inverse_ballot_w64(rv)) cand[nc_ + pos] = (short)((k << 6) + 63 - lane);
+            if (__builtin_amdgcn_inverse_ballot_w64(rv)) cand[nc_ + pos] = (short)(y0 + (k << 6) + 63 - lane);
             nc_ += __popcll(cw[k]);
         }
     }
@@ -795,18 +797,24 @@ __device__ __forceinline__ void vacancy_filter(const DevProblem &P, int rb, int 
         int nc_ = 0;
         if (h <= H && w <= W) {
             const int ny = H - h + 1;
-            const int nk = (ny + 63) >> 6;   // <= 8 for plates <= 512 rows
+            const int nk = (ny + 63) >> 6;
             lds_cs16 *vr = (lds_cs16 *)vrow;
             lds_ci32 *dn = (lds_ci32 *)den;
-            switch (nk) {
-                case 1: nc_ = vacancy_rotation<1>(vr, dn, h, ny, cand); break;
-                case 2: nc_ = vacancy_rotation<2>(vr, dn, h, ny, cand); break;
-                case 3: nc_ = vacancy_rotation<3>(vr, dn, h, ny, cand); break;
-                case 4: nc_ = vacancy_rotation<4>(vr, dn, h, ny, cand); break;
-                case 5: nc_ = vacancy_rotation<5>(vr, dn, h, ny, cand); break;
-                case 6: nc_ = vacancy_rotation<6>(vr, dn, h, ny, cand); break;
-                case 7: nc_ = vacancy_rotation<7>(vr, dn, h, ny, cand); break;
-                default: nc_ = vacancy_rotation<8>(vr, dn, h, ny, cand); break;
+            // chunks of at most 8 words (512 top rows), the highest chunk first
+            // so that the list stays descending; one chunk when H - h < 512
+            for (int c = (nk - 1) >> 3; c >= 0; c--) {
+                const int y0 = c << 9, nkc = min(8, nk - (c << 3));
+                short *cc = cand + nc_;
+                switch (nkc) {
+                    case 1: nc_ += vacancy_rotation<1>(vr, dn, h, ny - y0, cc, y0); break;
+                    case 2: nc_ += vacancy_rotation<2>(vr, dn, h, ny - y0, cc, y0); break;
+                    case 3: nc_ += vacancy_rotation<3>(vr, dn, h, ny - y0, cc, y0); break;
+                    case 4: nc_ += vacancy_rotation<4>(vr, dn, h, ny - y0, cc, y0); break;
+                    case 5: nc_ += vacancy_rotation<5>(vr, dn, h, ny - y0, cc, y0); break;
+                    case 6: nc_ += vacancy_rotation<6>(vr, dn, h, ny - y0, cc, y0); break;
+                    case 7: nc_ += vacancy_rotation<7>(vr, dn, h, ny - y0, cc, y0); break;
+                    default: nc_ += vacancy_rotation<8>(vr, dn, h, ny - y0, cc, y0); break;
+                }
             }
             if (DENSE && nc_ > 0) {   // full valid region, bottom-up
                 nc_ = H - h + 1;
@@ -1280,6 +1288,10 @@ struct brkga_handle {
     double kernel_ms = 0.0, last_ms = 0.0;
     size_t smem = 0;
     int max_bins_req = 0;
+    // Bumped whenever a device buffer that a launch reads or writes moves
+    // (workspace growth, larger per-call buffers): a captured graph holds the
+    // old addresses and must be captured again.
+    unsigned ws_gen = 0;
     double wall_hz = 1e8;
     std::string err;
 };
@@ -1379,6 +1391,7 @@ static int device_workspace(brkga_handle *h, int max_bins) {
         }
     }
     w.max_bins = max_bins;
+    h->ws_gen++;
     void *p;
     int rc;
     if ((rc = alloc_dev(h, nwg * max_bins * (size_t)w.hmax * w.pitch, &p))) return rc;
@@ -1618,6 +1631,7 @@ int brkga_internal_reserve(brkga_handle *h, int64_t n_chrom) {
         HIPCHK(h, hipMalloc(&h->d_order, (size_t)n_chrom * M * sizeof(int)));
         HIPCHK(h, hipMalloc(&h->d_bucket, (size_t)n_chrom * M * sizeof(int)));
         h->d_ctx_n = (size_t)n_chrom * M;
+        h->ws_gen++;
     }
     if (!h->d_hist) {
         HIPCHK(h, hipMalloc(&h->d_hist, ORDER_BUCKETS * sizeof(int)));
@@ -1659,6 +1673,11 @@ int brkga_internal_launch(brkga_handle *h, const void *d_keys, int key_dtype, in
 }
 
 int brkga_internal_n_parts(const brkga_handle *h) { return h->prob.n_parts; }
+unsigned brkga_internal_ws_gen(const brkga_handle *h) { return h->ws_gen; }
+int brkga_internal_overflows(brkga_handle *h, unsigned long long *out) {
+    HIPCHK(h, hipMemcpy(out, h->dw.counters + ST_OVERFLOW, sizeof *out, hipMemcpyDeviceToHost));
+    return 0;
+}
 int brkga_internal_device(const brkga_handle *h) { return h->device; }
 void brkga_internal_error(brkga_handle *h, const char *msg) { h->err = msg; }
 

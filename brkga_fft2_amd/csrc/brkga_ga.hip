@@ -60,7 +60,10 @@ __global__ __launch_bounds__(GA_SORT_THREADS) void ga_sort_kernel(const double *
     int P = 1;
     while (P < N) P <<= 1;
     for (int i = threadIdx.x; i < P; i += blockDim.x) {
-        key[i] = i < N ? fit[i] : INFINITY;
+        // NaN (a context that overflowed its bins) sorts as +inf, so the
+        // comparisons below stay a total order
+        const double f = i < N ? fit[i] : INFINITY;
+        key[i] = f != f ? INFINITY : f;
         idx[i] = i < N ? i : 0x7fffffff;
     }
     __syncthreads();
@@ -176,6 +179,7 @@ struct brkga_ga {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
+    unsigned graph_ws_gen = 0;   // decoder workspace generation the graph was captured with
     std::vector<void *> allocs;
 };
 
@@ -279,6 +283,8 @@ extern "C" int brkga_ga_create(brkga_handle *dec, const brkga_ga_params *p, cons
         }
         if (hipMemcpy(ga->pop, h.data(), nk * 4, hipMemcpyHostToDevice) != hipSuccess) { g_ga_error = "hipMemcpy failed"; return bail(BRKGA_EDEVICE); }
     }
+    unsigned long long ov0 = 0, ov1 = 0;
+    if ((rc = brkga_internal_overflows(dec, &ov0))) { g_ga_error = brkga_last_error(dec); return bail(rc); }
     hipLaunchKernelGGL(ga_init_kernel, dim3(1), dim3(1), 0, 0, ga->gen, ga->best);
     if ((rc = brkga_internal_launch(dec, ga->pop, BRKGA_KEYS_F32, ga->N, ga->fit, nullptr, 0, false))) {
         g_ga_error = brkga_last_error(dec);
@@ -287,6 +293,11 @@ extern "C" int brkga_ga_create(brkga_handle *dec, const brkga_ga_params *p, cons
     hipLaunchKernelGGL(ga_best_kernel, dim3(1), dim3(256), 0, 0, ga->fit, ga->pop, ga->N, ga->G2, ga->gen,
                        ga->best, ga->best_keys, ga->hist, ga->hist_cap);
     if (hipDeviceSynchronize() != hipSuccess) { g_ga_error = "initial population decode failed"; return bail(BRKGA_EDEVICE); }
+    if ((rc = brkga_internal_overflows(dec, &ov1))) { g_ga_error = brkga_last_error(dec); return bail(rc); }
+    if (ov1 != ov0) {
+        g_ga_error = "a context of the initial population needed more bins than the decoder's max_bins";
+        return bail(BRKGA_EOVERFLOW);
+    }
     *out = ga;
     return BRKGA_OK;
 }
@@ -295,9 +306,14 @@ extern "C" int brkga_ga_run(brkga_ga *ga, int32_t generations, int use_graph, vo
     if (!ga || generations < 0) { g_ga_error = "bad argument"; return BRKGA_EINVAL; }
     GACHK(hipSetDevice(brkga_internal_device(ga->dec)));
     hipStream_t s = (hipStream_t)stream;
+    unsigned long long ov0 = 0;
+    GACHK(hipStreamSynchronize(s));
+    if (int rc = brkga_internal_overflows(ga->dec, &ov0)) { g_ga_error = brkga_last_error(ga->dec); return rc; }
     if (use_graph) {
         if (!s) { g_ga_error = "graph replay needs a non-null stream"; return BRKGA_EINVAL; }
-        if (!ga->exec || ga->graph_stream != s) {
+        // capture again when the stream changed or the decoder's buffers moved
+        // (a larger population or a grown bin workspace reallocates them)
+        if (!ga->exec || ga->graph_stream != s || ga->graph_ws_gen != brkga_internal_ws_gen(ga->dec)) {
             if (ga->exec) { (void)hipGraphExecDestroy(ga->exec); ga->exec = nullptr; }
             if (ga->graph) { (void)hipGraphDestroy(ga->graph); ga->graph = nullptr; }
             GACHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -309,6 +325,7 @@ extern "C" int brkga_ga_run(brkga_ga *ga, int32_t generations, int use_graph, vo
             ga->graph = g;
             GACHK(hipGraphInstantiate(&ga->exec, ga->graph, nullptr, nullptr, 0));
             ga->graph_stream = s;
+            ga->graph_ws_gen = brkga_internal_ws_gen(ga->dec);
         }
         for (int i = 0; i < generations; i++) GACHK(hipGraphLaunch(ga->exec, s));
     } else {
@@ -319,6 +336,14 @@ extern "C" int brkga_ga_run(brkga_ga *ga, int32_t generations, int use_graph, vo
     }
     GACHK(hipStreamSynchronize(s));
     ga->generations_done += generations;
+    // a context that ran out of bins decoded to NaN: the run's fitness is not usable
+    unsigned long long ov1 = 0;
+    if (int rc = brkga_internal_overflows(ga->dec, &ov1)) { g_ga_error = brkga_last_error(ga->dec); return rc; }
+    if (ov1 != ov0) {
+        g_ga_error = "a context needed more bins than the decoder's max_bins (create the decoder with "
+                     "max_bins = n_parts for the device loop)";
+        return BRKGA_EOVERFLOW;
+    }
     return 0;
 }
 

@@ -17,6 +17,11 @@ int brkga_internal_launch(brkga_handle *h, const void *d_keys, int key_dtype, in
 // captured launch allocates nothing).
 int brkga_internal_reserve(brkga_handle *h, int64_t n_chrom);
 int brkga_internal_n_parts(const brkga_handle *h);
+// Changes whenever a device buffer used by the decode launch is reallocated
+// (a graph captured before must be captured again).
+unsigned brkga_internal_ws_gen(const brkga_handle *h);
+// Contexts that have overflowed their bin workspace so far (synchronous read).
+int brkga_internal_overflows(brkga_handle *h, unsigned long long *out);
 int brkga_internal_device(const brkga_handle *h);
 void brkga_internal_error(brkga_handle *h, const char *msg);
 

@@ -164,6 +164,9 @@ class Decoder:
             raise DecoderError(rc, "brkga_create: " + self._lib.brkga_last_error(None).decode())
         self._h = h
         self.device = int(device)
+        # bins per context in the workspace (brkga_evaluate grows it on demand;
+        # device-side loops cannot, and need max_bins = n_parts to be safe)
+        self.max_bins = int(max_bins) if max_bins > 0 else min(self.n_parts, 32)
 
     def _check(self, rc, what):
         if rc != 0:

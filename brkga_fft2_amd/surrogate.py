@@ -232,13 +232,23 @@ class SurrogateBRKGA:
         self.log = {"min": [], "best": [], "time": [], "spearman": [], "screened": [],
                     "loss": [], "regret": [], "audit_spearman": [], "recall": []}
         self.decoder_calls = 0
+        # device decodes cannot grow the bin workspace: a context that runs out
+        # of bins decodes to NaN, so every run checks the overflow counter
+        self._overflows = decoder.stats()["overflows"]
         self._decode(self.pop, self.fit)
+        self._check_overflow()
         self._add_training(self.pop, self.fit)
         self.best = float(self.fit.min())
 
     def _decode(self, keys, out):
         self.dec.evaluate_device(keys, out, stream=self.stream, check_overflow=False)
         self.decoder_calls += keys.shape[0]
+
+    def _check_overflow(self):
+        ov = self.dec.stats()["overflows"]
+        if ov != self._overflows:
+            raise DecoderError(-4, "a context needed more bins than the decoder's max_bins (%d); create "
+                                   "the decoder with max_bins=n_parts" % self.dec.max_bins)
 
     def _add_training(self, keys, fit, feasible=None):
         """Append the feasible (keys, fitness) pairs to the ring buffer.  With
@@ -361,7 +371,9 @@ class SurrogateBRKGA:
             if target is not None and self.best <= target:
                 self.time_to_target = time.perf_counter() - t0
                 break
-        return time.perf_counter() - t0
+        elapsed = time.perf_counter() - t0
+        self._check_overflow()
+        return elapsed
 
     def close(self):
         if self.net is not None:

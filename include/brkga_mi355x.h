@@ -192,7 +192,11 @@ void brkga_destroy(brkga_handle *h);
  * tracking) is one captured hipGraph replay.  Random numbers come from a
  * counter-based hash of (seed, generation, row, gene), not NumPy's stream.
  * The decoder handle must have max_bins large enough for every context
- * (max_bins = n_parts is always enough): a graph replay cannot grow it.
+ * (max_bins = n_parts is always enough): a graph replay cannot grow it, and
+ * brkga_ga_create / brkga_ga_run return BRKGA_EOVERFLOW when a context ran out
+ * of bins.  The captured graph is captured again when the decoder's device
+ * buffers have moved since (a larger population decoded through the same
+ * handle, or a grown bin workspace), so the decoder may be used between runs.
  */
 typedef struct brkga_ga_params {
     int32_t num_individuals;   /* population N, 2..4096 */

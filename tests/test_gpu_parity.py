@@ -6,9 +6,11 @@ import sys
 import numpy as np
 import pytest
 
+from brkga_fft2_amd.problem import synthetic_population
 from oracle import oracle
-from tests.helpers import (GOLDEN, ref_instance, syn_instance, syn_keys, load_fingerprints,
-                           fingerprint_population, fingerprint_of, feasible_logs_equal)
+from tests.helpers import (GOLDEN, custom_instance, feasible_logs_equal, fingerprint_of,
+                           fingerprint_population, load_fingerprints, oracle_threads, ref_instance,
+                           syn_instance, syn_keys)
 
 pytestmark = pytest.mark.gpu
 
@@ -28,7 +30,7 @@ def test_synthetic_parity(cfg, n_chrom):
     _say("synthetic parity %s N=%d" % (cfg, n_chrom))
     prob, n, M, packed = syn_instance(cfg)
     keys = syn_keys(cfg, n_chrom, n)
-    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=os.cpu_count() or 4, with_log=True)
+    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=oracle_threads(), with_log=True)
     dec = _decoder(packed)
     ms, log = dec.evaluate(keys, with_log=True)
     np.testing.assert_array_equal(ms, ref_ms)
@@ -36,6 +38,50 @@ def test_synthetic_parity(cfg, n_chrom):
     st = dec.stats()
     assert st["chromosomes"] == n_chrom and st["overflows"] == 0
     dec.close()
+
+
+@pytest.mark.parametrize("cfg,n_chrom", [("C3", 1024), ("C5", 512)])
+def test_full_population_parity(cfg, n_chrom):
+    """Populations of the configurations' real sizes (C3: 4096 contexts on 512
+    persistent workgroups, C5: 2048 contexts on 256), so the work queue hands
+    out later contexts in the heaviest-first order: every makespan and every
+    placement against the oracle, element-wise
+    (verify_full_population_vs_pp.py:88-194 does the same for P50M2-0)."""
+    _say("full population parity %s N=%d" % (cfg, n_chrom))
+    prob, n, M, packed = syn_instance(cfg)
+    keys = syn_keys(cfg, n_chrom, n, seed=23)
+    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=oracle_threads(), with_log=True)
+    dec = _decoder(packed)
+    ms, log = dec.evaluate(keys, with_log=True)
+    np.testing.assert_array_equal(ms, ref_ms)
+    assert feasible_logs_equal(ref_ms, log, ref_log) is None
+    st = dec.stats()
+    assert st["contexts"] == n_chrom * M and st["overflows"] == 0
+    dec.close()
+
+
+@pytest.mark.parametrize("plates", [((600, 600), (520, 700)), ((1100, 640),)])
+def test_tall_plates_vs_oracle(plates):
+    """Plates with more than 512 valid top rows: the candidate rows of a
+    rotation span several 512-row blocks of the vacancy filter, and the
+    bottom-most ones (the preferred placements) must be found
+    (collision_backend.py:115-128 searches the whole valid region)."""
+    _say("tall plates %s" % (plates,))
+    prob, n, M, packed = custom_instance(24, plates, (30, 160), seed=4)
+    keys = synthetic_population(12, n, seed=31)
+    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=oracle_threads(), with_log=True)
+    assert np.sum(ref_ms < 1e15) >= 6
+    # placements with top rows >= 512 exist, so the lower 512-row blocks are exercised
+    tindex = {int(p): t for t, p in enumerate(packed["type_id"])}
+    e = ref_log[ref_ms < 1e15].reshape(-1, 6)
+    h = np.array([packed["rot_h"][packed["rot_base"][tindex[int(r[1])]] + r[5]] for r in e])
+    assert np.sum(e[:, 4] - h + 1 >= 512) >= 10
+    for dense in (False, True):
+        dec = _decoder(packed, dense=dense)
+        ms, log = dec.evaluate(keys, with_log=True)
+        np.testing.assert_array_equal(ms, ref_ms)
+        assert feasible_logs_equal(ref_ms, log, ref_log) is None
+        dec.close()
 
 
 @pytest.mark.parametrize("dense,tiles", [(True, 8), (True, 1), (False, 2)])
@@ -64,7 +110,7 @@ def test_random_tile_tables_agree(cfg):
     from brkga_fft2_amd.native import TILE_CLASSES
     prob, n, M, packed = syn_instance(cfg)
     keys = syn_keys(cfg, 12, n, seed=5)
-    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=os.cpu_count() or 4, with_log=True)
+    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=oracle_threads(), with_log=True)
     dec = _decoder(packed)
     rng = np.random.default_rng(3)
     for trial in range(3):
@@ -83,7 +129,7 @@ def test_autotune_keeps_results():
     from brkga_fft2_amd import autotune
     prob, n, M, packed = syn_instance("C2")
     keys = syn_keys("C2", 64, n, seed=8)
-    ref_ms, _, _ = oracle.decode(packed, keys, n_threads=os.cpu_count() or 4)
+    ref_ms, _, _ = oracle.decode(packed, keys, n_threads=oracle_threads())
     dec = _decoder(packed)
     dk = torch.from_numpy(keys).cuda()
     rec = autotune.autotune(dec, packed, dk, reps=1, windows=(8, 64), cache=None)
@@ -154,7 +200,7 @@ def test_every_reference_instance_vs_oracle(name):
     _say("instance %s" % name)
     prob, n, M, packed = ref_instance(name)
     keys = _routed_keys(packed, 6, np.random.default_rng(ALL_INSTANCES.index(name)))
-    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=os.cpu_count() or 4, with_log=True)
+    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=oracle_threads(), with_log=True)
     assert np.sum(ref_ms < 1e15) >= 1
     dec = _decoder(packed)
     ms, log = dec.evaluate(keys, with_log=True)
@@ -169,7 +215,7 @@ def test_full_population_vs_oracle_p50m2_0():
     prob, n, M, packed = ref_instance("P50M2-0")
     np.random.seed(42)
     pop = np.random.uniform(0.0, 1.0, size=(500, 100)).astype(np.float32)
-    ref_ms, ref_log, _ = oracle.decode(packed, pop, n_threads=os.cpu_count() or 4, with_log=True)
+    ref_ms, ref_log, _ = oracle.decode(packed, pop, n_threads=oracle_threads(), with_log=True)
     dec = _decoder(packed)
     ms, log = dec.evaluate(pop, with_log=True)
     np.testing.assert_array_equal(ms, ref_ms)

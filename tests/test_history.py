@@ -8,7 +8,7 @@ import numpy as np
 from brkga_fft2_amd.brkga import BRKGA, read_history, write_history
 from brkga_fft2_amd.problem import thresholds_for
 from oracle import oracle
-from tests.helpers import syn_instance
+from tests.helpers import oracle_threads, syn_instance
 
 
 class _OracleEvaluator:
@@ -16,7 +16,7 @@ class _OracleEvaluator:
         self.packed = packed
 
     def evaluate_batch(self, chromosomes):
-        ms, _, _ = oracle.decode(self.packed, np.asarray(chromosomes), n_threads=os.cpu_count() or 4)
+        ms, _, _ = oracle.decode(self.packed, np.asarray(chromosomes), n_threads=oracle_threads())
         return ms.tolist()
 
 

@@ -7,7 +7,7 @@ import pytest
 
 from oracle import oracle
 from tests.helpers import (GOLDEN, ref_instance, load_fingerprints, fingerprint_population,
-                           fingerprint_of)
+                           fingerprint_of, oracle_threads)
 
 
 def test_p75m2_0_placement_log():
@@ -67,7 +67,7 @@ def test_makespan_fingerprints(name):
     fp = load_fingerprints()[name]
     prob, n, M, packed = ref_instance(name)
     pop = fingerprint_population(fp, n)
-    ms, _, _ = oracle.decode(packed, pop, n_threads=os.cpu_count() or 4)
+    ms, _, _ = oracle.decode(packed, pop, n_threads=oracle_threads())
     got, n_feas = fingerprint_of(ms)
     assert n_feas == fp["n_feasible"]
     assert got == fp["fingerprint"]
