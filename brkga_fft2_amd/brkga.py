@@ -14,8 +14,17 @@ numbers as the reference's:
 * next population ``elites + mutants + offspring``, fitness of the elites
   reused (``:244-250``).
 
-Fitness comes from the MI355X decoder through ``cal_fitness`` (the
-``native_full`` / ``wave_batch`` evaluators of ``BRKGA_alg3.py:119-123``).
+Fitness comes from the MI355X decoder through ``cal_fitness``.  Every
+``eval_mode`` of the reference decodes on the GPU (``BRKGA_alg3.py:40-46,119-163``):
+
+* ``native_full`` / ``wave_batch``: the whole population in one batch;
+* ``serial`` / ``thread`` / ``process`` (and ``auto``, which the reference
+  maps to ``serial`` for GPU collision backends and to ``thread`` otherwise):
+  the reference's fitness cache keyed by the chromosome rounded to 4
+  decimals -- a chromosome whose key was seen before takes the cached
+  makespan -- and the misses decoded together in one batch instead of one
+  per worker.  The cache changes which makespans a run sees, so it is kept
+  for a run to follow the reference's trajectory in these modes.
 """
 import json
 import math
@@ -31,7 +40,7 @@ class BRKGA:
     """Biased random-key GA over the population decoder."""
 
     def __init__(self, problem_data, nbParts, nbMachines, thresholds, instanceParts, initialSol,
-                 collision_backend=None, eval_mode="native_full", eval_workers=4, eval_chunksize=1,
+                 collision_backend=None, eval_mode="auto", eval_workers=4, eval_chunksize=1,
                  num_generations=200, num_individuals=100, num_elites=12, num_mutants=18,
                  eliteCProb=0.7, evaluator=None):
         self.problem_data = problem_data
@@ -41,12 +50,21 @@ class BRKGA:
         self.instanceParts = instanceParts
         self.initialSol = initialSol
         self.collision_backend = collision_backend
-        if eval_mode not in ("native_full", "wave_batch", "auto"):
-            raise ValueError("eval_mode %r: this build decodes on the GPU (native_full / wave_batch)"
-                             % eval_mode)
-        self.eval_mode = "native_full"
-        self.eval_workers = eval_workers
-        self.eval_chunksize = eval_chunksize
+        if eval_mode == "auto":   # BRKGA_alg3.py:40-46
+            name = getattr(collision_backend, "name", None)
+            gpu = name is None or "gpu" in name or "cuda" in name   # this decoder is a GPU one
+            self.eval_mode = "serial" if gpu else "thread"
+        else:
+            self.eval_mode = eval_mode
+        if self.eval_mode == "wave_batch" and os.getenv("ABRKGA_FULL_NATIVE_DECODER", "0").strip() not in \
+                {"0", "false", "False"}:   # BRKGA_alg3.py:77-79
+            self.eval_mode = "native_full"
+        self.eval_workers = int(eval_workers) if eval_workers else 0
+        self.eval_chunksize = int(eval_chunksize) if eval_chunksize else 1
+        if self.eval_mode not in {"wave_batch", "native_full"}:   # BRKGA_alg3.py:96-101
+            self._fitness_cache = {}
+            self._cache_hits = 0
+            self._cache_misses = 0
         self.num_generations = num_generations
         self.num_individuals = int(num_individuals)
         self.num_gene = 2 * self.N
@@ -60,9 +78,48 @@ class BRKGA:
         self._native_decoder = evaluator or FullNativeDecoderEvaluator(
             problem_data, nbParts, nbMachines, thresholds, instanceParts, collision_backend)
 
+    def _hash_solution(self, solution):
+        """``BRKGA_alg3.py:114-116``: the chromosome rounded to 4 decimals."""
+        return tuple(np.round(solution, 4))
+
+    def evaluate_solution(self, solution):
+        """Makespan of one chromosome (``BRKGA_alg3.py:103-112``)."""
+        return self._native_decoder.evaluate_batch(np.asarray(solution)[None])[0]
+
     def cal_fitness(self, population):
-        """``BRKGA_alg3.py:119-121``: a Python list of makespans."""
-        return self._native_decoder.evaluate_batch(np.array(population))
+        """``BRKGA_alg3.py:119-163``: a Python list of makespans."""
+        if self.eval_mode in ("native_full", "wave_batch"):
+            return self._native_decoder.evaluate_batch(np.array(population))
+        if self.eval_mode not in ("serial", "thread", "process"):
+            raise ValueError(f"Unsupported eval_mode: {self.eval_mode}")
+        if self.eval_mode == "process" and "cuda" in getattr(self.collision_backend, "name", ""):
+            raise ValueError("eval_mode=process is not supported with CUDA backends. Use thread or serial.")
+        results = [None] * len(population)
+        todo = []
+        for i, sol in enumerate(population):
+            key = self._hash_solution(sol)
+            if key in self._fitness_cache:
+                results[i] = self._fitness_cache[key]
+                self._cache_hits += 1
+            else:
+                todo.append((i, key))
+                self._cache_misses += 1
+        if todo:
+            batch = np.array([population[i] for i, _ in todo])
+            new = self._native_decoder.evaluate_batch(batch)
+            for (i, key), f in zip(todo, new):
+                results[i] = f
+                self._fitness_cache[key] = f
+        return results
+
+    def shutdown(self):
+        """``BRKGA_alg3.py:164-175``: report the fitness cache's effectiveness."""
+        if self.eval_mode not in {"wave_batch", "native_full"}:
+            total = self._cache_hits + self._cache_misses
+            if total > 0:
+                hit_rate = 100.0 * self._cache_hits / total
+                print(f"Fitness cache: {self._cache_hits}/{total} hits ({hit_rate:.1f}%), "
+                      f"{len(self._fitness_cache)} unique solutions")
 
     def partition(self, population, fitness_list):
         fitness_arr = np.asarray(fitness_list)

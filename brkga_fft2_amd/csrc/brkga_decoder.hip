@@ -82,7 +82,10 @@ typedef LDSP const int lds_ci32;
 
 enum {
     ST_CALLS = 0, ST_CHROM, ST_CTX, ST_PLACE, ST_BINS, ST_TESTS, ST_GROUPS, ST_MFMA, ST_ALGO,
-    ST_OVERFLOW, ST_PH0, ST_CTXMAX = ST_PH0 + 8, ST_CTXSUM, ST_COUNT
+    ST_OVERFLOW, ST_PH0, ST_CTXMAX = ST_PH0 + 8, ST_CTXSUM,
+    // per machine (BRKGA_MAX_MACHINES each): contexts, bins opened, most bins of one context
+    ST_MCTX, ST_MBINS = ST_MCTX + BRKGA_MAX_MACHINES, ST_MPEAK = ST_MBINS + BRKGA_MAX_MACHINES,
+    ST_COUNT = ST_MPEAK + BRKGA_MAX_MACHINES
 };
 
 struct DevProblem {
@@ -1044,7 +1047,10 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
         const int log_off = uni(s_off);
         int *logp = L.log ? L.log + (chrom * n + log_off) * 6 : nullptr;
         if (s_infeasible) {   // placement.py:47-53
-            if (tid == 0) L.ctx_ms[ctx] = -1.0;
+            if (tid == 0) {
+                L.ctx_ms[ctx] = -1.0;
+                atomicAdd(&Wk.counters[ST_MCTX + mach], 1ull);
+            }
             if (logp)
                 for (int k = tid; k < cnt * 6; k += WG_THREADS) logp[k] = -1;
             __syncthreads();
@@ -1136,15 +1142,19 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
             }
         }
         if (tid == 0) {
+            const int nbf = uni(s_nb);
             if (overflow) {
                 L.ctx_ms[ctx] = -2.0;
                 c_over++;
             } else {
                 double total = 0.0;   // placement.py:86-89
-                const int nbf = uni(s_nb);
                 for (int b = 0; b < nbf; b++) total = total + ((b_pt[b] + b_pth[b]) + P.setup[mach]);
                 L.ctx_ms[ctx] = total;
             }
+            // per-machine bin statistics (full_native_decoder.py:1405-1417)
+            atomicAdd(&Wk.counters[ST_MCTX + mach], 1ull);
+            atomicAdd(&Wk.counters[ST_MBINS + mach], (unsigned long long)nbf);
+            atomicMax(&Wk.counters[ST_MPEAK + mach], (unsigned long long)nbf);
             const unsigned long long t = tick();
             ph[7] += t - tp;
             ctx_sum += t - tc0;
@@ -1292,6 +1302,7 @@ struct brkga_handle {
     // (workspace growth, larger per-call buffers): a captured graph holds the
     // old addresses and must be captured again.
     unsigned ws_gen = 0;
+    long long growths = 0;   // workspace growths after an overflow (since the last reset)
     double wall_hz = 1e8;
     std::string err;
 };
@@ -1333,7 +1344,7 @@ static int validate(const brkga_problem *p, std::string &msg) {
     char buf[256];
     if (!p) { msg = "problem is NULL"; return BRKGA_EINVAL; }
     if (p->n_parts < 1 || p->n_parts > 8192) { msg = "n_parts must be in [1, 8192]"; return BRKGA_EINVAL; }
-    if (p->n_machines < 1 || p->n_machines > 64) { msg = "n_machines must be in [1, 64]"; return BRKGA_EINVAL; }
+    if (p->n_machines < 1 || p->n_machines > BRKGA_MAX_MACHINES) { msg = "n_machines must be in [1, 64]"; return BRKGA_EINVAL; }
     if (p->n_types < 1 || p->n_rots < 1) { msg = "n_types and n_rots must be >= 1"; return BRKGA_EINVAL; }
     if (!p->slot_type || !p->type_id || !p->area || !p->nrot || !p->best_rot || !p->rot_base ||
         !p->rot_h || !p->rot_w || !p->mask_off || !p->masks || !p->plate_h || !p->plate_w ||
@@ -1640,26 +1651,38 @@ int brkga_internal_reserve(brkga_handle *h, int64_t n_chrom) {
     return 0;
 }
 
-int brkga_internal_launch(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_chrom,
-                          double *d_out, int32_t *d_log, hipStream_t s, bool timed) {
+// Per-call device buffers of one decode launch: the contexts' makespans, their
+// decode order and cost buckets, the order histogram (zero before the launch,
+// cleared again by order_scatter_kernel) and the work-queue counter.
+struct CallBuffers {
+    double *ctx_ms;
+    int *order, *bucket, *hist;
+    unsigned int *queue;
+};
+
+// Enqueue the decode of n_chrom device-resident chromosomes on s with the
+// given per-call buffers (sized for n_chrom * n_machines contexts); allocates
+// nothing.
+static int launch_decode(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_chrom, double *d_out,
+                         int32_t *d_log, hipStream_t s, bool timed, const CallBuffers &cb) {
     const int M = h->prob.n_machines;
-    int rc = brkga_internal_reserve(h, n_chrom);
-    if (rc) return rc;
     if (timed) settle_events(h);
     LaunchArgs L;
     L.keys = d_keys; L.key_f64 = key_dtype == BRKGA_KEYS_F64; L.n_chrom = n_chrom;
-    L.ctx_ms = h->d_ctx; L.log = d_log; L.order = h->d_order;
+    L.ctx_ms = cb.ctx_ms; L.log = d_log; L.order = cb.order;
+    DevWork wk = h->dw;
+    wk.queue = cb.queue;
     const long long n_ctx = n_chrom * M;
     const int grid = (int)std::min<long long>(h->n_wg, std::max<long long>(n_ctx, 1));
     hipLaunchKernelGGL(order_cost_kernel, dim3((unsigned)((n_ctx + 3) / 4)), dim3(256), 0, s, h->dp, d_keys,
-                       L.key_f64, (long long)n_chrom, h->cost_all, h->d_bucket, h->d_hist);
+                       L.key_f64, (long long)n_chrom, h->cost_all, cb.bucket, cb.hist);
     HIPCHK(h, hipGetLastError());
-    hipLaunchKernelGGL(order_scatter_kernel, dim3(1), dim3(ORDER_BUCKETS), 0, s, h->d_bucket, h->d_hist, n_ctx,
-                       h->d_order, h->dw.queue);
+    hipLaunchKernelGGL(order_scatter_kernel, dim3(1), dim3(ORDER_BUCKETS), 0, s, cb.bucket, cb.hist, n_ctx,
+                       cb.order, cb.queue);
     HIPCHK(h, hipGetLastError());
     if (timed) HIPCHK(h, hipEventRecord(h->ev0, s));
     auto kern = h->dense ? decode_kernel<true> : decode_kernel<false>;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, L);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(WG_THREADS), h->smem, s, h->dp, wk, L);
     HIPCHK(h, hipGetLastError());
     if (timed) {
         HIPCHK(h, hipEventRecord(h->ev1, s));
@@ -1667,9 +1690,17 @@ int brkga_internal_launch(brkga_handle *h, const void *d_keys, int key_dtype, in
     }
     const int fb = 256;
     hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)((n_chrom + fb - 1) / fb)), dim3(fb), 0, s,
-                       h->d_ctx, (long long)n_chrom, M, d_out, h->dw.counters);
+                       cb.ctx_ms, (long long)n_chrom, M, d_out, h->dw.counters);
     HIPCHK(h, hipGetLastError());
     return 0;
+}
+
+int brkga_internal_launch(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_chrom,
+                          double *d_out, int32_t *d_log, hipStream_t s, bool timed) {
+    int rc = brkga_internal_reserve(h, n_chrom);
+    if (rc) return rc;
+    const CallBuffers cb{h->d_ctx, h->d_order, h->d_bucket, h->d_hist, h->dw.queue};
+    return launch_decode(h, d_keys, key_dtype, n_chrom, d_out, d_log, s, timed, cb);
 }
 
 int brkga_internal_n_parts(const brkga_handle *h) { return h->prob.n_parts; }
@@ -1748,6 +1779,7 @@ extern "C" int brkga_evaluate(brkga_handle *h, const void *keys, int key_dtype, 
         if (nb == h->dw.max_bins) break;
         HIPCHK(h, hipStreamSynchronize(s));
         if ((rc = device_workspace(h, nb))) return rc;
+        h->growths++;
     }
     if (rc) return rc;
     HIPCHK(h, hipMemcpyAsync(out, h->d_out, (size_t)n_chrom * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1975,6 +2007,15 @@ extern "C" int brkga_get_stats(const brkga_handle *hc, brkga_stats *out) {
     for (int k = 0; k < 8; k++) out->phase_us[k] = (double)c[ST_PH0 + k] * us_per_tick;
     out->context_us_max = (double)c[ST_CTXMAX] * us_per_tick;
     out->context_us_sum = (double)c[ST_CTXSUM] * us_per_tick;
+    out->growths = h->growths;
+    out->max_bins = h->dw.max_bins;
+    out->n_workgroups = h->n_wg;
+    for (int m = 0; m < BRKGA_MAX_MACHINES; m++) {
+        const bool used = m < h->prob.n_machines;
+        out->machine_contexts[m] = used ? (int64_t)c[ST_MCTX + m] : 0;
+        out->machine_bins[m] = used ? (int64_t)c[ST_MBINS + m] : 0;
+        out->machine_peak_bins[m] = used ? (int32_t)c[ST_MPEAK + m] : 0;
+    }
     return 0;
 }
 
@@ -1983,6 +2024,7 @@ extern "C" int brkga_reset_stats(brkga_handle *h) {
     HIPCHK(h, hipSetDevice(h->device));
     settle_events(h);
     HIPCHK(h, hipMemset(h->dw.counters, 0, ST_COUNT * sizeof(unsigned long long)));
+    h->growths = 0;
     h->kernel_ms = 0.0;
     h->last_ms = 0.0;
     return 0;

@@ -12,7 +12,11 @@ Same constructor and ``evaluate_batch`` contract as the reference's
   too, ``full_native_decoder.py:2950``);
 * setting ``_collect_contexts = True`` fills ``_placement_log`` with
   ``machine -> [(part_id, bin_idx, col, row, rot, shape), ...]`` as the
-  reference does (``wave_batch_evaluator.py:124-136,704-706``).
+  reference does (``wave_batch_evaluator.py:124-136,704-706``), and
+  ``_last_contexts`` with ``machine -> [BatchPlacementContext per solution]``
+  whose ``open_bins`` carry each bin's grid, free runs, area, processing
+  times and parts (``wave_batch_evaluator.py:74-106,251-252``), rebuilt on the
+  host from the decoder's placement log (``contexts.py``).
 
 Keys are decoded in the caller's dtype (float32 or float64), as the
 placement procedure does (``placement.py:109-133``).
@@ -21,6 +25,7 @@ from typing import List
 
 import numpy as np
 
+from .contexts import build_contexts
 from .native import Decoder
 from .problem import pack_problem
 
@@ -65,6 +70,9 @@ class FullNativeDecoderEvaluator:
         ms, log = self._decoder.evaluate(keys, with_log=self._collect_contexts)
         if self._collect_contexts:
             self._record_log(log)
+            self._last_contexts.update(build_contexts(
+                self.problem_data, self.nbParts, self.nbMachines, self.thresholds,
+                self.instance_parts, keys, log))
         return ms.tolist()
 
     def evaluate_device(self, keys, out=None, log=None, stream=None):
@@ -88,6 +96,15 @@ class FullNativeDecoderEvaluator:
         return self._decoder.stats()
 
     def reset_stats(self):
+        self._decoder.reset_stats()
+
+    def get_bin_stats(self):
+        """Per-machine bin statistics, as the reference's native decoder reports
+        them (``full_native_decoder.py:2286-2301``)."""
+        return self._decoder.bin_stats()
+
+    def reset_bin_stats(self):
+        """``full_native_decoder.py:2303-2312`` (resets every counter)."""
         self._decoder.reset_stats()
 
     def close(self):

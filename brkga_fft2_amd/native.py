@@ -44,7 +44,12 @@ class BrkgaStats(ctypes.Structure):
                 ("algo_ops", ctypes.c_int64), ("overflows", ctypes.c_int64),
                 ("kernel_ms", ctypes.c_double), ("last_kernel_ms", ctypes.c_double),
                 ("phase_us", ctypes.c_double * 8), ("context_us_max", ctypes.c_double),
-                ("context_us_sum", ctypes.c_double)]
+                ("context_us_sum", ctypes.c_double), ("growths", ctypes.c_int64),
+                ("max_bins", ctypes.c_int32), ("n_workgroups", ctypes.c_int32),
+                ("machine_contexts", ctypes.c_int64 * 64), ("machine_bins", ctypes.c_int64 * 64),
+                ("machine_peak_bins", ctypes.c_int32 * 64)]
+
+MAX_MACHINES = 64   # BRKGA_MAX_MACHINES
 
 PHASES = ["sequence", "vacancy_filter", "band_staging", "correlation", "group_reduce",
           "stamp_update", "new_bin", "makespan"]
@@ -223,7 +228,22 @@ class Decoder:
         self._check(self._lib.brkga_get_stats(self._h, ctypes.byref(st)), "brkga_get_stats")
         d = {k: getattr(st, k) for k, _ in BrkgaStats._fields_}
         d["phase_us"] = dict(zip(PHASES, list(st.phase_us)))
+        M = self.n_machines
+        for k in ("machine_contexts", "machine_bins", "machine_peak_bins"):
+            d[k] = list(getattr(st, k))[:M]
         return d
+
+    def bin_stats(self):
+        """Per machine ``(peak_bins_per_sol, total_bins, total_sols, num_batches,
+        peak_slots, num_growth_events)``, the tuple of the reference's
+        ``get_bin_stats`` (``full_native_decoder.py:2286-2301``): the most bins one
+        context of the machine opened, the bins its contexts opened, its
+        contexts, the decode calls, the bin slots one context may use (each
+        persistent workgroup reuses its own ``max_bins`` slots), and the
+        workspace growths."""
+        d = self.stats()
+        return [(d["machine_peak_bins"][m], d["machine_bins"][m], d["machine_contexts"][m], d["calls"],
+                 d["max_bins"], d["growths"]) for m in range(self.n_machines)]
 
     def reset_stats(self):
         self._check(self._lib.brkga_reset_stats(self._h), "brkga_reset_stats")

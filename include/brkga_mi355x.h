@@ -46,6 +46,8 @@ extern "C" {
 #define BRKGA_KEYS_F32 0
 #define BRKGA_KEYS_F64 1
 
+#define BRKGA_MAX_MACHINES 64   /* printer machines per problem at most */
+
 /* Infeasible chromosomes decode to this makespan (placement.py:9). */
 #define BRKGA_INFEASIBLE 1e16
 
@@ -111,6 +113,14 @@ typedef struct brkga_stats {
     double phase_us[8];
     double context_us_max;   /* longest single context */
     double context_us_sum;   /* all contexts */
+    /* Bin statistics, the reference's get_bin_stats (full_native_decoder.py:2286-2312,
+     * per machine: peak bins per solution, total bins, solutions, growth events). */
+    int64_t growths;         /* workspace growths: brkga_evaluate doubled max_bins after an overflow */
+    int32_t max_bins;        /* bins per context the workspace holds now */
+    int32_t n_workgroups;    /* persistent workgroups of a decode launch */
+    int64_t machine_contexts[BRKGA_MAX_MACHINES];    /* contexts decoded on machine m (feasible or not) */
+    int64_t machine_bins[BRKGA_MAX_MACHINES];        /* bins the contexts of machine m opened */
+    int32_t machine_peak_bins[BRKGA_MAX_MACHINES];   /* most bins one context of machine m opened */
 } brkga_stats;
 
 typedef struct brkga_handle brkga_handle;
