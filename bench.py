@@ -86,7 +86,7 @@ def _workload(config, pop, n, M, cfg):
                                    cfg["bbox"][0], cfg["bbox"][1], cfg["nrot"], pop, cfg["key_dtype"]))
 
 
-def _timed_decode(config, pop, steps, warmup, seed, dev, autotune_reps=0, only_timed=False):
+def _timed_decode(config, pop, steps, warmup, seed, dev, autotune_reps=0, only_timed=False, dense=False):
     """Decode ``warmup`` then ``steps`` device-resident populations of ``config``
     back to back on one stream; returns the record and the state later legs use."""
     import numpy as np
@@ -103,7 +103,7 @@ def _timed_decode(config, pop, steps, warmup, seed, dev, autotune_reps=0, only_t
     keys = [torch.from_numpy(p).to(dev) for p in host_pops]
     outs = [torch.empty(pop, dtype=torch.float64, device=dev) for _ in range(total)]
     stream = torch.cuda.current_stream(dev)
-    dec = Decoder(packed, device=0)
+    dec = Decoder(packed, device=0, dense=dense)
     torch.cuda.synchronize()
     tuned = None
     if autotune_reps:
@@ -425,6 +425,9 @@ def main():
     ap.add_argument("--only-timed", action="store_true",
                     help="stop after the timed steps (for profiler runs: every decode launch is then "
                          "a warm-up or timed one)")
+    ap.add_argument("--dense", action="store_true",
+                    help="with --only-timed: time the full-shape decode instead (every placement row, no "
+                         "early exit; profiler runs of the roofline kernel)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true",
                     help="skip the plain single-chromosome path (population 1 per launch)")
@@ -470,8 +473,10 @@ def main():
     dev = torch.device("cuda:0")
     reps = args.autotune if args.autotune >= 0 else (0 if args.config == "C5" else 3)
     _say("config %s pop=%d steps=%d warmup=%d" % (args.config, pop, args.steps, args.warmup))
+    if args.dense and not args.only_timed:
+        ap.error("--dense needs --only-timed (the full-shape decode is not the headline)")
     rec, state = _timed_decode(args.config, pop, args.steps, args.warmup, args.seed, dev,
-                               autotune_reps=reps, only_timed=args.only_timed)
+                               autotune_reps=reps, only_timed=args.only_timed, dense=args.dense)
     if args.only_timed:
         state["dec"].close()
         print(json.dumps({"config": rec["workload"], "value": round(rec["value"], 2),

@@ -92,3 +92,36 @@ def gradients(x, target, params, fp, hidden):
     db1 = dH1.sum(axis=0)
     dW1 = x.T @ dH1
     return np.concatenate([dW1.ravel(), db1, dW2.ravel(), db2, dw3, db3])
+
+
+def gradient_magnitudes(x, target, params, fp, hidden):
+    """Per gradient element (same layout as ``gradients``): S, the sum of the
+    absolute values of the products its sum adds (float64), and T, an upper
+    bound of any single product.  A float32 accumulation of bfloat16 operands
+    differs from the float64 value by at most (K u32 + 2 u16) S (K terms, u32 =
+    2^-24 and u16 = 2^-8 the unit roundoffs, two stored bf16 operands per
+    product off by at most one unit each), plus T for a term whose ReLU mask
+    flips between the two computations."""
+    W1, b1, W2, b2, w3, b3 = unpack(params, fp, hidden)
+    y, H1, H2 = forward(x, params, fp, hidden)
+    B = x.shape[0]
+    dy = 2.0 * (y - np.asarray(target, np.float64)) / B
+    dH2 = bf16(np.where(H2 > 0, dy[:, None] * w3[None, :], 0.0))
+    dH1u = bf16(dH2 @ bf16(W2).T)          # dH1 without its mask: a flipped mask adds this term
+    dH1 = np.where(H1 > 0, dH1u, 0.0)
+    ax, aH1, aH2, ady, adH2, adH1, adH1u = (np.abs(v) for v in (x, H1, H2, dy, dH2, dH1, dH1u))
+    S = np.concatenate([(ax.T @ adH1).ravel(), adH1.sum(axis=0), (aH1.T @ adH2).ravel(), adH2.sum(axis=0),
+                        aH2.T @ ady, [ady.sum()]])
+    dH2u = np.abs(dy)[:, None] * np.abs(w3)[None, :]
+    T = np.concatenate([np.outer(ax.max(axis=0), adH1u.max(axis=0)).ravel(), adH1u.max(axis=0),
+                        np.outer(aH1.max(axis=0), dH2u.max(axis=0)).ravel(), dH2u.max(axis=0),
+                        aH2.max(axis=0) * ady.max(), [ady.max()]])
+    return S, T
+
+
+def gradient_tolerance(x, target, params, fp, hidden):
+    """Element-wise bound on |GPU gradient - float64 gradient| (see
+    ``gradient_magnitudes``)."""
+    S, T = gradient_magnitudes(x, target, params, fp, hidden)
+    K = max(x.shape[0], fp, hidden)
+    return (K * 2.0 ** -23 + 2.0 ** -6) * S + T + 1e-30

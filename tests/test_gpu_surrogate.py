@@ -41,11 +41,17 @@ def test_features_bit_exact():
     net.close()
 
 
-@pytest.mark.parametrize("B", [64, 200, 333, 1024])   # 333, 1024: split-k weight gradients
-def test_forward_and_gradients_match_oracle(B):
-    """Tolerance: float32 accumulation vs float64, plus occasional one-ulp bf16 flips of a
-    stored activation; per tensor max |diff| <= 2e-2 * max |value|, cosine >= 0.999."""
-    _say("mlp gradients B=%d" % B)
+@pytest.mark.parametrize("B,same_sign", [(64, False), (200, False), (333, False), (1024, False), (1024, True)])
+def test_forward_and_gradients_match_oracle(B, same_sign):
+    """Element-wise tolerance (mlp_oracle.gradient_tolerance): |GPU - float64| <=
+    (K 2^-23 + 2^-6) S + T per gradient element, S the sum of |products| the
+    element adds and T one product (float32 accumulation of K terms, one-ulp
+    bf16 flips of the two stored operands of a product, one flipped ReLU
+    mask); also cosine >= 0.999 per tensor.  B = 333 and 1024 take the split-k
+    weight gradients; with same_sign the targets lie below every prediction, so
+    the output-side gradients add products of one sign and a lost split-k
+    partial exceeds the bound (test_mlp_oracle pins that on the CPU)."""
+    _say("mlp gradients B=%d same_sign=%s" % (B, same_sign))
     import torch
     net, area, height = _net(seed=B)
     rng = np.random.default_rng(B)
@@ -55,28 +61,33 @@ def test_forward_and_gradients_match_oracle(B):
     net.set_params(p.astype(np.float32))
     p = net.get_params().astype(np.float64)
     keys = rng.random((B, 100)).astype(np.float32)
-    t = rng.normal(0, 1, B).astype(np.float32)
     dk = torch.from_numpy(keys).cuda()
     y = net.predict(dk).cpu().numpy().astype(np.float64)
     x = mo.features(keys, area, height, net.fp, THR)
     y_ref, _, _ = mo.forward(x, p, net.fp, net.hidden)
     assert np.max(np.abs(y - y_ref)) <= 2e-2 * max(1.0, np.max(np.abs(y_ref)))
+    if same_sign:
+        t = (y_ref - 1.0 - rng.random(B)).astype(np.float32)
+    else:
+        t = rng.normal(0, 1, B).astype(np.float32)
     g = net.gradients(dk, torch.from_numpy(t).cuda()).cpu().numpy().astype(np.float64)
-    g_ref = mo.gradients(x, t, p, net.fp, net.hidden)
+    t64 = t.astype(np.float64)
+    g_ref = mo.gradients(x, t64, p, net.fp, net.hidden)
+    tol = mo.gradient_tolerance(x, t64, p, net.fp, net.hidden)
     fp, hd = net.fp, net.hidden
     sizes = [fp * hd, hd, hd * hd, hd, hd, 1]
     o = 0
     for name, sz in zip(["W1", "b1", "W2", "b2", "w3", "b3"], sizes):
-        a, b = g[o:o + sz], g_ref[o:o + sz]
+        a, b, tl = g[o:o + sz], g_ref[o:o + sz], tol[o:o + sz]
         o += sz
-        scale = max(np.max(np.abs(b)), 1e-12)
-        assert np.max(np.abs(a - b)) <= 2e-2 * scale, name
+        bad = np.flatnonzero(np.abs(a - b) > tl)
+        assert bad.size == 0, (name, bad[:5], (a - b)[bad[:5]], tl[bad[:5]])
         if sz > 1:
             cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300)
             assert cos >= 0.999, name
     # the loss a training step reports is the mean squared error before its update
     loss = net.train_step(dk, torch.from_numpy(t).cuda())
-    loss_ref = float(np.mean((y_ref - t.astype(np.float64)) ** 2))
+    loss_ref = float(np.mean((y_ref - t64) ** 2))
     assert abs(loss - loss_ref) <= 2e-2 * max(1.0, loss_ref)
     net.close()
 

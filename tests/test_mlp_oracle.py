@@ -74,3 +74,44 @@ def test_machine_bucket_matches_decoder_assignment():
         want = Counter(zip(mach.tolist(), type_id[slot_type].tolist()))
         got = Counter((int(r[0]), int(r[1])) for r in log[b])
         assert got == want
+
+
+def _net_params(fp, hidden, rng):
+    p = np.concatenate([rng.normal(0, 1 / np.sqrt(fp), fp * hidden), rng.normal(0, 0.1, hidden),
+                        rng.normal(0, 1 / np.sqrt(hidden), hidden * hidden), rng.normal(0, 0.1, hidden),
+                        np.abs(rng.normal(0, 1 / np.sqrt(hidden), hidden)), [0.1]])
+    return p
+
+
+def test_gradient_tolerance_accepts_float32_and_rejects_a_dropped_chunk():
+    """The GPU gradient test's element-wise tolerance: a float32 evaluation of
+    the same network (products and sums rounded to float32) stays inside it,
+    while a gradient that loses one 64-row chunk of the batch (as a split-k
+    partial left out would) falls outside it for most elements of every
+    tensor (targets below every prediction and w3 > 0, so the products of the
+    output-side gradients share one sign)."""
+    rng = np.random.default_rng(5)
+    n, hidden, B = 50, 128, 1024
+    fp = 4 * n + (-4 * n) % 32
+    x = mo.features(rng.random((B, 2 * n)).astype(np.float32), rng.random(n), rng.random(n), fp,
+                    [1 / 3, 2 / 3])
+    p = _net_params(fp, hidden, rng)
+    y, _, _ = mo.forward(x, p, fp, hidden)
+    t = y - 1.0 - rng.random(B)              # every prediction above its target
+    g = mo.gradients(x, t, p, fp, hidden)
+    tol = mo.gradient_tolerance(x, t, p, fp, hidden)
+    # float32 evaluation
+    p32 = p.astype(np.float32).astype(np.float64)
+    g32 = mo.gradients(x.astype(np.float32).astype(np.float64), t.astype(np.float32).astype(np.float64),
+                       p32, fp, hidden).astype(np.float32).astype(np.float64)
+    assert np.all(np.abs(g32 - g) <= tol)
+    # one 64-row chunk of the batch missing from the weight gradients
+    keep = np.ones(B, bool)
+    keep[128:192] = False
+    gd = mo.gradients(x[keep], t[keep], p, fp, hidden) * (keep.sum() / B)   # same 2/B scale as the full batch
+    sizes = [fp * hidden, hidden, hidden * hidden, hidden, hidden, 1]
+    offs = np.cumsum([0] + sizes)
+    for k, name in enumerate(("W1", "b1", "W2", "b2", "w3", "b3")):
+        sl = slice(offs[k], offs[k + 1])
+        frac = np.mean(np.abs(gd[sl] - g[sl]) > tol[sl])
+        assert frac > 0.5, (name, frac)   # elements with no product in the dropped rows stay put
