@@ -48,6 +48,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1304,6 +1305,7 @@ struct brkga_handle {
     unsigned ws_gen = 0;
     long long growths = 0;   // workspace growths after an overflow (since the last reset)
     double wall_hz = 1e8;
+    void *scratch = nullptr;   // device address of `scratch` on this handle's GPU
     std::string err;
 };
 
@@ -1411,6 +1413,8 @@ static int device_workspace(brkga_handle *h, int max_bins) {
     w.vac = (int *)p;
     return 0;
 }
+
+static int scratch_address(brkga_handle *h, int device, void **out);
 
 extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkga_handle **out) {
     if (!out) return fail(nullptr, BRKGA_EINVAL, "out is NULL");
@@ -1615,6 +1619,7 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
             return bail(BRKGA_EDEVICE);
         }
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) { h->err = "hipEventCreate failed"; return bail(BRKGA_EDEVICE); }
+    if ((rc = scratch_address(h, h->device, &h->scratch))) return bail(rc);
     *out = h;
     return BRKGA_OK;
 }
@@ -1695,6 +1700,55 @@ static int launch_decode(brkga_handle *h, const void *d_keys, int key_dtype, int
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free decode.  The per-call buffers of brkga_evaluate_device_noalloc
+// live in `scratch`, one array per GPU for the life of the process, laid out
+// as: order histogram [ORDER_BUCKETS] int32, queue counter (256 bytes), then
+// for up to BRKGA_NOALLOC_MAX_CONTEXTS contexts their makespans (f64), decode
+// order and cost buckets (int32).  In host code `scratch` names a host-side
+// stand-in: its device address on the current GPU comes from
+// hipGetSymbolAddress, resolved once per device by brkga_create.
+// ---------------------------------------------------------------------------
+#define NOALLOC_HIST_OFF 0
+#define NOALLOC_QUEUE_OFF (ORDER_BUCKETS * 4)
+#define NOALLOC_CTX_OFF (NOALLOC_QUEUE_OFF + 256)
+#define NOALLOC_ORDER_OFF (NOALLOC_CTX_OFF + 8 * BRKGA_NOALLOC_MAX_CONTEXTS)
+#define NOALLOC_BUCKET_OFF (NOALLOC_ORDER_OFF + 4 * BRKGA_NOALLOC_MAX_CONTEXTS)
+#define NOALLOC_BYTES (NOALLOC_BUCKET_OFF + 4 * BRKGA_NOALLOC_MAX_CONTEXTS)
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+#define NOALLOC_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];   // device address of `scratch` per GPU
+
+// Device address of `scratch` on `device`, which must be current; resolved on
+// the first call for the device and kept.
+static int scratch_address(brkga_handle *h, int device, void **out) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) {
+        h->err = "device ordinal out of range for the allocation-free decode's scratch table";
+        return BRKGA_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            h->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return BRKGA_EDEVICE;
+        }
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
+}
+
+// Zero the histogram and the queue counter of the scratch buffers (a kernel,
+// not a memset node, so that a caller may capture the call into a graph).
+__global__ __launch_bounds__(ORDER_BUCKETS) void scratch_clear_kernel(int *hist, unsigned int *queue) {
+    hist[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *queue = 0u;
+}
+
 int brkga_internal_launch(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_chrom,
                           double *d_out, int32_t *d_log, hipStream_t s, bool timed) {
     int rc = brkga_internal_reserve(h, n_chrom);
@@ -1742,6 +1796,36 @@ extern "C" int brkga_evaluate_device(brkga_handle *h, const void *d_keys, int ke
     if (check_overflow) {
         if ((rc = read_overflow(h, s, &ov1))) return rc;
         settle_events(h);
+        if (ov1 != ov0) return fail(h, BRKGA_EOVERFLOW, "a context needed more bins than max_bins");
+    }
+    return 0;
+}
+
+extern "C" int brkga_evaluate_device_noalloc(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_chrom,
+                                             int64_t n_genes, double *d_out, int32_t *d_log, void *stream,
+                                             int check_overflow) {
+    int rc = check_args(h, d_keys, key_dtype, n_chrom, n_genes, d_out);
+    if (rc) return rc;
+    if (n_chrom * h->prob.n_machines > BRKGA_NOALLOC_MAX_CONTEXTS) {
+        char b[160];
+        snprintf(b, sizeof b, "n_chrom * n_machines = %lld exceeds the allocation-free decode's limit of %d contexts",
+                 (long long)(n_chrom * h->prob.n_machines), BRKGA_NOALLOC_MAX_CONTEXTS);
+        return fail(h, BRKGA_EINVAL, b);
+    }
+    if (!h->scratch) return fail(h, BRKGA_ESTATE, "scratch buffers not resolved");
+    HIPCHK(h, hipSetDevice(h->device));
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long ov0 = 0, ov1 = 0;
+    if (check_overflow && (rc = read_overflow(h, s, &ov0))) return rc;
+    unsigned char *base = (unsigned char *)h->scratch;
+    const CallBuffers cb{(double *)(base + NOALLOC_CTX_OFF), (int *)(base + NOALLOC_ORDER_OFF),
+                         (int *)(base + NOALLOC_BUCKET_OFF), (int *)(base + NOALLOC_HIST_OFF),
+                         (unsigned int *)(base + NOALLOC_QUEUE_OFF)};
+    hipLaunchKernelGGL(scratch_clear_kernel, dim3(1), dim3(ORDER_BUCKETS), 0, s, cb.hist, cb.queue);
+    HIPCHK(h, hipGetLastError());
+    if ((rc = launch_decode(h, d_keys, key_dtype, n_chrom, d_out, d_log, s, false, cb))) return rc;
+    if (check_overflow) {
+        if ((rc = read_overflow(h, s, &ov1))) return rc;
         if (ov1 != ov0) return fail(h, BRKGA_EOVERFLOW, "a context needed more bins than max_bins");
     }
     return 0;

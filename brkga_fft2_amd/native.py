@@ -65,6 +65,7 @@ TILE_CLASSES = 16   # BRKGA_TILE_CLASSES: mask width classes nc = ceil((w + 31) 
 
 EXPORTED = ["brkga_create", "brkga_evaluate", "brkga_evaluate_device", "brkga_get_stats",
             "brkga_reset_stats", "brkga_set_tiles", "brkga_get_tiles", "brkga_initial_solution",
+            "brkga_evaluate_device_noalloc",
             "brkga_last_error",
             "brkga_destroy",
             "brkga_ga_create", "brkga_ga_run", "brkga_ga_best", "brkga_ga_population",
@@ -100,6 +101,9 @@ def load_library(path=LIB_PATH):
     lib.brkga_evaluate_device.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
                                           _P, _P, _P, ctypes.c_int]
     lib.brkga_evaluate_device.restype = ctypes.c_int
+    lib.brkga_evaluate_device_noalloc.argtypes = [_P, _P, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                                  _P, _P, _P, ctypes.c_int]
+    lib.brkga_evaluate_device_noalloc.restype = ctypes.c_int
     lib.brkga_get_stats.argtypes = [_P, ctypes.POINTER(BrkgaStats)]
     lib.brkga_get_stats.restype = ctypes.c_int
     lib.brkga_reset_stats.argtypes = [_P]
@@ -221,6 +225,29 @@ class Decoder:
             log.data_ptr() if log is not None else None, ctypes.c_void_p(stream.cuda_stream),
             int(bool(check_overflow)))
         self._check(rc, "brkga_evaluate_device")
+        return out
+
+    def evaluate_device_noalloc(self, keys, out, log=None, stream=None, check_overflow=False):
+        """``evaluate_device`` through the allocation-free entry point: nothing is
+        allocated (the per-call buffers are the library's static scratch on this
+        GPU, for up to 65536 (chromosome, machine) contexts), so it may run inside
+        a captured stream; calls on one GPU must not overlap."""
+        import torch
+        if keys.device.type != "cuda" or out.device.type != "cuda" or keys.dtype not in (torch.float32, torch.float64):
+            raise ValueError("keys (float32/float64) and out must be GPU tensors")
+        if (not keys.is_contiguous() or keys.dim() != 2 or out.dtype != torch.float64 or not out.is_contiguous()
+                or out.dim() != 1 or out.shape[0] != keys.shape[0]):
+            raise ValueError("keys must be contiguous [N, 2*n_parts] and out contiguous float64 [N]")
+        if keys.shape[0] == 0:
+            return out
+        dt = BRKGA_KEYS_F64 if keys.dtype == torch.float64 else BRKGA_KEYS_F32
+        if stream is None:
+            stream = torch.cuda.current_stream(keys.device)
+        rc = self._lib.brkga_evaluate_device_noalloc(
+            self._h, keys.data_ptr(), dt, keys.shape[0], keys.shape[1], out.data_ptr(),
+            log.data_ptr() if log is not None else None, ctypes.c_void_p(stream.cuda_stream),
+            int(bool(check_overflow)))
+        self._check(rc, "brkga_evaluate_device_noalloc")
         return out
 
     def stats(self):

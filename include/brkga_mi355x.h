@@ -149,6 +149,23 @@ int brkga_evaluate_device(brkga_handle *h, const void *d_keys, int key_dtype, in
                           int64_t n_genes, double *d_out_makespan, int32_t *d_opt_log,
                           void *stream, int check_overflow);
 
+/* Allocation-free variant of brkga_evaluate_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results; it calls no allocating routine.  Its per-call
+ * buffers live in one static device array per GPU (about 1 MiB, held until
+ * the process ends) shared by every handle on that GPU, so calls that overlap
+ * on one GPU race: issue them one at a time (stream-ordered).  Limits:
+ * n_chrom * n_machines <= BRKGA_NOALLOC_MAX_CONTEXTS, else BRKGA_EINVAL; the
+ * handle's bin workspace (allocated by brkga_create) is not grown, so create
+ * the handle with a max_bins large enough (n_parts always is) -- with
+ * check_overflow a context that ran out of bins gives BRKGA_EOVERFLOW, without
+ * it a NaN makespan.  Decode time is not added to brkga_stats.kernel_ms. */
+#define BRKGA_NOALLOC_MAX_CONTEXTS 65536
+int brkga_evaluate_device_noalloc(brkga_handle *h, const void *d_keys, int key_dtype, int64_t n_chrom,
+                                  int64_t n_genes, double *d_out_makespan, int32_t *d_opt_log,
+                                  void *stream, int check_overflow);
+
 int brkga_get_stats(const brkga_handle *h, brkga_stats *out);
 int brkga_reset_stats(brkga_handle *h);
 
