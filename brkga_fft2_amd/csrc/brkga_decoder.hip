@@ -62,8 +62,6 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 // them are ds_read instructions (with in-order completion the compiler can
 // wait for precisely the ones it needs) rather than flat loads.
 #define LDSP __attribute__((address_space(3)))
-typedef LDSP unsigned char lds_u8;
-typedef LDSP const unsigned char lds_cu8;
 typedef LDSP const unsigned lds_cu32;
 typedef LDSP unsigned lds_u32;
 typedef LDSP const v4i lds_cv4i;
@@ -112,15 +110,15 @@ struct DevProblem {
 };
 
 struct DevWork {
-    unsigned char *grids;    // [n_wg][max_bins][hmax][pitch]
+    unsigned *grids;         // [n_wg][max_bins][hmax][pw] occupancy bit rows
     int *vac;                // [n_wg][max_bins][hmax]
     unsigned int *queue;
     unsigned long long *counters;
-    int pitch, hmax, max_bins;
+    int pw, hmax, max_bins;  // pw: 32-bit words per grid row (a multiple of 4)
     int hrot;                // max rotation height (LDS density rows)
     int q_off;               // LDS offset of the staged mask bits
     int vc_bins;             // bins whose vacancy vector is cached in LDS
-    int band_off, band_bytes;  // LDS band for staged grid rows
+    int band_off, band_words;  // LDS band for staged grid bit rows (32-bit words)
     int zw_off;                // LDS zero masks: [2 passes][4 waves][8 tiles * 16] u64
     int bins_off;              // LDS bin bookkeeping: area, pt, pth [n_parts] f64, minrow [n_parts] i32
 };
@@ -168,26 +166,35 @@ struct DivBy {
 // `val`.  The compiler exposes it as an intrinsic only, bound here by name.
 extern "C" __device__ int lane_write(int val, int lane, int old) __asm("llvm.amdgcn.writelane.i32");
 
-// B fragment of the 32x32x32 MFMA for Toeplitz chunk c of one mask row,
-// built in registers from the row's bits.  Q is the mask row padded with 32
-// zero bits in front (Q[i] = P[a][i-32]); lane (n = lane&31, g = lane>>5)
-// needs the 16 bytes B[16g + j][n] = P[a][32c + 16g + j - n] = Q[32c + shift + j]
-// with shift = 32 + 16g - n in [1, 48], i.e. 16 bits of the 64-bit word
-// (Q dword c+1 : Q dword c) shifted right by `shift`.  Each bit becomes one
-// byte: v_perm replicates the byte holding four of the bits into all four
-// bytes and the AND keeps bit j in byte j, so a set bit becomes 1, 2, 4 or 8.
-// Only "count == 0" is ever tested and every product is non-negative (grid
-// bytes are 0 or 1), so any positive value for a set mask cell gives the same
-// decisions; counts stay below 8 * 200 * 200, far from int32 overflow.
-__device__ __forceinline__ v4i bfrag_from_bits(unsigned lo, unsigned hi, int shift) {
-    const unsigned long long w = ((unsigned long long)hi << 32) | lo;
-    const unsigned b = (unsigned)(w >> shift), b4 = b >> 4;
+// MFMA operand fragments from occupancy bits.  Grids, bands and mask rows are
+// bit rows (bit i of 32-bit word q = cell 32q + i, 1 = occupied / mask cell).
+// The 32x32x32 int8 MFMA sums A[m][k] * B[k][n] over k; lane (n or m = lane&31,
+// g = lane>>5) supplies the 16 k-values of group g as four VGPRs of 4 bytes.
+// Any bijection between (g, VGPR r, byte j) and the 32 columns k of a chunk
+// gives the same sums as long as A and B use the same one; this build uses
+// k = 8j + 4g + r, so VGPR r of a lane is its 32-bit word shifted right by 4g,
+// masked to bit r of every byte: a set cell becomes the byte 2^r (1, 2, 4 or
+// 8), two ANDs per byte lane fewer than a per-byte expansion.  Only
+// "count == 0" is ever tested and every product is non-negative, so any
+// positive value per set cell gives the same decisions; counts stay below 64 *
+// 4096 * 4096 < 2^31.
+__device__ __forceinline__ v4i frag_from_word(unsigned y) {   // y: the lane's word >> 4g
     v4i r;
-    r[0] = (int)(__builtin_amdgcn_perm(b, b, 0u) & 0x08040201u);
-    r[1] = (int)(__builtin_amdgcn_perm(b4, b4, 0u) & 0x08040201u);
-    r[2] = (int)(__builtin_amdgcn_perm(b, b, 0x01010101u) & 0x08040201u);
-    r[3] = (int)(__builtin_amdgcn_perm(b4, b4, 0x01010101u) & 0x08040201u);
+    r[0] = (int)(y & 0x01010101u);
+    r[1] = (int)(y & 0x02020202u);
+    r[2] = (int)(y & 0x04040404u);
+    r[3] = (int)(y & 0x08080808u);
     return r;
+}
+
+// B fragment for Toeplitz chunk c of one mask row: B[k][n] = P[a][32c + k - n].
+// Q is the mask row with 32 zero bits in front (Q[i] = P[a][i - 32]), held as
+// words qv; lane n needs Q bits 32c + 32 - n + k, i.e. the 64-bit word (Q dword
+// c+1 : Q dword c) shifted right by 32 - n, then by 4g for its k-group: one
+// 64-bit shift by bs = 32 - n + 4g in [1, 36] (bits 28-31 of the result, which
+// the shift may leave empty, are never used).
+__device__ __forceinline__ v4i bfrag_from_bits(unsigned lo, unsigned hi, int bs) {
+    return frag_from_word((unsigned)((((unsigned long long)hi << 32) | lo) >> bs));
 }
 
 // Correlation pass of one wave over mask rows [a0, a1) for TP x-tiles of 32
@@ -222,51 +229,54 @@ __device__ __forceinline__ void store_zero_masks(const v16i (&acc)[TP], int tcnt
 }
 
 template <int TP, int NC>
-__device__ __forceinline__ void corr_pass(lds_cu8 *band, int stride, int loff, int a0,
+__device__ __forceinline__ void corr_pass(lds_cu32 *band, int stride, int loff, int a0,
                                           int a1, lds_cu32 *qbits, int tloc, int tcnt,
                                           lds_u64 *zw) {
     constexpr int NA = TP + NC - 1;
     const int lane = threadIdx.x & 63;
-    const int shift = 32 + ((lane >> 5) << 4) - (lane & 31);
+    const int ks = (lane >> 5) << 2;          // the lane's k-group shift, 4g
+    const int bs = 32 - (lane & 31) + ks;     // B window shift
     v16i acc[TP];
 #pragma unroll
     for (int t = 0; t < TP; t++) acc[t] = (v16i){};
-    lds_cu8 *base = band + loff + (tloc << 5) + ((lane >> 5) << 4);
+    // the lane's band row (its candidate's row + mask row a) starts at base + a *
+    // stride; A fragment j is word j of it (tile t, chunk c: j = t + c)
+    lds_cu32 *base = band + loff + tloc;
     if (a0 < a1) {
-        // A fragments in order j = 0..NA-1 per mask row, each feeding the
-        // pairs (t = j - c, c); fragment j + D is loaded while j is used (the
-        // last D of a row load the next row's first D), so only ~D+1
-        // fragments are live instead of two full rows.
+        // A words in order j = 0..NA-1 per mask row, each feeding the pairs
+        // (t = j - c, c); word j + D is loaded while j is used (the last D of a
+        // row load the next row's first D)
         constexpr int D = NA < 2 ? NA : 2;
-        v4i N[D];
+        unsigned N[D];
         unsigned qv[NC + 1];
-        lds_cu8 *r0 = base + a0 * stride;
+        lds_cu32 *r0 = base + a0 * stride;
 #pragma unroll
-        for (int j = 0; j < D; j++) N[j] = *(lds_cv4i *)(r0 + (j << 5));
+        for (int j = 0; j < D; j++) N[j] = r0[j];
 #pragma unroll
         for (int c = 0; c <= NC; c++) qv[c] = qbits[a0 * (NC + 1) + c];
         for (int a = a0; a < a1; a++) {
             v4i B[NC];
 #pragma unroll
-            for (int c = 0; c < NC; c++) B[c] = bfrag_from_bits(qv[c], qv[c + 1], shift);
+            for (int c = 0; c < NC; c++) B[c] = bfrag_from_bits(qv[c], qv[c + 1], bs);
             const int an = a + 1 < a1 ? a + 1 : a;
-            lds_cu8 *rc = base + a * stride;
-            lds_cu8 *rn = base + an * stride;
+            lds_cu32 *rc = base + a * stride;
+            lds_cu32 *rn = base + an * stride;
 #pragma unroll
             for (int c = 0; c <= NC; c++) qv[c] = qbits[an * (NC + 1) + c];
-            v4i A[NA];
+            unsigned Aw[NA];
 #pragma unroll
-            for (int j = 0; j < D; j++) A[j] = N[j];
+            for (int j = 0; j < D; j++) Aw[j] = N[j];
 #pragma unroll
             for (int j = 0; j < NA; j++) {
                 const int p = j + D;
-                if (p < NA) A[p] = *(lds_cv4i *)(rc + (p << 5));
-                else N[p - NA] = *(lds_cv4i *)(rn + ((p - NA) << 5));
+                if (p < NA) Aw[p] = rc[p];
+                else N[p - NA] = rn[p - NA];
+                const v4i A = frag_from_word(Aw[j] >> ks);
 #pragma unroll
                 for (int c = 0; c < NC; c++) {
                     const int t = j - c;
                     if (t >= 0 && t < TP)
-                        acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[j], B[c], acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, B[c], acc[t], 0, 0, 0);
                 }
             }
         }
@@ -344,7 +354,7 @@ __host__ __device__ constexpr int pass_tiles(int nc) { return nc <= 5 ? 8 : (nc 
 // enough: a wave can only write buffer p%2 again (pass p+2) after every wave
 // has passed the barrier of pass p+1, i.e. has finished decoding pass p.
 template <int TP, int NC>
-__device__ __forceinline__ int one_pass(lds_cu8 *band, int stride, const Fold &f, int a0, int a1,
+__device__ __forceinline__ int one_pass(lds_cu32 *band, int stride, const Fold &f, int a0, int a1,
                                         lds_cu32 *qbits, int t0, int xbase, int xmax, int nrows,
                                         lds_u64 *s_zw, int &parity) {
     lds_u64 *zp = s_zw + parity * ZW_PASS;
@@ -360,15 +370,15 @@ __device__ __forceinline__ int one_pass(lds_cu8 *band, int stride, const Fold &f
 // remainder of 6 or 7 tiles takes one 8-tile pass and a remainder of 3 one
 // 4-tile pass (one or two tiles of MFMAs cost less than the extra pass's
 // zero-mask exchange and barrier).  Tiles past the staged window read
-// whatever the band holds there (at most BAND_SLACK bytes past the band's
-// last row stay inside the allocation); the caller's xmax ends at the
-// window's last column, so their counts are never decoded.  Returns the
+// whatever the band row holds there (at most two words past the staged ones,
+// inside the row's stride); the caller's xmax ends at the window's last
+// column, so their counts are never decoded.  Returns the
 // window's minimum key; with `early`, stops once candidate 0 has a zero left
 // of every tile still to come (a key below xbase + 32 * t0; in a folded group
 // the later passes of segment 0 come before the current ones of the other
 // segments).
 template <int NC>
-__device__ __attribute__((always_inline)) int window_passes(lds_cu8 *band, int stride, const Fold &f, int h,
+__device__ __attribute__((always_inline)) int window_passes(lds_cu32 *band, int stride, const Fold &f, int h,
                              lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
                              lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
     constexpr int TPM = pass_tiles(NC);
@@ -400,23 +410,23 @@ __device__ __attribute__((always_inline)) int window_passes(lds_cu8 *band, int s
 }
 
 // Generic chunk count (NC above the compiled set): one tile per pass, runtime
-// chunk loop, one LDS fragment per MFMA.
-__device__ int window_passes_generic(lds_cu8 *band, int stride, const Fold &f, int h, int nc,
+// chunk loop, one band word per MFMA.
+__device__ int window_passes_generic(lds_cu32 *band, int stride, const Fold &f, int h, int nc,
                                      lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
                                      lds_u64 *s_zw, unsigned long long &mfma, bool early) {
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int shift = 32 + ((lane >> 5) << 4) - (lane & 31);
+    const int ks = (lane >> 5) << 2, bs = 32 - (lane & 31) + ks;
     const int a0 = f.skip + (((h - f.skip) * wave) >> 2), a1 = f.skip + (((h - f.skip) * (wave + 1)) >> 2);
     int key = KEY_NONE, parity = 0;
     for (int t0 = 0; t0 < tl; t0++) {
         v16i acc[1] = {(v16i){}};
-        lds_cu8 *base = band + f.loff + (t0 << 5) + ((lane >> 5) << 4);
+        lds_cu32 *base = band + f.loff + t0;
         for (int a = a0; a < a1; a++) {
-            lds_cu8 *rp = base + a * stride;
+            lds_cu32 *rp = base + a * stride;
             lds_cu32 *q = qbits + a * (nc + 1);
             for (int c = 0; c < nc; c++)
-                acc[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(*(lds_cv4i *)(rp + (c << 5)),
-                                                               bfrag_from_bits(q[c], q[c + 1], shift), acc[0], 0, 0, 0);
+                acc[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(frag_from_word(rp[c] >> ks),
+                                                               bfrag_from_bits(q[c], q[c + 1], bs), acc[0], 0, 0, 0);
         }
         mfma += (unsigned long long)(a1 - a0) * nc;
         lds_u64 *zp = s_zw + parity * ZW_PASS;
@@ -430,7 +440,7 @@ __device__ int window_passes_generic(lds_cu8 *band, int stride, const Fold &f, i
     return key;
 }
 
-__device__ __forceinline__ int window_dispatch(int nc, lds_cu8 *band, int stride, const Fold &f, int h,
+__device__ __forceinline__ int window_dispatch(int nc, lds_cu32 *band, int stride, const Fold &f, int h,
                                lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
                                lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
 #define CASE(K) case K: return window_passes<K>(band, stride, f, h, qbits, tl, xbase, xmax, nrows, s_zw, mfma, tpcap, early);
@@ -442,25 +452,28 @@ __device__ __forceinline__ int window_dispatch(int nc, lds_cu8 *band, int stride
 #undef CASE
 }
 
-// LDS row stride of a window of tw x-tiles: the staged width rounded to 128
-// bytes plus 16, so the 32 rows of an A fragment fall into distinct banks.
+// LDS row stride (32-bit words) of a window of tw x-tiles.  A window of tiles
+// [tw0, tw0 + tw) reads the words [tw0, tw0 + tw + nc - 1) of its rows; they are
+// staged as the 16-byte-aligned words from tw0 rounded down (up to 3 words in
+// front) to the end rounded up, and a pass rounded up past the window reads up
+// to two words further: tw + nc + 5 words cover all of it.  The stride is odd,
+// so the 32 band rows one A-fragment load touches fall into 32 distinct banks.
 __host__ __device__ __forceinline__ int band_stride(int tw, int nc) {
-    return ((((tw + nc - 1) << 5) + 127) & ~127) + 16;
+    return (tw + nc + 5) | 1;
 }
 
-// Bytes kept free after the band's last row: a pass rounded up past its
-// window reads at most two tiles (64 bytes) beyond a row's staged columns.
-#define BAND_SLACK 64
+// Words kept free after the band's last row.
+#define BAND_SLACK 16
 
 // Largest x-window (in tiles, <= cap) whose 32-row band fits (with
-// BAND_SLACK bytes to spare); shrinks the window rather than the row group.
-__host__ __device__ __forceinline__ int band_window(int T, int nc, int h, int band_bytes, int cap,
+// BAND_SLACK words to spare); shrinks the window rather than the row group.
+__host__ __device__ __forceinline__ int band_window(int T, int nc, int h, int band_words, int cap,
                                                     int *stride_out, int *rcap_out) {
     int tw = T < cap ? T : cap;
     int st, rc;
     for (;;) {
         st = band_stride(tw, nc);
-        rc = (band_bytes - BAND_SLACK) / st;
+        rc = (band_words - BAND_SLACK) / st;
         if (rc >= h + GROUP_ROWS - 1 || tw == 1) break;
         tw = (tw + 1) >> 1;
     }
@@ -493,17 +506,18 @@ __device__ __forceinline__ void wg_copy(int n, Load load, Store store) {
 }
 
 // Collision test of one rotation against one bin, run by the whole workgroup.
-// cand: candidate top rows, descending.  Groups of up to 32 candidates are
-// staged into the LDS band (rows ylo..ytop+h-1, one x-window at a time) and
-// correlated.  Returns (candidate index)*XKEY + x of the bottom-most, then
-// left-most, placement with zero overlap, or KEY_NONE.  Uniform across the
-// workgroup.  Grid rows above r0 are empty (no part reaches above the bin's
-// top-most placement), so mask rows that only meet them are neither staged
-// nor correlated, except in the dense mode.
+// grid: the bin's bit rows (gpw words each).  cand: candidate top rows,
+// descending.  Groups of up to 32 candidates are staged into the LDS band (bit
+// rows ylo..ytop+h-1, one x-window at a time) and correlated.  Returns
+// (candidate index)*XKEY + x of the bottom-most, then left-most, placement with
+// zero overlap, or KEY_NONE.  Uniform across the workgroup.  Grid rows above r0
+// are empty (no part reaches above the bin's top-most placement), so mask rows
+// that only meet them are neither staged nor correlated, except in the dense
+// mode.
 template <bool DENSE>
-__device__ __attribute__((always_inline)) int test_rotation(const unsigned char *grid, int gpitch, int W, int h, int w, int nc,
-                             const unsigned *qbits_g, const short *cand, int ncand, int r0, lds_u8 *band,
-                             int band_bytes, lds_u32 *s_q, lds_u64 *s_zw, unsigned long long &groups,
+__device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid, int gpw, int W, int h, int w, int nc,
+                             const unsigned *qbits_g, const short *cand, int ncand, int r0, lds_u32 *band,
+                             int band_words, lds_u32 *s_q, lds_u64 *s_zw, unsigned long long &groups,
                              unsigned long long &mfma, unsigned long long *ph, int tpcap, int wcap) {
     const int tid = threadIdx.x, lane = tid & 63;
     // mask bits of the rotation -> LDS (read by every wave for every group)
@@ -511,7 +525,7 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
                       [&](int k, unsigned v) { s_q[k] = v; });
     const int T = (W - w + 1 + 31) >> 5;
     int stride, rcap;
-    const int TW = band_window(T, nc, h, band_bytes, wcap, &stride, &rcap);
+    const int TW = band_window(T, nc, h, band_words, wcap, &stride, &rcap);
     const int maxrows = min(GROUP_ROWS, rcap - h + 1);
     int found = KEY_NONE;
     for (int g0 = 0; g0 < ncand;) {
@@ -535,15 +549,20 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
         for (int tw0 = 0; tw0 < T && skip < h; tw0 += TW) {
             unsigned long long t0 = tid == 0 ? tick() : 0;
             const int tl = min(TW, T - tw0);
-            const int q16 = (tl + nc - 1) * 2;
-            const DivBy byq(q16);
-            const unsigned char *src = grid + (long long)(ylo + skip) * gpitch + (tw0 << 5);
-            lds_u8 *dst = band + skip * stride;
-            wg_copy((rows - skip) * q16,
-                    [&](int k) { const int r = byq(k), q = k - r * q16;
-                                 return *(const v4i *)(src + (long long)r * gpitch + (q << 4)); },
-                    [&](int k, v4i v) { const int r = byq(k), q = k - r * q16;
-                                        *(lds_v4i *)(dst + r * stride + (q << 4)) = v; });
+            // words [tw0, tw0 + tl + nc - 1) of each row, as the 16-byte vectors
+            // from tw0 rounded down (c0) to the end rounded up (<= gpw)
+            const int c0 = tw0 & ~3;
+            const int q4 = (((tw0 + tl + nc - 1 + 3) & ~3) - c0) >> 2;
+            const DivBy byq(q4);
+            const unsigned *src = grid + (long long)(ylo + skip) * gpw + c0;
+            lds_u32 *dst = band + skip * stride;
+            wg_copy((rows - skip) * q4,
+                    [&](int k) { const int r = byq(k), q = k - r * q4;
+                                 return *(const v4i *)(src + (long long)r * gpw + (q << 2)); },
+                    [&](int k, v4i v) { const int r = byq(k), q = k - r * q4;
+                                        lds_u32 *d = dst + r * stride + (q << 2);
+                                        d[0] = (unsigned)v[0]; d[1] = (unsigned)v[1];
+                                        d[2] = (unsigned)v[2]; d[3] = (unsigned)v[3]; });
             __syncthreads();
             unsigned long long t1 = tid == 0 ? tick() : 0;
             if (tid == 0) ph[2] += t1 - t0;
@@ -553,7 +572,8 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
             const int tf = (tl + (32 >> f.rlog) - 1) >> (5 - f.rlog);   // tiles per segment
             const int m = lane & 31;
             f.segx = min((m >> f.rlog) * tf, tl - tf) << 5;
-            f.loff = (cand[g0 + min(m & ((1 << f.rlog) - 1), nrows - 1)] - ylo) * stride + f.segx;
+            f.loff = (cand[g0 + min(m & ((1 << f.rlog) - 1), nrows - 1)] - ylo) * stride + (tw0 - c0) +
+                     (f.segx >> 5);
             f.skip = skip;
             const int k = window_dispatch(nc, band, stride, f, h, s_q, tf, tw0 << 5,
                                           min(W - w, ((tw0 + tl) << 5) - 1), nrows, s_zw, mfma, tpcap, !DENSE);
@@ -570,62 +590,68 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned char 
     return found;
 }
 
-// Longest free run (longest run of zero cells) of `rows` grid rows in LDS
-// (row r at band + r * rstride), numba_utils.py:161-188 computes the same
-// value sequentially per row.  Eight lanes per row, eight rows per wave
-// pass: lane j of a row scans the cells [j*S, j*S + S), S = 8 * ceil(W / 64),
-// with one 8-byte LDS read per 8 cells (cells past W count as occupied), and
-// keeps its free cells at the segment start (pre) and end (suf) and its
-// longest run (best); three ordered butterfly steps join the eight segments
-// of the row.  put(r, v) receives each row's value from one lane.
-__device__ __forceinline__ unsigned occupied4(unsigned v) {   // bit j: byte j non-zero
-    v |= v >> 4;
-    v |= v >> 2;
-    v |= v >> 1;
-    return ((v & 0x01010101u) * 0x01020408u) >> 24;   // gathers bit 8j to bit 24 + j
-}
-
-__device__ __forceinline__ int free_run4(unsigned o4) {   // longest free run of 4 cells
-    return (int)(((o4 < 8 ? 0x11122234u : 0x01121123u) >> ((o4 & 7u) << 2)) & 0xFu);
+// Longest free run (longest run of zero cells) of `rows` grid bit rows in LDS
+// (row r at band + r * rstride words); numba_utils.py:161-188 computes the same
+// value sequentially per row.  L = 2^lg lanes per row (the smallest power of
+// two >= ceil(W / 32), at most 64), lane j scanning the wpl = ceil(ceil(W/32) /
+// L) words [j*wpl, j*wpl + wpl) (cells past W count as occupied): per word the
+// free cells at its start (ctz of the occupied bits) and end (clz) and its
+// longest inner run (longest_free_in_word); the lane keeps its free cells at
+// the segment start (pre) and end (suf) and its longest run (best), and lg
+// ordered butterfly steps join the L segments of the row.  put(r, v) receives
+// each row's value from one lane.
+__device__ __forceinline__ int longest_free_in_word(unsigned o) {   // o: occupied bits
+    // R_s: bit i set iff cells i .. i+s-1 are free; R_{L+s} = R_L & (R_s >> L),
+    // so binary lifting from L = 0 finds the longest run (< 32 unless o == 0)
+    const unsigned r1 = ~o, r2 = r1 & (r1 >> 1), r4 = r2 & (r2 >> 2), r8 = r4 & (r4 >> 4), r16 = r8 & (r8 >> 8);
+    if (!o) return 32;
+    unsigned cur = r16;
+    int L = 16;
+    if (!cur) { cur = r1 | ~r1; L = 0; }
+    unsigned c = cur & (r8 >> L);
+    if (c) { cur = c; L += 8; }
+    c = cur & (r4 >> L);
+    if (c) { cur = c; L += 4; }
+    c = cur & (r2 >> L);
+    if (c) { cur = c; L += 2; }
+    c = cur & (r1 >> L);
+    if (c) L += 1;
+    return L;
 }
 
 template <typename Put>
-__device__ __forceinline__ void rows_free_run(lds_cu8 *band, int rstride, int rows, int W, Put put) {
+__device__ __forceinline__ void rows_free_run(lds_cu32 *band, int rstride, int rows, int W, Put put) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int sub = lane & 7;
-    const int S = ((W + 63) >> 6) << 3;   // cells per segment, a multiple of 8
-    const int nch = S >> 3;
-    for (int r0 = wave << 3; r0 < rows; r0 += 32) {
-        const int r = r0 + (lane >> 3);
-        lds_cu8 *rp = band + min(r, rows - 1) * rstride;
-        const int x0 = sub * S;
+    const int nw = (W + 31) >> 5;   // words of a row
+    int lg = 0;
+    while ((1 << lg) < nw && lg < 6) lg++;
+    const int L = 1 << lg, wpl = (nw + L - 1) >> lg, rpw = 64 >> lg;
+    const int sub = lane & (L - 1);
+    for (int r0 = wave * rpw; r0 < rows; r0 += 4 * rpw) {
+        const int r = r0 + (lane >> lg);
+        lds_cu32 *rp = band + min(r, rows - 1) * rstride;
         int pre = 0, run = 0, best = 0;
         bool first = true;
-        for (int i = 0; i < nch; i++) {
-            const int x = x0 + (i << 3);
-            unsigned o = 0xFFu;   // bit t: cell x + t occupied
-            if (x < W) {
-                // x + 7 stays inside the row's 16-byte padded span
-                const unsigned long long v = *(LDSP const unsigned long long *)(rp + x);
-                o = occupied4((unsigned)v) | (occupied4((unsigned)(v >> 32)) << 4);
-                const int valid = W - x;
-                if (valid < 8) o |= 0xFFu & (0xFFu << valid);
+        for (int i = 0; i < wpl; i++) {
+            const int q = sub * wpl + i;
+            unsigned o = 0xFFFFFFFFu;   // bit t: cell 32q + t occupied
+            if (q < nw) {
+                o = rp[q];
+                const int valid = W - (q << 5);
+                if (valid < 32) o |= 0xFFFFFFFFu << valid;
             }
             const bool z = o == 0;
-            const int lead = __builtin_ctz(o | 0x100u);
-            const int trail = o ? __builtin_clz(o) - 24 : 8;
-            const unsigned lo4 = o & 0xFu, hi4 = o >> 4;
-            const int across = (lo4 ? __builtin_clz(lo4) - 28 : 4) + __builtin_ctz(hi4 | 0x10u);
-            const int inner = max(max(free_run4(lo4), free_run4(hi4)), across);
+            const int lead = z ? 32 : __builtin_ctz(o);
+            const int trail = z ? 32 : __builtin_clz(o);
+            const int inner = longest_free_in_word(o);
             if (first && !z) pre = run + lead;
             if (!z) best = max(best, max(run + lead, inner));
-            run = z ? run + 8 : trail;
+            run = z ? run + 32 : trail;
             first = first && z;
         }
-        if (first) pre = run;   // all S cells free: pre = suf = S
-        int suf = run, len = S;
-#pragma unroll
-        for (int d = 1; d < 8; d <<= 1) {   // join segments [.. d) and [d .. 2d)
+        if (first) pre = run;   // every cell of the segment free: pre = suf = its length
+        int suf = run, len = wpl << 5;
+        for (int d = 1; d < L; d <<= 1) {   // join segments [.. d) and [d .. 2d)
             const int ps = __shfl_xor(pre | (suf << 16), d, 64);
             const int pb = __shfl_xor(best, d, 64);
             const int opre = ps & 0xFFFF, osuf = ps >> 16;
@@ -864,8 +890,8 @@ __device__ __forceinline__ int winning_prefix(const short *cand, int ncand, int 
 // its top-left (by, bx); uniform across the workgroup.
 template <bool DENSE>
 __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork &Wk, int mach, int rb, int nr, int rf,
-                                              int H, int W, int enc, const unsigned char *grid,
-                                              const short *s_cand, const int *s_ncand, int hmax, lds_u8 *band,
+                                              int H, int W, int enc, const unsigned *grid,
+                                              const short *s_cand, const int *s_ncand, int hmax, lds_u32 *band,
                                               lds_u32 *s_q, lds_u64 *s_zw, unsigned long long &c_tests,
                                               unsigned long long &c_algo, unsigned long long &c_groups,
                                               unsigned long long &c_mfma, unsigned long long *ph, int &by, int &bx) {
@@ -885,9 +911,9 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
         c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
         const int ncr = P.rot_nc[rg];
         const unsigned char *tc = P.tiles + ((long long)mach * TUNE_NC + min(ncr, TUNE_NC - 1)) * 2;
-        const int key = test_rotation<DENSE>(grid, Wk.pitch, W, h, w, ncr,
+        const int key = test_rotation<DENSE>(grid, Wk.pw, W, h, w, ncr,
                                              P.qbits + P.qbits_off[rg], cand, ncand, H - enc,
-                                             band, Wk.band_bytes, s_q, s_zw, c_groups, c_mfma, ph,
+                                             band, Wk.band_words, s_q, s_zw, c_groups, c_mfma, ph,
                                              uni(tc[0]), uni(tc[1]));
         if (key == KEY_NONE) continue;
         // density (new enclosure length) -> row -> col (binClassNew.py:101-117)
@@ -903,61 +929,58 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
 
 // Stamp rotation rg at top-left (py, px) into a bin (binClassNew.py:155-160,
 // cuda_batch_update.py:90-126), whole workgroup: rows [py, py+h) pass through
-// the LDS band in chunks -- the grid rows (a fresh bin's rows are zero) and
-// the mask bits come in one batched copy, each 4-byte band word that the
-// part covers takes its four mask bits as bytes, the rows go back to HBM,
-// and their free runs go to vv (HBM) and to vcache when it is not null.
-__device__ __forceinline__ void stamp_part(const DevProblem &P, const DevWork &Wk, int rg, int W, unsigned char *grid,
-                                           int *vv, short *vcache, bool fresh, int py, int px, lds_u8 *band,
+// the LDS band in chunks -- the grid's bit rows (a fresh bin's rows are zero)
+// and the mask bits come in one batched copy, every grid word the part covers
+// takes the 32 mask bits that fall on it, those words go back to HBM, and the
+// rows' free runs go to vv (HBM) and to vcache when it is not null.
+__device__ __forceinline__ void stamp_part(const DevProblem &P, const DevWork &Wk, int rg, int W, unsigned *grid,
+                                           int *vv, short *vcache, bool fresh, int py, int px, lds_u32 *band,
                                            lds_u32 *s_q) {
     const int tid = threadIdx.x;
     const int h = P.rot_h[rg], w = P.rot_w[rg];
-    const int wq = (W + 15) >> 4;               // 16-byte columns per row
-    const int rb16 = wq << 4;                   // band row bytes
-    const DivBy bywq(wq);
-    const int chunk = min(h, Wk.band_bytes / rb16);
+    const int pw = Wk.pw, q4 = pw >> 2;         // words and 16-byte vectors per row
+    const DivBy byq4(q4);
+    const int chunk = min(h, Wk.band_words / pw);
     const int nq = P.rot_nc[rg] + 1;            // mask bit words per row
-    const int q0 = px >> 2, nw = ((px + w - 1) >> 2) - q0 + 1;   // band words per row
+    const int q0 = px >> 5, nw = ((px + w - 1) >> 5) - q0 + 1;   // grid words per row the part covers
     const DivBy bynw(nw);
     for (int a0 = 0; a0 < h; a0 += chunk) {
         const int rows = min(chunk, h - a0);
-        const unsigned char *g0 = grid + (long long)(py + a0) * Wk.pitch;
+        const unsigned *g0 = grid + (long long)(py + a0) * pw;
         const unsigned *qg = P.qbits + P.qbits_off[rg] + (long long)a0 * nq;
         if (fresh)
-            for (int k = tid; k < rows * wq; k += WG_THREADS) *(lds_v4i *)(band + (k << 4)) = (v4i){0, 0, 0, 0};
-        const int ng = fresh ? 0 : rows * wq;
+            for (int k = tid; k < rows * q4; k += WG_THREADS) *(lds_v4i *)(band + (k << 2)) = (v4i){0, 0, 0, 0};
+        const int ng = fresh ? 0 : rows * q4;
         wg_copy(ng + rows * nq,
                 [&](int k) -> v4i {
                     if (k < ng) {
-                        const int r = bywq(k), q = k - r * wq;
-                        return *(const v4i *)(g0 + (long long)r * Wk.pitch + (q << 4));
+                        const int r = byq4(k), q = k - r * q4;
+                        return *(const v4i *)(g0 + (long long)r * pw + (q << 2));
                     }
                     return (v4i){(int)qg[k - ng], 0, 0, 0};
                 },
                 [&](int k, v4i v) {
-                    if (k < ng) *(lds_v4i *)(band + (k << 4)) = v;
+                    if (k < ng) *(lds_v4i *)(band + (k << 2)) = v;
                     else s_q[k - ng] = (unsigned)v[0];
                 });
         __syncthreads();
         // mask row bits Q[i] = P[a][i - 32] (32 zero bits in front, zero past w):
-        // the word at band bytes 4q..4q+3 takes bits 4q - px + 32 .. + 3
+        // grid word q holds cells 32q .. 32q+31, i.e. mask columns 32q - px + t,
+        // Q bits from off = 32q - px + 32 >= 1 on
         for (int k = tid; k < rows * nw; k += WG_THREADS) {
             const int r = bynw(k), q = q0 + (k - r * nw);
-            const int i = (q << 2) - px + 32;   // >= 29
-            const int wi = i >> 5;
+            const int off = (q << 5) - px + 32, wi = off >> 5;
             const unsigned lo = s_q[r * nq + wi];
             const unsigned hi = wi + 1 < nq ? s_q[r * nq + wi + 1] : 0u;
-            const unsigned b4 = (unsigned)((((unsigned long long)hi << 32) | lo) >> (i & 31)) & 0xFu;
-            // bit t -> byte t (the four shifted copies never overlap)
-            if (b4) *(lds_u32 *)(band + r * rb16 + (q << 2)) |= (b4 * 0x00204081u) & 0x01010101u;
+            const unsigned bits = (unsigned)((((unsigned long long)hi << 32) | lo) >> (off & 31));
+            if (bits) band[r * pw + q] |= bits;
         }
         __syncthreads();
-        for (int k = tid; k < rows * wq; k += WG_THREADS) {
-            const int r = bywq(k), q = k - r * wq;
-            *(v4i *)(grid + (long long)(py + a0 + r) * Wk.pitch + (q << 4)) =
-                *(lds_cv4i *)(band + ((r * wq + q) << 4));
+        for (int k = tid; k < rows * nw; k += WG_THREADS) {
+            const int r = bynw(k), q = q0 + (k - r * nw);
+            grid[(long long)(py + a0 + r) * pw + q] = band[r * pw + q];
         }
-        rows_free_run(band, rb16, rows, W, [&](int r, int v) {
+        rows_free_run(band, pw, rows, W, [&](int r, int v) {
             vv[py + a0 + r] = v;
             if (vcache) vcache[py + a0 + r] = (short)v;
         });
@@ -980,7 +1003,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
     short *s_vac = s_cand + MAX_ROT * hmax;                 // [hmax] vacancy of an uncached bin
     short *s_vcache = s_vac + hmax;                         // [vc_bins][hmax] vacancy of bins < vc_bins
     lds_u32 *s_q = (lds_u32 *)(smem + Wk.q_off);            // [hrot * (max nc + 1)]
-    lds_u8 *band = (lds_u8 *)(smem + Wk.band_off);          // [band_bytes]
+    lds_u32 *band = (lds_u32 *)(smem + Wk.band_off);        // [band_words]
     lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);          // [2][4][ZW_WAVE] zero masks
     __shared__ int s_ctx, s_cnt, s_off, s_infeasible, s_nb;
     __shared__ int s_ncand[MAX_ROT];
@@ -989,7 +1012,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
     const int wave = tid >> 6;
     const int lane = tid & 63;
     const long long wg = blockIdx.x;
-    unsigned char *grids = Wk.grids + wg * (long long)Wk.max_bins * hmax * Wk.pitch;
+    unsigned *grids = Wk.grids + wg * (long long)Wk.max_bins * hmax * Wk.pw;
     int *vacs = Wk.vac + wg * (long long)Wk.max_bins * hmax;
     // bin bookkeeping of the running context (a context opens at most n bins)
     LDSP double *b_area = (LDSP double *)(smem + Wk.bins_off);
@@ -1072,7 +1095,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
             bool fresh = false;
             for (int b = 0; b < nb && !placed; b++) {
                 if (b_area[b] + parea > plate_area) continue;   // placement.py:57-59
-                const unsigned char *grid = grids + (long long)b * hmax * Wk.pitch;
+                const unsigned *grid = grids + (long long)b * hmax * Wk.pw;
                 if (tid == 0) tp = tick();
                 const short *vrow = s_vcache + b * hmax;
                 if (b >= Wk.vc_bins) {   // vacancy of bins beyond the LDS cache comes from HBM
@@ -1103,8 +1126,8 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                 py = H - P.rot_h[rb + prot]; px = 0;
                 fresh = true;
                 // fresh bin: zero grid, full vacancy (full_native_decoder.py:2087-2089)
-                v4i *g4 = (v4i *)(grids + (long long)pb * hmax * Wk.pitch);
-                const int nvec = H * Wk.pitch / 16;
+                v4i *g4 = (v4i *)(grids + (long long)pb * hmax * Wk.pw);
+                const int nvec = H * Wk.pw / 4;
                 for (int k = tid; k < nvec; k += WG_THREADS) g4[k] = (v4i){0, 0, 0, 0};
                 int *vv = vacs + (long long)pb * hmax;
                 for (int k = tid; k < H; k += WG_THREADS) vv[k] = W;
@@ -1122,7 +1145,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
             {
                 const int rg = rb + prot;
                 const int h = P.rot_h[rg];
-                stamp_part(P, Wk, rg, W, grids + (long long)pb * hmax * Wk.pitch, vacs + (long long)pb * hmax,
+                stamp_part(P, Wk, rg, W, grids + (long long)pb * hmax * Wk.pw, vacs + (long long)pb * hmax,
                            pb < Wk.vc_bins ? s_vcache + pb * hmax : nullptr, fresh, py, px, band, s_q);
                 if (tid == 0) {   // binClassNew.py:147-153,172-174 ; placement.py:81-82
                     const long long mp = (long long)mach * P.n_types + ty;
@@ -1186,7 +1209,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
 // chosen placement.  Batches live in [machine][batch] slots of grids/vac.
 // ---------------------------------------------------------------------------
 struct GreedyArgs {
-    unsigned char *grids;   // [M][max_batches][hmax][pitch]
+    unsigned *grids;        // [M][max_batches][hmax][pw] bit rows
     int *vac;               // [M][max_batches][hmax]
     int max_batches;
     int ty;                 // part type under test
@@ -1204,14 +1227,14 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P
     short *s_cand = (short *)(s_den + MAX_ROT * Wk.hrot);
     short *s_vac = s_cand + MAX_ROT * hmax;
     lds_u32 *s_q = (lds_u32 *)(smem + Wk.q_off);
-    lds_u8 *band = (lds_u8 *)(smem + Wk.band_off);
+    lds_u32 *band = (lds_u32 *)(smem + Wk.band_off);
     lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);
     __shared__ int s_ncand[MAX_ROT];
     const int *job = A.jobs + 3 * blockIdx.x;
     const int m = job[0], b = job[1], enc = job[2], ty = A.ty;
     const int H = P.plate_h[m], W = P.plate_w[m], nr = P.nrot[ty], rb = P.rot_base[ty];
     const long long slot = (long long)m * A.max_batches + b;
-    const unsigned char *grid = A.grids + slot * hmax * Wk.pitch;
+    const unsigned *grid = A.grids + slot * hmax * Wk.pw;
     const int *vac = A.vac + slot * hmax;
     load_densities(P, rb, nr, s_den);
     for (int y = tid; y < H; y += WG_THREADS) s_vac[y] = (short)vac[y];
@@ -1232,7 +1255,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P
 }
 
 struct GreedyInsert {
-    unsigned char *grid;
+    unsigned *grid;
     int *vac;
     int rg, H, W, py, px, fresh;
 };
@@ -1242,10 +1265,10 @@ struct GreedyInsert {
 __global__ __launch_bounds__(WG_THREADS, 2) void greedy_insert_kernel(DevProblem P, DevWork Wk, GreedyInsert I) {
     extern __shared__ __align__(16) unsigned char smem[];
     lds_u32 *s_q = (lds_u32 *)(smem + Wk.q_off);
-    lds_u8 *band = (lds_u8 *)(smem + Wk.band_off);
+    lds_u32 *band = (lds_u32 *)(smem + Wk.band_off);
     if (I.fresh) {
         v4i *g4 = (v4i *)I.grid;
-        const int nvec = I.H * Wk.pitch / 16;
+        const int nvec = I.H * Wk.pw / 4;
         for (int k = threadIdx.x; k < nvec; k += WG_THREADS) g4[k] = (v4i){0, 0, 0, 0};
         for (int k = threadIdx.x; k < I.H; k += WG_THREADS) I.vac[k] = I.W;
         __syncthreads();
@@ -1407,8 +1430,8 @@ static int device_workspace(brkga_handle *h, int max_bins) {
     h->ws_gen++;
     void *p;
     int rc;
-    if ((rc = alloc_dev(h, nwg * max_bins * (size_t)w.hmax * w.pitch, &p))) return rc;
-    w.grids = (unsigned char *)p;
+    if ((rc = alloc_dev(h, nwg * max_bins * (size_t)w.hmax * w.pw * sizeof(unsigned), &p))) return rc;
+    w.grids = (unsigned *)p;
     if ((rc = alloc_dev(h, nwg * max_bins * (size_t)w.hmax * sizeof(int), &p))) return rc;
     w.vac = (int *)p;
     return 0;
@@ -1454,7 +1477,7 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     std::vector<int> dens, nc(R);
     std::vector<long long> dens_off(R), qbits_off(R);
     std::vector<unsigned> qbits;
-    int hmax = 0, wmax = 0, pitch = 64;
+    int hmax = 0, wmax = 0, pw = 4;   // pw: 32-bit words per grid bit row
     for (int m = 0; m < M; m++) { hmax = std::max(hmax, p->plate_h[m]); wmax = std::max(wmax, p->plate_w[m]); }
     for (int r = 0; r < R; r++) {
         const int rh = p->rot_h[r], rw = p->rot_w[r];
@@ -1477,11 +1500,11 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
             const int H = p->plate_h[m], W = p->plate_w[m];
             if (rh > H || rw > W) continue;
             const int Tt = (W - rw + 1 + 31) / 32;
-            pitch = std::max(pitch, 32 * (Tt + nc[r] - 1));
+            pw = std::max(pw, Tt + nc[r] - 1);   // the widest span a staged window reads
         }
     }
-    pitch = std::max(pitch, wmax);
-    pitch = (pitch + 63) & ~63;
+    pw = std::max(pw, (wmax + 31) / 32);
+    pw = (pw + 3) & ~3;   // whole 16-byte vectors; the words past W stay zero
     std::vector<unsigned char> fits((size_t)M * T);
     for (int m = 0; m < M; m++)
         for (int t = 0; t < T; t++) {
@@ -1550,7 +1573,7 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     h->prob.thresholds = nullptr;   // host pointers are not kept
     DevWork &w = h->dw;
     w.hmax = hmax;
-    w.pitch = pitch;
+    w.pw = pw;
     void *q;
     if ((rc = alloc_dev(h, 64, &q))) return bail(rc);
     w.queue = (unsigned int *)q;
@@ -1576,28 +1599,21 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     fixed += (size_t)2 * ZW_PASS * sizeof(unsigned long long);
     w.bins_off = (int)fixed;
     fixed += ((size_t)p->n_parts * (3 * sizeof(double) + sizeof(int)) + 15) & ~(size_t)15;
-    size_t need_full = 0, need_min = 0;
+    // band sizes in bytes: the smallest that works (one row group of one tile,
+    // and one grid row for the stamp) and one holding a 32-row group of every
+    // rotation with at least a 4-tile window (windows shrink at run time when
+    // a row is wider)
+    size_t need_min = ((size_t)pw + BAND_SLACK) * 4, need_4 = need_min;
     for (int m = 0; m < M; m++)
         for (int r = 0; r < R; r++) {
             const int H = p->plate_h[m], W = p->plate_w[m], rh = p->rot_h[r], rw = p->rot_w[r];
             if (rh > H || rw > W) continue;
             const int Tt = (W - rw + 1 + 31) / 32;
-            need_full = std::max(need_full, (size_t)(rh + GROUP_ROWS - 1) *
-                                                band_stride(std::min(Tt, WINDOW_CAP), nc[r]));
-            need_min = std::max(need_min, (size_t)rh * band_stride(1, nc[r]) + BAND_SLACK);
+            need_min = std::max(need_min, ((size_t)rh * band_stride(1, nc[r]) + BAND_SLACK) * 4);
+            need_4 = std::max(need_4, ((size_t)(rh + GROUP_ROWS - 1) * band_stride(std::min(Tt, 4), nc[r]) +
+                                       BAND_SLACK) * 4);
         }
     size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (size_t)prop.maxSharedMemoryPerMultiProcessor : 65536;
-    // Two workgroups per CU when a 32-row group of every rotation fits with at
-    // least a 4-tile window; windows shrink at run time when a row is wider.
-    size_t need_4 = 0;
-    for (int m = 0; m < M; m++)
-        for (int r = 0; r < R; r++) {
-            const int H = p->plate_h[m], W = p->plate_w[m], rh = p->rot_h[r], rw = p->rot_w[r];
-            if (rh > H || rw > W) continue;
-            const int Tt = (W - rw + 1 + 31) / 32;
-            need_4 = std::max(need_4, (size_t)(rh + GROUP_ROWS - 1) * band_stride(std::min(Tt, 4), nc[r]) + BAND_SLACK);
-        }
-    (void)need_full;
     int wpc2 = opt.wgs_per_cu > 0 ? opt.wgs_per_cu : (fixed + need_4 + LDS_RESERVE <= lds_cu / 2 ? 2 : 1);
     size_t per_wg = lds_cu / wpc2;
     if (per_wg < fixed + need_min + LDS_RESERVE) {
@@ -1608,8 +1624,8 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
         return bail(BRKGA_EINVAL);
     }
     w.band_off = (int)fixed;
-    w.band_bytes = (int)((per_wg - fixed - LDS_RESERVE) & ~(size_t)15);
-    h->smem = fixed + w.band_bytes;
+    w.band_words = (int)(((per_wg - fixed - LDS_RESERVE) & ~(size_t)15) / 4);
+    h->smem = fixed + (size_t)w.band_words * 4;
     h->n_wg = std::max(1, h->n_cu * wpc2);   // persistent grid at the occupancy the LDS allows
     if ((rc = device_workspace(h, mb))) return bail(rc);
     for (const void *kern : {(const void *)decode_kernel<false>, (const void *)decode_kernel<true>,
@@ -1918,7 +1934,7 @@ extern "C" int brkga_initial_solution(brkga_handle *h, const double *type_height
         return fail(h, BRKGA_EINVAL, "type_height, machine_keys, out_chromosome and out_makespan are required");
     const int n = h->prob.n_parts, M = h->prob.n_machines, T = h->prob.n_types, R = h->prob.n_rots;
     const int hmax = h->dw.hmax;
-    const size_t pitch = (size_t)h->dw.pitch;
+    const size_t gpw = (size_t)h->dw.pw;   // words per grid bit row
     hipStream_t s = (hipStream_t)stream;
     HIPCHK(h, hipSetDevice(h->device));
     // host copies of the tables the greedy's decisions read
@@ -1948,7 +1964,7 @@ extern "C" int brkga_initial_solution(brkga_handle *h, const double *type_height
     });
     // device batches: every machine may open up to n
     const size_t nslot = (size_t)M * n;
-    unsigned char *d_grids = nullptr;
+    unsigned *d_grids = nullptr;
     int *d_vac = nullptr, *d_jobs = nullptr, *d_res = nullptr;
     int rc = BRKGA_OK;
     auto cleanup = [&]() {
@@ -1957,7 +1973,7 @@ extern "C" int brkga_initial_solution(brkga_handle *h, const double *type_height
         if (d_jobs) (void)hipFree(d_jobs);
         if (d_res) (void)hipFree(d_res);
     };
-    if (hipMalloc((void **)&d_grids, nslot * hmax * pitch) != hipSuccess ||
+    if (hipMalloc((void **)&d_grids, nslot * hmax * gpw * sizeof(unsigned)) != hipSuccess ||
         hipMalloc((void **)&d_vac, nslot * hmax * sizeof(int)) != hipSuccess ||
         hipMalloc((void **)&d_jobs, nslot * 3 * sizeof(int)) != hipSuccess ||
         hipMalloc((void **)&d_res, nslot * 4 * sizeof(int)) != hipSuccess) {
@@ -2028,7 +2044,7 @@ extern "C" int brkga_initial_solution(brkga_handle *h, const double *type_height
         const int rg = r0 + crot, hh = rot_h[rg];
         GreedyInsert I;
         const size_t slot = (size_t)cm * n + cb;
-        I.grid = d_grids + slot * hmax * pitch; I.vac = d_vac + slot * hmax;
+        I.grid = d_grids + slot * hmax * gpw; I.vac = d_vac + slot * hmax;
         I.rg = rg; I.H = H; I.W = pw[cm]; I.py = crow - hh + 1; I.px = cx; I.fresh = kind == 1;
         hipLaunchKernelGGL(greedy_insert_kernel, dim3(1), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, I);
         HIPCHK(h, hipGetLastError());

@@ -1,0 +1,262 @@
+"""The decoder's bit-row correlation (csrc/brkga_decoder.hip): grids and LDS
+bands hold one bit per cell, MFMA operands are expanded from 32-bit words with
+the k order k = 16g + 4r + j <-> column 8j + 4g + r (lane group g, VGPR r, byte
+j) in both A and B (frag_from_word, bfrag_from_bits), windows are staged as
+16-byte-aligned word runs into rows of an odd stride (band_stride, test_rotation)
+and every lane reads its A words at loff + t0 + j (corr_pass).  This restates
+those steps in Python on random grids and masks and checks, per candidate and
+position, that "overlap count == 0" matches the brute-force overlap; it also
+restates the stamp's word extraction (stamp_part) and the bit-row free-run scan
+(rows_free_run, longest_free_in_word) against plain scans."""
+import random
+
+import numpy as np
+
+WORD = 32
+
+
+def band_stride(tw, nc):
+    return (tw + nc + 5) | 1
+
+
+def qbits_row(mask_row, nc):
+    """The mask row's bits after 32 zero bits, nc + 1 words (brkga_create)."""
+    q = [0] * (nc + 1)
+    for b, v in enumerate(mask_row):
+        if v:
+            q[(b + 32) // 32] |= 1 << ((b + 32) % 32)
+    return q
+
+
+def frag_bytes(y):
+    """frag_from_word: VGPR r byte j = bit 8j + r of y, weight 2^r."""
+    return [[((y >> (8 * j + r)) & 1) << r for j in range(4)] for r in range(4)]
+
+
+def lane_k_values(y):
+    """The lane's 16 operand values in MFMA k order (k = 4r + j within its group)."""
+    fb = frag_bytes(y)
+    return [fb[r][j] for r in range(4) for j in range(4)]
+
+
+def mfma_tile(a_words, b_words):
+    """int32 32x32x32 MFMA from the lanes' words: a_words[m][g], b_words[n][g] are
+    the 32-bit words lane (row/col, group g) expands; returns D[m][n]."""
+    D = np.zeros((32, 32), np.int64)
+    for m in range(32):
+        av = lane_k_values(a_words[m][0] >> 0) + lane_k_values(a_words[m][1])
+        for n in range(32):
+            bv = lane_k_values(b_words[n][0]) + lane_k_values(b_words[n][1])
+            D[m, n] = sum(x * y for x, y in zip(av, bv))
+    return D
+
+
+def test_fragment_order_is_a_bijection_of_the_chunk_columns():
+    cols = sorted(8 * j + 4 * g + r for g in range(2) for r in range(4) for j in range(4))
+    assert cols == list(range(32))
+
+
+def test_tile_counts_match_brute_force_overlap():
+    """One 32x32 tile: A = 32 grid rows' word (the lane's word >> 4g), B = the
+    64-bit window of the mask row's bits >> (32 - n + 4g); zero counts are the
+    zero overlaps of the placements x0 + n, for every chunk of a wide mask."""
+    rng = random.Random(3)
+    for _ in range(12):
+        w = rng.randint(1, 90)
+        nc = (w + 62) // 32
+        row = [1 if rng.random() < 0.6 else 0 for _ in range(w)]
+        row[rng.randrange(w)] = 1
+        q = qbits_row(row, nc)
+        width = 32 * (nc + 1)
+        grid = [[1 if rng.random() < rng.choice([0.02, 0.2]) else 0 for _ in range(width)] for _ in range(32)]
+        total = np.zeros((32, 32), np.int64)
+        for c in range(nc):
+            a_words, b_words = [], []
+            for m in range(32):
+                wd = sum(grid[m][32 * c + i] << i for i in range(32))
+                a_words.append((wd >> 0, wd >> 4))
+            for n in range(32):
+                win = (q[c + 1] << 32) | q[c]
+                b_words.append(tuple((win >> (32 - n + 4 * g)) & 0xFFFFFFFF for g in range(2)))
+            total += mfma_tile(a_words, b_words)
+        for m in range(32):
+            for n in range(32):
+                ov = sum(row[b] * grid[m][n + b] for b in range(w))
+                assert (total[m, n] == 0) == (ov == 0), (w, m, n)
+
+
+def test_staged_window_addressing():
+    """test_rotation's staging (words c0 = tw0 & ~3 onward, q4 vectors, odd stride)
+    and corr_pass's word address loff + t0 + t + c for folded segments and
+    rounded-up passes read exactly the grid word of column x0 + 32(t + c) of row
+    y + a, or (past the window) a word inside the same band row."""
+    rng = random.Random(7)
+    for _ in range(300):
+        W = rng.randint(40, 700)
+        w = rng.randint(1, min(W, 200))
+        nc = (w + 62) // 32
+        T = (W - w + 1 + 31) // 32
+        pw = max(4, T + nc - 1, (W + 31) // 32)
+        pw = (pw + 3) & ~3
+        TW = rng.randint(1, T)
+        stride = band_stride(TW, nc)
+        for tw0 in range(0, T, TW):
+            tl = min(TW, T - tw0)
+            c0 = tw0 & ~3
+            q4 = (((tw0 + tl + nc - 1 + 3) & ~3) - c0) >> 2
+            assert c0 + 4 * q4 <= pw and 4 * q4 <= stride
+            for rl in range(6):
+                nseg = 32 >> rl
+                if nseg > tl:
+                    continue
+                tf = (tl + nseg - 1) >> (5 - rl)
+                for seg in range(nseg):
+                    segx = min(seg * tf, tl - tf) << 5
+                    t0 = 0
+                    while t0 < tf:
+                        rem = tf - t0
+                        tp = 8 if rem >= 6 else (4 if rem >= 3 else (2 if rem >= 2 else 1))
+                        for t in range(tp):
+                            for c in range(nc):
+                                j = t + c
+                                off = (tw0 - c0) + (segx >> 5) + t0 + j   # word within the band row
+                                assert off < stride
+                                col = (tw0 << 5) + segx + 32 * (t0 + t) + 32 * c
+                                if off < 4 * q4:   # a staged word: the grid's word of that column
+                                    assert c0 + off == col >> 5 and col % 32 == 0
+                                else:              # past the staged words: only tiles past the window
+                                    assert segx + 32 * (t0 + t) >= 32 * tl
+                        t0 += tp
+
+
+def test_stamp_word_extraction():
+    """stamp_part: grid word q takes the mask bits (Q >> (32q - px + 32)) & 0xFFFFFFFF;
+    OR-ing them into the covered words stamps exactly the mask at column px."""
+    rng = random.Random(11)
+    for _ in range(500):
+        w = rng.randint(1, 200)
+        nc = (w + 62) // 32
+        nq = nc + 1
+        row = [1 if rng.random() < 0.5 else 0 for _ in range(w)]
+        W = rng.randint(w, 600)
+        px = rng.randint(0, W - w)
+        q = qbits_row(row, nc)
+        Q = sum(v << (32 * i) for i, v in enumerate(q))
+        q0, q1 = px >> 5, (px + w - 1) >> 5
+        words = {}
+        for qq in range(q0, q1 + 1):
+            off = (qq << 5) - px + 32
+            wi = off >> 5
+            assert 1 <= off and wi < nq
+            lo = q[wi]
+            hi = q[wi + 1] if wi + 1 < nq else 0
+            words[qq] = (((hi << 32) | lo) >> (off & 31)) & 0xFFFFFFFF
+            assert words[qq] == (Q >> off) & 0xFFFFFFFF
+        cells = [0] * (32 * (q1 + 1))
+        for qq, v in words.items():
+            for i in range(32):
+                cells[32 * qq + i] |= (v >> i) & 1
+        assert cells[px:px + w] == row and sum(cells) == sum(row)
+
+
+def longest_free_in_word(o):
+    """The kernel's binary lifting over runs of 2, 4, 8 and 16 free cells."""
+    M = 0xFFFFFFFF
+    r1 = ~o & M
+    r2 = r1 & (r1 >> 1)
+    r4 = r2 & (r2 >> 2)
+    r8 = r4 & (r4 >> 4)
+    r16 = r8 & (r8 >> 8)
+    if o == 0:
+        return 32
+    cur, L = r16, 16
+    if not cur:
+        cur, L = M, 0
+    for rs, s in ((r8, 8), (r4, 4), (r2, 2), (r1, 1)):
+        c = cur & (rs >> L)
+        if c:
+            cur, L = c, L + s
+    return L
+
+
+def bit_row_scan(row):
+    W = len(row)
+    nw = (W + 31) // 32
+    lg = 0
+    while (1 << lg) < nw and lg < 6:
+        lg += 1
+    L = 1 << lg
+    wpl = (nw + L - 1) >> lg
+    lanes = []
+    for sub in range(L):
+        pre = run = best = 0
+        first = True
+        for i in range(wpl):
+            q = sub * wpl + i
+            o = 0xFFFFFFFF
+            if q < nw:
+                o = sum(row[32 * q + t] << t for t in range(32) if 32 * q + t < W)
+                valid = W - 32 * q
+                if valid < 32:
+                    o |= (0xFFFFFFFF << valid) & 0xFFFFFFFF
+            z = o == 0
+            lead = 32 if z else (o & -o).bit_length() - 1
+            trail = 32 if z else 32 - o.bit_length()
+            inner = longest_free_in_word(o)
+            if first and not z:
+                pre = run + lead
+            if not z:
+                best = max(best, run + lead, inner)
+            run = run + 32 if z else trail
+            first = first and z
+        if first:
+            pre = run
+        lanes.append((pre, run, best))
+    length = 32 * wpl
+    d = 1
+    while d < L:
+        joined = []
+        for i in range(L):
+            a, b = (lanes[i], lanes[i ^ d]) if (i & d) == 0 else (lanes[i ^ d], lanes[i])
+            pre = length + b[0] if a[0] == length else a[0]
+            suf = length + a[1] if b[0] == length else b[1]
+            joined.append((pre, suf, max(a[2], b[2], a[1] + b[0])))
+        lanes = joined
+        length *= 2
+        d *= 2
+    assert len(set(lanes)) == 1
+    return max(lanes[0])
+
+
+def plain_scan(row):
+    best = cur = 0
+    for v in row:
+        cur = 0 if v else cur + 1
+        best = max(best, cur)
+    return best
+
+
+def test_longest_free_in_word():
+    rng = random.Random(5)
+    for _ in range(20000):
+        o = rng.getrandbits(32) & rng.getrandbits(32) if rng.random() < 0.5 else rng.getrandbits(32)
+        if rng.random() < 0.1:
+            o = 0 if rng.random() < 0.5 else 0xFFFFFFFF
+        bits = [(o >> t) & 1 for t in range(32)]
+        assert longest_free_in_word(o) == plain_scan(bits), hex(o)
+
+
+def test_bit_row_scan_matches_plain_scan():
+    rng = random.Random(2)
+    for _ in range(1500):
+        W = rng.choice([1, 2, 31, 32, 33, 63, 64, 65, 250, 300, 350, 400, 500, 511, 512, 513, 700, 1100,
+                        2048, 2049, 4096])
+        p = rng.random() ** 2
+        row = [1 if rng.random() < p else 0 for _ in range(W)]
+        if rng.random() < 0.3:
+            a = rng.randrange(W)
+            b = rng.randrange(a, W + 1)
+            row[a:b] = [0] * (b - a)
+        assert bit_row_scan(row) == plain_scan(row), W
+    for W in (1, 64, 300, 512, 4096):   # all free, all occupied
+        assert bit_row_scan([0] * W) == W and bit_row_scan([1] * W) == 0
