@@ -51,13 +51,16 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/brkga_mi355x.h"
 #include "brkga_internal.h"
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef float v16f __attribute__((ext_vector_type(16)));
 // Pointers into LDS carry the LDS address space explicitly, so loads through
 // them are ds_read instructions (with in-order completion the compiler can
 // wait for precisely the ones it needs) rather than flat loads.
@@ -93,11 +96,12 @@ struct DevProblem {
     const int *slot_type, *type_id;
     const double *area;
     const int *nrot, *best_rot, *rot_base;
-    const int *rot_h, *rot_w, *rot_nc;
+    const int *rot_h, *rot_w;
+    const int *rot_nc, *rot_nc4;   // Toeplitz chunks of 32 / 64 columns: ceil((w + 31) / 32), ceil((w + 31) / 64)
     const long long *mask_off, *dens_off, *qbits_off;
     const unsigned char *masks;
     const int *dens;
-    const unsigned *qbits;   // per rotation: h rows of (nc+1) dwords, mask bits after 32 zero bits
+    const unsigned *qbits;   // per rotation: h rows of 2 nc4 + 2 dwords, mask bits after 32 zero bits, zero padded
     const int *plate_h, *plate_w;
     const double *setup, *proc_time, *pth;
     const unsigned char *fits;   // [M][n_types] rotation 0 fits the plate in some orientation
@@ -197,20 +201,41 @@ __device__ __forceinline__ v4i bfrag_from_bits(unsigned lo, unsigned hi, int bs)
     return frag_from_word((unsigned)((((unsigned long long)hi << 32) | lo) >> bs));
 }
 
-// Correlation pass of one wave over mask rows [a0, a1) for TP x-tiles of 32
-// placements and the group's 32 candidate rows.  The four waves of the
-// workgroup take disjoint mask-row ranges of the same tiles: every partial
-// overlap count is a sum of non-negative products, so the total is zero iff
-// every wave's partial is zero, and the waves combine their results with an
-// AND of zero masks instead of adding counts: each wave stores its masks
-// (zw[wave][t*16 + i], bit = lane) with plain LDS stores and the decode ANDs
-// the four.  A operands come from the LDS band, B operands are built from the
-// mask bits; with NC (Toeplitz chunks) known at compile time each A fragment
-// feeds up to NC MFMAs and each B fragment feeds all TP tiles.
+// fp4 (e2m1) fragment of the 32x32x64 MFMA: a lane supplies 32 k-values, the
+// 32 cells of one band or mask word, in VGPRs 0-3 (8 nibbles each, low nibble
+// first; VGPRs 4-7 are unused by this format).  As for the int8 fragments, A
+// and B share one bijection between (VGPR v, nibble) and the word's bits: the
+// nibbles of byte j of VGPR v take bits (8j + v, 8j + v + 5) for v < 3 and
+// (8j + 3, 8j + 4) for v = 3, a set cell becoming 0.5 (low nibble, code 0001) or
+// 1.0 (high nibble, code 0010).  Products are 0, 0.25, 0.5 or 1 and a count is
+// at most h * w <= 2^24, so the float32 accumulation is exact and "count == 0"
+// is the same test as with integer counts.
+__device__ __forceinline__ v8i nib_frag(unsigned w) {
+    v8i r;
+    r[0] = (int)(w & 0x21212121u);
+    r[1] = (int)((w >> 1) & 0x21212121u);
+    r[2] = (int)((w >> 2) & 0x21212121u);
+    r[3] = (int)(((w >> 3) & 0x01010101u) | ((w << 1) & 0x20202020u));
+    r[4] = 0; r[5] = 0; r[6] = 0; r[7] = 0;
+    return r;
+}
+
+// One fp4 MFMA, both operands e2m1, block scales 2^0 (E8M0 127).
+__device__ __forceinline__ v16f mfma_f4(const v8i &a, const v8i &b, const v16f &c) {
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 4, 4, 0, 127, 0, 127);
+}
+
+// Accumulator of a 32x32 tile: int32 counts (int8 operands) or float32 (fp4).
+template <bool F4> struct TileAcc { typedef v16i T; };
+template <> struct TileAcc<true> { typedef v16f T; };
+
+// Columns of one Toeplitz chunk: 32 (int8, K = 32) or 64 (fp4, K = 64).
+__host__ __device__ constexpr int chunk_cols(bool f4) { return f4 ? 64 : 32; }
+
 // Store one wave's zero masks of `tcnt` tiles: lane i < 16 of the wave
 // collects accumulator i's ballot, then one 8-byte store per lane.
-template <int TP>
-__device__ __forceinline__ void store_zero_masks(const v16i (&acc)[TP], int tcnt, lds_u64 *zw) {
+template <int TP, typename AT>
+__device__ __forceinline__ void store_zero_masks(const AT (&acc)[TP], int tcnt, lds_u64 *zw) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int t = 0; t < TP; t++) {
@@ -228,41 +253,63 @@ __device__ __forceinline__ void store_zero_masks(const v16i (&acc)[TP], int tcnt
     }
 }
 
-template <int TP, int NC>
+// Correlation pass of one wave over mask rows [a0, a1) for TP x-tiles of 32
+// placements and the group's 32 candidate rows.  The four waves of the
+// workgroup take disjoint mask-row ranges of the same tiles: every partial
+// overlap count is a sum of non-negative products, so the total is zero iff
+// every wave's partial is zero, and the waves combine their results with an
+// AND of zero masks instead of adding counts: each wave stores its masks
+// (zw[wave][t*16 + i], bit = lane) with plain LDS stores and the decode ANDs
+// the four.  A operands come from the band's words, B operands from the mask
+// bits (qs words per mask row in qbits); with NC (Toeplitz chunks of
+// chunk_cols(F4) columns) known at compile time each A fragment feeds up to NC
+// MFMAs and each B fragment feeds all TP tiles.
+//   int8: tile t, chunk c reads band word t + c; both k-groups of a lane read
+//         the same word and take its bits 4g onward (frag_from_word).
+//   fp4:  tile t, chunk c is 64 columns, words t + 2c (k-group 0) and t + 2c +
+//         1 (k-group 1): a lane's word stream starts g words later, and its B
+//         window starts g mask words later.
+template <int TP, int NC, bool F4>
 __device__ __forceinline__ void corr_pass(lds_cu32 *band, int stride, int loff, int a0,
-                                          int a1, lds_cu32 *qbits, int tloc, int tcnt,
+                                          int a1, lds_cu32 *qbits, int qs, int tloc, int tcnt,
                                           lds_u64 *zw) {
-    constexpr int NA = TP + NC - 1;
+    constexpr int NA = F4 ? TP + 2 * NC - 1 : TP + NC - 1;   // band words per mask row
+    constexpr int NQ = F4 ? 2 * NC : NC + 1;                 // mask words per row a lane reads
     const int lane = threadIdx.x & 63;
-    const int ks = (lane >> 5) << 2;          // the lane's k-group shift, 4g
-    const int bs = 32 - (lane & 31) + ks;     // B window shift
-    v16i acc[TP];
+    const int g = lane >> 5;
+    const int ks = g << 2;                               // int8: the lane's k-group shift, 4g
+    const int bs = F4 ? 32 - (lane & 31) : 32 - (lane & 31) + ks;   // B window shift
+    typedef typename TileAcc<F4>::T AT;
+    AT acc[TP];
 #pragma unroll
-    for (int t = 0; t < TP; t++) acc[t] = (v16i){};
-    // the lane's band row (its candidate's row + mask row a) starts at base + a *
-    // stride; A fragment j is word j of it (tile t, chunk c: j = t + c)
-    lds_cu32 *base = band + loff + tloc;
+    for (int t = 0; t < TP; t++) acc[t] = (AT){};
+    // the lane's band row (its candidate's row + mask row a) starts at base + a * stride
+    lds_cu32 *base = band + loff + tloc + (F4 ? g : 0);
+    lds_cu32 *qb = qbits + (F4 ? g : 0);
     if (a0 < a1) {
-        // A words in order j = 0..NA-1 per mask row, each feeding the pairs
-        // (t = j - c, c); word j + D is loaded while j is used (the last D of a
-        // row load the next row's first D)
+        // band words in order j = 0..NA-1 per mask row; word j + D is loaded
+        // while j is used (the last D of a row load the next row's first D)
         constexpr int D = NA < 2 ? NA : 2;
         unsigned N[D];
-        unsigned qv[NC + 1];
+        unsigned qv[NQ];
         lds_cu32 *r0 = base + a0 * stride;
 #pragma unroll
         for (int j = 0; j < D; j++) N[j] = r0[j];
 #pragma unroll
-        for (int c = 0; c <= NC; c++) qv[c] = qbits[a0 * (NC + 1) + c];
+        for (int c = 0; c < NQ; c++) qv[c] = qb[a0 * qs + c];
         for (int a = a0; a < a1; a++) {
-            v4i B[NC];
+            typedef typename std::conditional<F4, v8i, v4i>::type FT;
+            FT B[NC];
 #pragma unroll
-            for (int c = 0; c < NC; c++) B[c] = bfrag_from_bits(qv[c], qv[c + 1], bs);
+            for (int c = 0; c < NC; c++) {
+                if constexpr (F4) B[c] = nib_frag((unsigned)((((unsigned long long)qv[2 * c + 1] << 32) | qv[2 * c]) >> bs));
+                else B[c] = bfrag_from_bits(qv[c], qv[c + 1], bs);
+            }
             const int an = a + 1 < a1 ? a + 1 : a;
             lds_cu32 *rc = base + a * stride;
             lds_cu32 *rn = base + an * stride;
 #pragma unroll
-            for (int c = 0; c <= NC; c++) qv[c] = qbits[an * (NC + 1) + c];
+            for (int c = 0; c < NQ; c++) qv[c] = qb[an * qs + c];
             unsigned Aw[NA];
 #pragma unroll
             for (int j = 0; j < D; j++) Aw[j] = N[j];
@@ -271,12 +318,21 @@ __device__ __forceinline__ void corr_pass(lds_cu32 *band, int stride, int loff, 
                 const int p = j + D;
                 if (p < NA) Aw[p] = rc[p];
                 else N[p - NA] = rn[p - NA];
-                const v4i A = frag_from_word(Aw[j] >> ks);
+                if constexpr (F4) {
+                    const v8i A = nib_frag(Aw[j]);
 #pragma unroll
-                for (int c = 0; c < NC; c++) {
-                    const int t = j - c;
-                    if (t >= 0 && t < TP)
-                        acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, B[c], acc[t], 0, 0, 0);
+                    for (int c = 0; c < NC; c++) {
+                        const int t = j - 2 * c;
+                        if (t >= 0 && t < TP) acc[t] = mfma_f4(A, B[c], acc[t]);
+                    }
+                } else {
+                    const v4i A = frag_from_word(Aw[j] >> ks);
+#pragma unroll
+                    for (int c = 0; c < NC; c++) {
+                        const int t = j - c;
+                        if (t >= 0 && t < TP)
+                            acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, B[c], acc[t], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -353,13 +409,13 @@ __host__ __device__ constexpr int pass_tiles(int nc) { return nc <= 5 ? 8 : (nc 
 // Passes alternate between two mask buffers, so one barrier per pass is
 // enough: a wave can only write buffer p%2 again (pass p+2) after every wave
 // has passed the barrier of pass p+1, i.e. has finished decoding pass p.
-template <int TP, int NC>
+template <int TP, int NC, bool F4>
 __device__ __forceinline__ int one_pass(lds_cu32 *band, int stride, const Fold &f, int a0, int a1,
-                                        lds_cu32 *qbits, int t0, int xbase, int xmax, int nrows,
+                                        lds_cu32 *qbits, int qs, int t0, int xbase, int xmax, int nrows,
                                         lds_u64 *s_zw, int &parity) {
     lds_u64 *zp = s_zw + parity * ZW_PASS;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    corr_pass<TP, NC>(band, stride, f.loff, a0, a1, qbits, t0, TP, zp + wave * ZW_WAVE);
+    corr_pass<TP, NC, F4>(band, stride, f.loff, a0, a1, qbits, qs, t0, TP, zp + wave * ZW_WAVE);
     __syncthreads();
     parity ^= 1;
     return decode_zero<TP>(zp, xbase + (t0 << 5), xmax, nrows, f.rlog, f.segx);
@@ -372,14 +428,14 @@ __device__ __forceinline__ int one_pass(lds_cu32 *band, int stride, const Fold &
 // zero-mask exchange and barrier).  Tiles past the staged window read
 // whatever the band row holds there (at most two words past the staged ones,
 // inside the row's stride); the caller's xmax ends at the window's last
-// column, so their counts are never decoded.  Returns the
-// window's minimum key; with `early`, stops once candidate 0 has a zero left
-// of every tile still to come (a key below xbase + 32 * t0; in a folded group
-// the later passes of segment 0 come before the current ones of the other
-// segments).
-template <int NC>
+// column, so their counts are never decoded.  Returns the window's minimum
+// key; with `early`, stops once candidate 0 has a zero left of every tile
+// still to come (a key below xbase + 32 * t0; in a folded group the later
+// passes of segment 0 come before the current ones of the other segments).
+// mfma counts the MFMA instructions issued.
+template <int NC, bool F4>
 __device__ __attribute__((always_inline)) int window_passes(lds_cu32 *band, int stride, const Fold &f, int h,
-                             lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
+                             lds_cu32 *qbits, int qs, int tl, int xbase, int xmax, int nrows,
                              lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
     constexpr int TPM = pass_tiles(NC);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -389,16 +445,16 @@ __device__ __attribute__((always_inline)) int window_passes(lds_cu32 *band, int 
         const int rem = tl - t0;
         int k, tp;
         if (TPM >= 8 && tpcap >= 8 && rem >= 6) {
-            k = one_pass<(TPM >= 8 ? 8 : 1), NC>(band, stride, f, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
+            k = one_pass<(TPM >= 8 ? 8 : 1), NC, F4>(band, stride, f, a0, a1, qbits, qs, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 8;
         } else if (TPM >= 4 && tpcap >= 4 && rem >= 3) {
-            k = one_pass<(TPM >= 4 ? 4 : 1), NC>(band, stride, f, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
+            k = one_pass<(TPM >= 4 ? 4 : 1), NC, F4>(band, stride, f, a0, a1, qbits, qs, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 4;
         } else if (TPM >= 2 && tpcap >= 2 && rem >= 2) {
-            k = one_pass<(TPM >= 2 ? 2 : 1), NC>(band, stride, f, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
+            k = one_pass<(TPM >= 2 ? 2 : 1), NC, F4>(band, stride, f, a0, a1, qbits, qs, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 2;
         } else {
-            k = one_pass<1, NC>(band, stride, f, a0, a1, qbits, t0, xbase, xmax, nrows, s_zw, parity);
+            k = one_pass<1, NC, F4>(band, stride, f, a0, a1, qbits, qs, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 1;
         }
         mfma += (unsigned long long)tp * (a1 - a0) * NC;
@@ -410,23 +466,31 @@ __device__ __attribute__((always_inline)) int window_passes(lds_cu32 *band, int 
 }
 
 // Generic chunk count (NC above the compiled set): one tile per pass, runtime
-// chunk loop, one band word per MFMA.
+// chunk loop, one band word (int8) or two (fp4) per MFMA.
+template <bool F4>
 __device__ int window_passes_generic(lds_cu32 *band, int stride, const Fold &f, int h, int nc,
-                                     lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
+                                     lds_cu32 *qbits, int qs, int tl, int xbase, int xmax, int nrows,
                                      lds_u64 *s_zw, unsigned long long &mfma, bool early) {
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int ks = (lane >> 5) << 2, bs = 32 - (lane & 31) + ks;
+    const int g = lane >> 5, ks = g << 2, bs = F4 ? 32 - (lane & 31) : 32 - (lane & 31) + ks;
     const int a0 = f.skip + (((h - f.skip) * wave) >> 2), a1 = f.skip + (((h - f.skip) * (wave + 1)) >> 2);
     int key = KEY_NONE, parity = 0;
+    typedef typename TileAcc<F4>::T AT;
     for (int t0 = 0; t0 < tl; t0++) {
-        v16i acc[1] = {(v16i){}};
-        lds_cu32 *base = band + f.loff + t0;
+        AT acc[1] = {(AT){}};
+        lds_cu32 *base = band + f.loff + t0 + (F4 ? g : 0);
         for (int a = a0; a < a1; a++) {
             lds_cu32 *rp = base + a * stride;
-            lds_cu32 *q = qbits + a * (nc + 1);
-            for (int c = 0; c < nc; c++)
-                acc[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(frag_from_word(rp[c] >> ks),
-                                                               bfrag_from_bits(q[c], q[c + 1], bs), acc[0], 0, 0, 0);
+            lds_cu32 *q = qbits + a * qs + (F4 ? g : 0);
+            for (int c = 0; c < nc; c++) {
+                if constexpr (F4) {
+                    const unsigned bw = (unsigned)((((unsigned long long)q[2 * c + 1] << 32) | q[2 * c]) >> bs);
+                    acc[0] = mfma_f4(nib_frag(rp[2 * c]), nib_frag(bw), acc[0]);
+                } else {
+                    acc[0] = __builtin_amdgcn_mfma_i32_32x32x32_i8(frag_from_word(rp[c] >> ks),
+                                                                   bfrag_from_bits(q[c], q[c + 1], bs), acc[0], 0, 0, 0);
+                }
+            }
         }
         mfma += (unsigned long long)(a1 - a0) * nc;
         lds_u64 *zp = s_zw + parity * ZW_PASS;
@@ -440,26 +504,29 @@ __device__ int window_passes_generic(lds_cu32 *band, int stride, const Fold &f, 
     return key;
 }
 
+template <bool F4>
 __device__ __forceinline__ int window_dispatch(int nc, lds_cu32 *band, int stride, const Fold &f, int h,
-                               lds_cu32 *qbits, int tl, int xbase, int xmax, int nrows,
+                               lds_cu32 *qbits, int qs, int tl, int xbase, int xmax, int nrows,
                                lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
-#define CASE(K) case K: return window_passes<K>(band, stride, f, h, qbits, tl, xbase, xmax, nrows, s_zw, mfma, tpcap, early);
+#define CASE(K) case K: return window_passes<K, F4>(band, stride, f, h, qbits, qs, tl, xbase, xmax, nrows, s_zw, mfma, tpcap, early);
     switch (nc) {
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
         default:
-            return window_passes_generic(band, stride, f, h, nc, qbits, tl, xbase, xmax, nrows, s_zw, mfma, early);
+            return window_passes_generic<F4>(band, stride, f, h, nc, qbits, qs, tl, xbase, xmax, nrows, s_zw, mfma, early);
     }
 #undef CASE
 }
 
-// LDS row stride (32-bit words) of a window of tw x-tiles.  A window of tiles
-// [tw0, tw0 + tw) reads the words [tw0, tw0 + tw + nc - 1) of its rows; they are
-// staged as the 16-byte-aligned words from tw0 rounded down (up to 3 words in
-// front) to the end rounded up, and a pass rounded up past the window reads up
-// to two words further: tw + nc + 5 words cover all of it.  The stride is odd,
-// so the 32 band rows one A-fragment load touches fall into 32 distinct banks.
-__host__ __device__ __forceinline__ int band_stride(int tw, int nc) {
-    return (tw + nc + 5) | 1;
+// LDS row stride (32-bit words) of a window of tw x-tiles for a mask whose
+// Toeplitz chunks span cw words (nc for int8 chunks of 32 columns, 2 * nc4 for
+// fp4 chunks of 64).  A window of tiles [tw0, tw0 + tw) reads the words [tw0,
+// tw0 + tw + cw - 1) of its rows; they are staged as the 16-byte-aligned words
+// from tw0 rounded down (up to 3 words in front) to the end rounded up, and a
+// pass rounded up past the window reads up to two words further: tw + cw + 5
+// words cover all of it.  The stride is odd, so the 32 band rows one
+// A-fragment load touches fall into 32 distinct banks.
+__host__ __device__ __forceinline__ int band_stride(int tw, int cw) {
+    return (tw + cw + 5) | 1;
 }
 
 // Words kept free after the band's last row.
@@ -467,12 +534,12 @@ __host__ __device__ __forceinline__ int band_stride(int tw, int nc) {
 
 // Largest x-window (in tiles, <= cap) whose 32-row band fits (with
 // BAND_SLACK words to spare); shrinks the window rather than the row group.
-__host__ __device__ __forceinline__ int band_window(int T, int nc, int h, int band_words, int cap,
+__host__ __device__ __forceinline__ int band_window(int T, int cw, int h, int band_words, int cap,
                                                     int *stride_out, int *rcap_out) {
     int tw = T < cap ? T : cap;
     int st, rc;
     for (;;) {
-        st = band_stride(tw, nc);
+        st = band_stride(tw, cw);
         rc = (band_words - BAND_SLACK) / st;
         if (rc >= h + GROUP_ROWS - 1 || tw == 1) break;
         tw = (tw + 1) >> 1;
@@ -513,19 +580,21 @@ __device__ __forceinline__ void wg_copy(int n, Load load, Store store) {
 // zero overlap, or KEY_NONE.  Uniform across the workgroup.  Grid rows above r0
 // are empty (no part reaches above the bin's top-most placement), so mask rows
 // that only meet them are neither staged nor correlated, except in the dense
-// mode.
-template <bool DENSE>
+// mode.  nc: Toeplitz chunks of the operand format (32 columns for int8, 64
+// for fp4), cw: the words they span, qs: mask words per row in qbits_g.
+template <bool DENSE, bool F4>
 __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid, int gpw, int W, int h, int w, int nc,
-                             const unsigned *qbits_g, const short *cand, int ncand, int r0, lds_u32 *band,
-                             int band_words, lds_u32 *s_q, lds_u64 *s_zw, unsigned long long &groups,
-                             unsigned long long &mfma, unsigned long long *ph, int tpcap, int wcap) {
+                             int cw, int qs, const unsigned *qbits_g, const short *cand, int ncand, int r0,
+                             lds_u32 *band, int band_words, lds_u32 *s_q, lds_u64 *s_zw,
+                             unsigned long long &groups, unsigned long long &mfma, unsigned long long *ph,
+                             int tpcap, int wcap) {
     const int tid = threadIdx.x, lane = tid & 63;
     // mask bits of the rotation -> LDS (read by every wave for every group)
-    wg_copy<unsigned>(h * (nc + 1), [&](int k) { return qbits_g[k]; },
+    wg_copy<unsigned>(h * qs, [&](int k) { return qbits_g[k]; },
                       [&](int k, unsigned v) { s_q[k] = v; });
     const int T = (W - w + 1 + 31) >> 5;
     int stride, rcap;
-    const int TW = band_window(T, nc, h, band_words, wcap, &stride, &rcap);
+    const int TW = band_window(T, cw, h, band_words, wcap, &stride, &rcap);
     const int maxrows = min(GROUP_ROWS, rcap - h + 1);
     int found = KEY_NONE;
     for (int g0 = 0; g0 < ncand;) {
@@ -549,10 +618,10 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid
         for (int tw0 = 0; tw0 < T && skip < h; tw0 += TW) {
             unsigned long long t0 = tid == 0 ? tick() : 0;
             const int tl = min(TW, T - tw0);
-            // words [tw0, tw0 + tl + nc - 1) of each row, as the 16-byte vectors
+            // words [tw0, tw0 + tl + cw - 1) of each row, as the 16-byte vectors
             // from tw0 rounded down (c0) to the end rounded up (<= gpw)
             const int c0 = tw0 & ~3;
-            const int q4 = (((tw0 + tl + nc - 1 + 3) & ~3) - c0) >> 2;
+            const int q4 = (((tw0 + tl + cw - 1 + 3) & ~3) - c0) >> 2;
             const DivBy byq(q4);
             const unsigned *src = grid + (long long)(ylo + skip) * gpw + c0;
             lds_u32 *dst = band + skip * stride;
@@ -575,7 +644,7 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid
             f.loff = (cand[g0 + min(m & ((1 << f.rlog) - 1), nrows - 1)] - ylo) * stride + (tw0 - c0) +
                      (f.segx >> 5);
             f.skip = skip;
-            const int k = window_dispatch(nc, band, stride, f, h, s_q, tf, tw0 << 5,
+            const int k = window_dispatch<F4>(nc, band, stride, f, h, s_q, qs, tf, tw0 << 5,
                                           min(W - w, ((tw0 + tl) << 5) - 1), nrows, s_zw, mfma, tpcap, !DENSE);
             gkey = k < gkey ? k : gkey;
             if (tid == 0) ph[3] += tick() - t1;
@@ -888,7 +957,7 @@ __device__ __forceinline__ int winning_prefix(const short *cand, int ncand, int 
 // mode tests them all), and equal placements go to the smaller rotation
 // index, so the order changes no choice.  Returns the rotation, or -1, with
 // its top-left (by, bx); uniform across the workgroup.
-template <bool DENSE>
+template <bool DENSE, bool F4>
 __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork &Wk, int mach, int rb, int nr, int rf,
                                               int H, int W, int enc, const unsigned *grid,
                                               const short *s_cand, const int *s_ncand, int hmax, lds_u32 *band,
@@ -909,10 +978,10 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
         }
         c_tests++;
         c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
-        const int ncr = P.rot_nc[rg];
+        const int ncr = P.rot_nc[rg], nc4 = P.rot_nc4[rg];
         const unsigned char *tc = P.tiles + ((long long)mach * TUNE_NC + min(ncr, TUNE_NC - 1)) * 2;
-        const int key = test_rotation<DENSE>(grid, Wk.pw, W, h, w, ncr,
-                                             P.qbits + P.qbits_off[rg], cand, ncand, H - enc,
+        const int key = test_rotation<DENSE, F4>(grid, Wk.pw, W, h, w, F4 ? nc4 : ncr, F4 ? 2 * nc4 : ncr,
+                                                 2 * nc4 + 2, P.qbits + P.qbits_off[rg], cand, ncand, H - enc,
                                              band, Wk.band_words, s_q, s_zw, c_groups, c_mfma, ph,
                                              uni(tc[0]), uni(tc[1]));
         if (key == KEY_NONE) continue;
@@ -941,7 +1010,7 @@ __device__ __forceinline__ void stamp_part(const DevProblem &P, const DevWork &W
     const int pw = Wk.pw, q4 = pw >> 2;         // words and 16-byte vectors per row
     const DivBy byq4(q4);
     const int chunk = min(h, Wk.band_words / pw);
-    const int nq = P.rot_nc[rg] + 1;            // mask bit words per row
+    const int nq = 2 * P.rot_nc4[rg] + 2;       // mask bit words per row
     const int q0 = px >> 5, nw = ((px + w - 1) >> 5) - q0 + 1;   // grid words per row the part covers
     const DivBy bynw(nw);
     for (int a0 = 0; a0 < h; a0 += chunk) {
@@ -988,7 +1057,9 @@ __device__ __forceinline__ void stamp_part(const DevProblem &P, const DevWork &W
     }
 }
 
-template <bool DENSE>
+// DENSE: every placement row, no early exit (the full-shape decode); F4: fp4
+// MFMA operands (else int8).  Every instance gives the same results.
+template <bool DENSE, bool F4>
 __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, DevWork Wk, LaunchArgs L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = P.n_parts;
@@ -1110,7 +1181,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                 if (tid == 0) ph[1] += tick() - tp;
                 // ---- collision tests, one rotation at a time (whole workgroup) ----
                 int bx = 0, by = 0;
-                const int best = best_placement<DENSE>(P, Wk, mach, rb, nr, P.best_rot[ty], H, W, H - b_minrow[b], grid, s_cand,
+                const int best = best_placement<DENSE, F4>(P, Wk, mach, rb, nr, P.best_rot[ty], H, W, H - b_minrow[b], grid, s_cand,
                                                        s_ncand, hmax, band, s_q, s_zw, c_tests, c_algo, c_groups,
                                                        c_mfma, ph, by, bx);
                 if (best >= 0) {
@@ -1187,7 +1258,8 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
         __syncthreads();
     }
     // ---- statistics ----
-    if (lane == 0) atomicAdd(&Wk.counters[ST_MFMA], c_mfma * 32ull * 32ull * 32ull * 2ull);
+    // ops per MFMA instruction: 32 x 32 outputs x K (32 int8, 64 fp4) x 2
+    if (lane == 0) atomicAdd(&Wk.counters[ST_MFMA], c_mfma * 32ull * 32ull * (unsigned long long)chunk_cols(F4) * 2ull);
     if (tid == 0) {
         atomicAdd(&Wk.counters[ST_TESTS], c_tests);
         atomicAdd(&Wk.counters[ST_GROUPS], c_groups);
@@ -1220,6 +1292,7 @@ struct GreedyArgs {
 // One workgroup per (machine, batch): binClassInitialSol.py:41-111 can_insert
 // -- per rotation the vacancy filter and the bottom-left collision test, then
 // the density -> row -> col choice (no area check; the greedy has none).
+template <bool F4>
 __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P, DevWork Wk, GreedyArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = P.n_parts, hmax = Wk.hmax, tid = threadIdx.x;
@@ -1243,7 +1316,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P
     __syncthreads();
     unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int bx = 0, by = 0;
-    const int best = best_placement<false>(P, Wk, m, rb, nr, P.best_rot[ty], H, W, enc, grid, s_cand, s_ncand, hmax, band, s_q,
+    const int best = best_placement<false, F4>(P, Wk, m, rb, nr, P.best_rot[ty], H, W, enc, grid, s_cand, s_ncand, hmax, band, s_q,
                                            s_zw, c0, c1, c2, c3, ph, by, bx);
     if (tid == 0) {
         int *r = A.res + 4 * blockIdx.x;
@@ -1303,6 +1376,7 @@ struct brkga_handle {
     int n_cu = 0;
     int n_wg = 0;
     int dense = 0;
+    int f4 = 1;   // fp4 MFMA operands (else int8)
     std::vector<unsigned char> tiles;   // host copy of DevProblem::tiles
     unsigned char *d_tiles = nullptr;
     brkga_problem prob{};     // scalars only
@@ -1451,6 +1525,11 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     if (o) opt = *o;
     h->device = opt.device;
     h->dense = opt.dense ? 1 : 0;
+    if (opt.mfma_format != BRKGA_MFMA_FP4 && opt.mfma_format != BRKGA_MFMA_INT8) {
+        delete h;
+        return fail(nullptr, BRKGA_EINVAL, "mfma_format must be BRKGA_MFMA_FP4 (0) or BRKGA_MFMA_INT8 (1)");
+    }
+    h->f4 = opt.mfma_format == BRKGA_MFMA_FP4;
     if (!(opt.tiles_per_pass == 0 || opt.tiles_per_pass == 1 || opt.tiles_per_pass == 2 ||
           opt.tiles_per_pass == 4 || opt.tiles_per_pass == 8)) {
         delete h;
@@ -1474,7 +1553,7 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
 
     const int M = p->n_machines, T = p->n_types, R = p->n_rots;
     // ---- per-rotation tables: densities (BRKGA_alg3.py:351-360) and Toeplitz operands ----
-    std::vector<int> dens, nc(R);
+    std::vector<int> dens, nc(R), nc4(R);
     std::vector<long long> dens_off(R), qbits_off(R);
     std::vector<unsigned> qbits;
     int hmax = 0, wmax = 0, pw = 4;   // pw: 32-bit words per grid bit row
@@ -1488,11 +1567,12 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
             for (int b = 0; b < rw; b++) { cur = mk[a * rw + b] ? cur + 1 : 0; best = std::max(best, cur); }
             dens.push_back(best);
         }
-        nc[r] = (rw + 31 + 31) / 32;
+        nc[r] = (rw + 31 + 31) / 32;    // 32-column chunks a 32-placement tile spans (int8)
+        nc4[r] = (rw + 31 + 63) / 64;   // 64-column chunks (fp4)
         qbits_off[r] = (long long)qbits.size();
-        for (int a = 0; a < rh; a++) {
+        for (int a = 0; a < rh; a++) {   // 2 nc4 + 2 >= nc + 1 words per row, zero past the mask
             const size_t row0 = qbits.size();
-            qbits.resize(row0 + nc[r] + 1, 0u);
+            qbits.resize(row0 + 2 * nc4[r] + 2, 0u);
             for (int b = 0; b < rw; b++)
                 if (mk[a * rw + b]) qbits[row0 + (b + 32) / 32] |= 1u << ((b + 32) % 32);
         }
@@ -1500,7 +1580,7 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
             const int H = p->plate_h[m], W = p->plate_w[m];
             if (rh > H || rw > W) continue;
             const int Tt = (W - rw + 1 + 31) / 32;
-            pw = std::max(pw, Tt + nc[r] - 1);   // the widest span a staged window reads
+            pw = std::max(pw, Tt + 2 * nc4[r] - 1);   // the widest span a staged window reads (2 nc4 >= nc)
         }
     }
     pw = std::max(pw, (wmax + 31) / 32);
@@ -1540,6 +1620,7 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
         (rc = upload(h, p->rot_h, (size_t)R, &d.rot_h)) ||
         (rc = upload(h, p->rot_w, (size_t)R, &d.rot_w)) ||
         (rc = upload(h, nc.data(), (size_t)R, &d.rot_nc)) ||
+        (rc = upload(h, nc4.data(), (size_t)R, &d.rot_nc4)) ||
         (rc = upload(h, (const long long *)p->mask_off, (size_t)R, &d.mask_off)) ||
         (rc = upload(h, dens_off.data(), (size_t)R, &d.dens_off)) ||
         (rc = upload(h, qbits_off.data(), (size_t)R, &d.qbits_off)) ||
@@ -1592,7 +1673,7 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
                    (size_t)MAX_ROT * hmax * 2 + (size_t)hmax * 2 + (size_t)w.vc_bins * hmax * 2 + 64 * 2;
     fixed = (fixed + 15) & ~(size_t)15;
     size_t qwords = 1;
-    for (int r = 0; r < R; r++) qwords = std::max(qwords, (size_t)p->rot_h[r] * (nc[r] + 1));
+    for (int r = 0; r < R; r++) qwords = std::max(qwords, (size_t)p->rot_h[r] * (2 * nc4[r] + 2));
     w.q_off = (int)fixed;
     fixed += (qwords * 4 + 15) & ~(size_t)15;
     w.zw_off = (int)fixed;
@@ -1609,8 +1690,8 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
             const int H = p->plate_h[m], W = p->plate_w[m], rh = p->rot_h[r], rw = p->rot_w[r];
             if (rh > H || rw > W) continue;
             const int Tt = (W - rw + 1 + 31) / 32;
-            need_min = std::max(need_min, ((size_t)rh * band_stride(1, nc[r]) + BAND_SLACK) * 4);
-            need_4 = std::max(need_4, ((size_t)(rh + GROUP_ROWS - 1) * band_stride(std::min(Tt, 4), nc[r]) +
+            need_min = std::max(need_min, ((size_t)rh * band_stride(1, 2 * nc4[r]) + BAND_SLACK) * 4);
+            need_4 = std::max(need_4, ((size_t)(rh + GROUP_ROWS - 1) * band_stride(std::min(Tt, 4), 2 * nc4[r]) +
                                        BAND_SLACK) * 4);
         }
     size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (size_t)prop.maxSharedMemoryPerMultiProcessor : 65536;
@@ -1628,8 +1709,10 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     h->smem = fixed + (size_t)w.band_words * 4;
     h->n_wg = std::max(1, h->n_cu * wpc2);   // persistent grid at the occupancy the LDS allows
     if ((rc = device_workspace(h, mb))) return bail(rc);
-    for (const void *kern : {(const void *)decode_kernel<false>, (const void *)decode_kernel<true>,
-                             (const void *)greedy_test_kernel, (const void *)greedy_insert_kernel})
+    for (const void *kern : {(const void *)decode_kernel<false, false>, (const void *)decode_kernel<true, false>,
+                             (const void *)decode_kernel<false, true>, (const void *)decode_kernel<true, true>,
+                             (const void *)greedy_test_kernel<false>, (const void *)greedy_test_kernel<true>,
+                             (const void *)greedy_insert_kernel})
         if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->smem) != hipSuccess) {
             h->err = "hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed";
             return bail(BRKGA_EDEVICE);
@@ -1702,7 +1785,8 @@ static int launch_decode(brkga_handle *h, const void *d_keys, int key_dtype, int
                        cb.order, cb.queue);
     HIPCHK(h, hipGetLastError());
     if (timed) HIPCHK(h, hipEventRecord(h->ev0, s));
-    auto kern = h->dense ? decode_kernel<true> : decode_kernel<false>;
+    auto kern = h->f4 ? (h->dense ? decode_kernel<true, true> : decode_kernel<false, true>)
+                      : (h->dense ? decode_kernel<true, false> : decode_kernel<false, false>);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(WG_THREADS), h->smem, s, h->dp, wk, L);
     HIPCHK(h, hipGetLastError());
     if (timed) {
@@ -2003,7 +2087,8 @@ extern "C" int brkga_initial_solution(brkga_handle *h, const double *type_height
             GreedyArgs A;
             A.grids = d_grids; A.vac = d_vac; A.max_batches = n; A.ty = ty; A.jobs = d_jobs; A.res = d_res;
             HIPCHK(h, hipMemcpyAsync(d_jobs, jobs.data(), jobs.size() * sizeof(int), hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(greedy_test_kernel, dim3(nj), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, A);
+            if (h->f4) hipLaunchKernelGGL(greedy_test_kernel<true>, dim3(nj), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, A);
+            else hipLaunchKernelGGL(greedy_test_kernel<false>, dim3(nj), dim3(WG_THREADS), h->smem, s, h->dp, h->dw, A);
             HIPCHK(h, hipGetLastError());
             HIPCHK(h, hipMemcpyAsync(res.data(), d_res, res.size() * sizeof(int), hipMemcpyDeviceToHost, s));
             HIPCHK(h, hipStreamSynchronize(s));

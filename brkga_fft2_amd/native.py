@@ -33,7 +33,11 @@ class BrkgaProblem(ctypes.Structure):
 class BrkgaOptions(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("wgs_per_cu", ctypes.c_int32),
                 ("max_bins", ctypes.c_int32), ("dense", ctypes.c_int32),
-                ("tiles_per_pass", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("tiles_per_pass", ctypes.c_int32), ("mfma_format", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
+
+
+MFMA_FORMATS = {"fp4": 0, "int8": 1}   # BRKGA_MFMA_FP4, BRKGA_MFMA_INT8
 
 
 class BrkgaStats(ctypes.Structure):
@@ -152,7 +156,7 @@ class Decoder:
     """One decoder handle: a problem uploaded to one GPU."""
 
     def __init__(self, packed, device=0, dense=False, tiles_per_pass=0, wgs_per_cu=0,
-                 max_bins=0):
+                 max_bins=0, mfma="fp4"):
         self._lib = load_library()
         self.n_parts = int(packed["n_parts"])
         self.n_machines = int(packed["n_machines"])
@@ -165,8 +169,11 @@ class Decoder:
             a = np.ascontiguousarray(packed[name], dtype=dt)
             keep.append(a)
             setattr(prob, name, a.ctypes.data if a.size else None)
+        if mfma not in MFMA_FORMATS:
+            raise ValueError("mfma must be one of %s" % sorted(MFMA_FORMATS))
+        self.mfma = mfma
         opt = BrkgaOptions(int(device), int(wgs_per_cu), int(max_bins), int(bool(dense)),
-                           int(tiles_per_pass))
+                           int(tiles_per_pass), MFMA_FORMATS[mfma])
         h = ctypes.c_void_p()
         rc = self._lib.brkga_create(ctypes.byref(prob), ctypes.byref(opt), ctypes.byref(h))
         if rc != 0:

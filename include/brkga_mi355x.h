@@ -90,8 +90,14 @@ typedef struct brkga_options {
                                 (same results; used to measure the kernel at full shape) */
     int32_t tiles_per_pass;  /* cap on correlation x-tiles per pass for every class of the
                                 tile table: 1, 2, 4 or 8 (0 = per-width default) */
-    int32_t reserved[3];
+    int32_t mfma_format;     /* operands of the collision correlation's MFMAs:
+                                BRKGA_MFMA_FP4 (0, default) or BRKGA_MFMA_INT8 (1); the
+                                counts are exact either way and the results identical */
+    int32_t reserved[2];
 } brkga_options;
+
+#define BRKGA_MFMA_FP4 0    /* e2m1 operands 0 / 0.5 / 1, float32 accumulation, K = 64 per MFMA */
+#define BRKGA_MFMA_INT8 1   /* int8 operands, int32 accumulation, K = 32 per MFMA */
 
 typedef struct brkga_stats {
     int64_t calls;           /* evaluate calls */

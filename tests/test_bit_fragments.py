@@ -7,7 +7,9 @@ and every lane reads its A words at loff + t0 + j (corr_pass).  This restates
 those steps in Python on random grids and masks and checks, per candidate and
 position, that "overlap count == 0" matches the brute-force overlap; it also
 restates the stamp's word extraction (stamp_part) and the bit-row free-run scan
-(rows_free_run, longest_free_in_word) against plain scans."""
+(rows_free_run, longest_free_in_word) against plain scans.  The fp4 path
+(nib_frag: 32x32x64 e2m1 MFMAs over 64-column chunks, a lane's word stream
+and mask window offset by its k-group) is restated the same way."""
 import random
 
 import numpy as np
@@ -19,9 +21,10 @@ def band_stride(tw, nc):
     return (tw + nc + 5) | 1
 
 
-def qbits_row(mask_row, nc):
-    """The mask row's bits after 32 zero bits, nc + 1 words (brkga_create)."""
-    q = [0] * (nc + 1)
+def qbits_row(mask_row, nc, nq=None):
+    """The mask row's bits after 32 zero bits, nq (default nc + 1) words, zero
+    padded (brkga_create keeps 2 nc4 + 2 words per row)."""
+    q = [0] * (nq or nc + 1)
     for b, v in enumerate(mask_row):
         if v:
             q[(b + 32) // 32] |= 1 << ((b + 32) % 32)
@@ -136,11 +139,12 @@ def test_stamp_word_extraction():
     for _ in range(500):
         w = rng.randint(1, 200)
         nc = (w + 62) // 32
-        nq = nc + 1
+        nq = 2 * ((w + 31 + 63) // 64) + 2   # the kernel's row of mask words
+        assert nq >= nc + 1
         row = [1 if rng.random() < 0.5 else 0 for _ in range(w)]
         W = rng.randint(w, 600)
         px = rng.randint(0, W - w)
-        q = qbits_row(row, nc)
+        q = qbits_row(row, nc, nq)
         Q = sum(v << (32 * i) for i, v in enumerate(q))
         q0, q1 = px >> 5, (px + w - 1) >> 5
         words = {}
@@ -260,3 +264,120 @@ def test_bit_row_scan_matches_plain_scan():
         assert bit_row_scan(row) == plain_scan(row), W
     for W in (1, 64, 300, 512, 4096):   # all free, all occupied
         assert bit_row_scan([0] * W) == W and bit_row_scan([1] * W) == 0
+
+
+# ---- fp4 (e2m1) operands: 32x32x64 MFMA, lane (row/col = l % 32, g = l // 32)
+# supplies k = 32g + 8v + e (VGPR v, nibble e, low nibble first) ----
+
+E2M1 = {0: 0.0, 1: 0.5, 2: 1.0, 4: 2.0}
+
+
+def nib_frag(w):
+    """nib_frag: the four VGPRs of e2m1 codes from one 32-bit word."""
+    M = 0xFFFFFFFF
+    return [w & 0x21212121, (w >> 1) & 0x21212121, (w >> 2) & 0x21212121,
+            ((w >> 3) & 0x01010101) | ((w << 1) & 0x20202020 & M)]
+
+
+def nib_values(w):
+    """The lane's 32 operand values in MFMA k order (8v + e)."""
+    out = []
+    for v in nib_frag(w):
+        for e in range(8):
+            out.append(E2M1[(v >> (4 * e)) & 0xF])
+    return out
+
+
+def nib_bits(w):
+    """Which bit of the word each k position holds (None when zero for every word)."""
+    pos = []
+    for k in range(32):
+        src = [b for b in range(32) if nib_values(1 << b)[k] != 0]
+        pos.append(src[0] if len(src) == 1 else None)
+    return pos
+
+
+def test_fp4_fragment_is_a_bijection_with_positive_values():
+    pos = nib_bits(0)
+    assert sorted(pos) == list(range(32))
+    for b in range(32):   # one set cell -> one positive value, nothing else
+        vals = nib_values(1 << b)
+        assert sum(1 for v in vals if v) == 1 and max(vals) in (0.5, 1.0)
+
+
+def test_fp4_tile_counts_match_brute_force_overlap():
+    """One fp4 tile per 64-column chunk c: lane (m, g) takes band word t + 2c + g,
+    lane (n, g) the mask window (Q words 2c + g, 2c + g + 1) >> (32 - n); zero
+    counts are the zero overlaps of the placements x0 + n."""
+    rng = random.Random(9)
+    for _ in range(10):
+        w = rng.randint(1, 150)
+        nc4 = (w + 31 + 63) // 64
+        nq = 2 * nc4 + 2
+        row = [1 if rng.random() < 0.6 else 0 for _ in range(w)]
+        row[rng.randrange(w)] = 1
+        q = qbits_row(row, None, nq)
+        width = 64 * (nc4 + 1)
+        grid = [[1 if rng.random() < rng.choice([0.02, 0.2]) else 0 for _ in range(width)] for _ in range(32)]
+        words = [[sum(grid[m][32 * j + i] << i for i in range(32)) for j in range(width // 32)] for m in range(32)]
+        total = np.zeros((32, 32))
+        for c in range(nc4):
+            A = np.zeros((32, 64))
+            B = np.zeros((64, 32))
+            for m in range(32):
+                for g in range(2):
+                    A[m, 32 * g:32 * g + 32] = nib_values(words[m][2 * c + g])
+            for n in range(32):
+                for g in range(2):
+                    win = (q[2 * c + g + 1] << 32) | q[2 * c + g]
+                    B[32 * g:32 * g + 32, n] = nib_values((win >> (32 - n)) & 0xFFFFFFFF)
+            total += A @ B
+        for m in range(32):
+            for n in range(32):
+                ov = sum(row[b] * grid[m][n + b] for b in range(w))
+                assert (total[m, n] == 0) == (ov == 0), (w, m, n)
+
+
+def test_fp4_staged_window_addressing():
+    """The fp4 path's word addresses loff + t0 + j + g (j = t + 2c) and mask words
+    g + 2c, g + 2c + 1 stay inside the band row's stride and the row's 2 nc4 + 2
+    mask words, and read the grid word of column x0 + 64c + 32g when staged."""
+    rng = random.Random(13)
+    for _ in range(300):
+        W = rng.randint(40, 700)
+        w = rng.randint(1, min(W, 200))
+        nc4 = (w + 31 + 63) // 64
+        cw = 2 * nc4
+        T = (W - w + 1 + 31) // 32
+        pw = max(4, T + cw - 1, (W + 31) // 32)
+        pw = (pw + 3) & ~3
+        TW = rng.randint(1, T)
+        stride = band_stride(TW, cw)
+        for tw0 in range(0, T, TW):
+            tl = min(TW, T - tw0)
+            c0 = tw0 & ~3
+            q4 = (((tw0 + tl + cw - 1 + 3) & ~3) - c0) >> 2
+            assert c0 + 4 * q4 <= pw and 4 * q4 <= stride
+            for rl in range(6):
+                nseg = 32 >> rl
+                if nseg > tl:
+                    continue
+                tf = (tl + nseg - 1) >> (5 - rl)
+                for seg in range(nseg):
+                    segx = min(seg * tf, tl - tf) << 5
+                    t0 = 0
+                    while t0 < tf:
+                        rem = tf - t0
+                        tp = 8 if rem >= 6 else (4 if rem >= 3 else (2 if rem >= 2 else 1))
+                        for t in range(tp):
+                            for c in range(nc4):
+                                for g in range(2):
+                                    assert g + 2 * c + 1 < 2 * nc4 + 2
+                                    off = (tw0 - c0) + (segx >> 5) + t0 + t + 2 * c + g
+                                    assert off < stride
+                                    col = (tw0 << 5) + segx + 32 * (t0 + t) + 64 * c + 32 * g
+                                    if off < 4 * q4:
+                                        assert c0 + off == col >> 5
+                                    else:
+                                        assert segx + 32 * (t0 + t) >= 32 * tl
+                        t0 += tp

@@ -76,11 +76,29 @@ def test_tall_plates_vs_oracle(plates):
     e = ref_log[ref_ms < 1e15].reshape(-1, 6)
     h = np.array([packed["rot_h"][packed["rot_base"][tindex[int(r[1])]] + r[5]] for r in e])
     assert np.sum(e[:, 4] - h + 1 >= 512) >= 10
-    for dense in (False, True):
-        dec = _decoder(packed, dense=dense)
+    for dense, mfma in ((False, "fp4"), (True, "fp4"), (False, "int8")):
+        dec = _decoder(packed, dense=dense, mfma=mfma)
         ms, log = dec.evaluate(keys, with_log=True)
         np.testing.assert_array_equal(ms, ref_ms)
         assert feasible_logs_equal(ref_ms, log, ref_log) is None
+        dec.close()
+
+
+@pytest.mark.parametrize("cfg,n_chrom", [("C1", 32), ("C2", 48), ("C3", 24), ("C5", 16)])
+def test_int8_operands_agree(cfg, n_chrom):
+    """The int8 MFMA operand format (brkga_options.mfma_format = BRKGA_MFMA_INT8),
+    pruned and at full shape: the same placements and makespans as the oracle."""
+    _say("int8 operands %s N=%d" % (cfg, n_chrom))
+    prob, n, M, packed = syn_instance(cfg)
+    keys = syn_keys(cfg, n_chrom, n, seed=17)
+    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=oracle_threads(), with_log=True)
+    for dense in (False, True):
+        dec = _decoder(packed, mfma="int8", dense=dense)
+        ms, log = dec.evaluate(keys, with_log=True)
+        np.testing.assert_array_equal(ms, ref_ms)
+        assert feasible_logs_equal(ref_ms, log, ref_log) is None
+        st = dec.stats()
+        assert st["mfma_ops"] % (2 * 32 * 32 * 32) == 0
         dec.close()
 
 
