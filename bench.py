@@ -31,7 +31,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA peak, 5.0 POP/s
+# MI355X dense MFMA peaks (TOP/s) of the collision correlation's operand formats
+PEAK_TOPS = {"fp4": 10100.0, "int8": 5000.0}
 
 
 def _say(msg):
@@ -86,7 +87,8 @@ def _workload(config, pop, n, M, cfg):
                                    cfg["bbox"][0], cfg["bbox"][1], cfg["nrot"], pop, cfg["key_dtype"]))
 
 
-def _timed_decode(config, pop, steps, warmup, seed, dev, autotune_reps=0, only_timed=False, dense=False):
+def _timed_decode(config, pop, steps, warmup, seed, dev, autotune_reps=0, only_timed=False, dense=False,
+                  mfma="fp4"):
     """Decode ``warmup`` then ``steps`` device-resident populations of ``config``
     back to back on one stream; returns the record and the state later legs use."""
     import numpy as np
@@ -103,7 +105,7 @@ def _timed_decode(config, pop, steps, warmup, seed, dev, autotune_reps=0, only_t
     keys = [torch.from_numpy(p).to(dev) for p in host_pops]
     outs = [torch.empty(pop, dtype=torch.float64, device=dev) for _ in range(total)]
     stream = torch.cuda.current_stream(dev)
-    dec = Decoder(packed, device=0, dense=dense)
+    dec = Decoder(packed, device=0, dense=dense, mfma=mfma)
     torch.cuda.synchronize()
     tuned = None
     if autotune_reps:
@@ -129,7 +131,7 @@ def _timed_decode(config, pop, steps, warmup, seed, dev, autotune_reps=0, only_t
     if st["overflows"]:
         raise RuntimeError("bin workspace overflowed during the timed steps")
     elapsed = t1 - t0
-    rec = {"config": config, "population": pop, "parts": n, "machines": M,
+    rec = {"config": config, "population": pop, "parts": n, "machines": M, "mfma": mfma,
            "workload": _workload(config, pop, n, M, cfg),
            "value": pop * steps / elapsed, "ms_per_step": elapsed * 1e3 / steps, "stats": st,
            "autotune": tuned}
@@ -174,9 +176,10 @@ def _roofline(rec, state, steps, warmup, roofline_steps, dev):
     algo_launch = st["algo_ops"] / steps
     issued = mfma_launch / (kern_ms * 1e-3) / 1e12
     keys, outs, stream = state["keys"], state["outs"], state["stream"]
+    peak = PEAK_TOPS[rec["mfma"]]
     full = None
     if roofline_steps > 0:
-        dd = Decoder(state["packed"], device=0, dense=True)
+        dd = Decoder(state["packed"], device=0, dense=True, mfma=rec["mfma"])
         dd.evaluate_device(keys[0], outs[0], stream=stream, check_overflow=False)
         torch.cuda.synchronize()
         dd.reset_stats()
@@ -190,28 +193,29 @@ def _roofline(rec, state, steps, warmup, roofline_steps, dev):
         full = {"kernel": "decode_kernel<true> (full shape: every placement row, no early exit)",
                 "basis": "algorithmic int8 ops of the tests, 2*h*w*(H-h+1)*(W-w+1) per test, "
                          "over the launch's HIP-event duration",
-                "achieved": round(achieved, 2), "frac": round(achieved / INT8_PEAK_TOPS, 4),
+                "achieved": round(achieved, 2), "frac": round(achieved / peak, 4),
                 "kernel_ms": round(dense_ms, 4), "algo_ops_per_launch": ds["algo_ops"] / roofline_steps,
                 "issued_mfma_tops": round(ds["mfma_ops"] / roofline_steps / (dense_ms * 1e-3) / 1e12, 2),
                 "traffic": _traffic_from_profile(config, pop, "dense")}
         tot = sum(ds["phase_us"].values()) or 1.0
-        _say("%s full-shape decode %.3f ms/launch, %.1f TOP/s algorithmic (%.1f%% of int8 peak), issued MFMA "
-             "%.1f TOP/s; phases: %s" % (config, dense_ms, achieved, 100 * achieved / INT8_PEAK_TOPS,
+        _say("%s full-shape decode %.3f ms/launch, %.1f TOP/s algorithmic (%.1f%% of the %s peak), issued MFMA "
+             "%.1f TOP/s; phases: %s" % (config, dense_ms, achieved, 100 * achieved / peak, rec["mfma"],
                                          full["issued_mfma_tops"],
                                          ", ".join("%s %.1f%%" % (k, 100 * v / tot)
                                                    for k, v in ds["phase_us"].items())))
-    roof = {"bound": "mfma", "achieved": round(issued, 2), "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
-            "frac": round(issued / INT8_PEAK_TOPS, 4),
+    roof = {"bound": "mfma", "achieved": round(issued, 2), "peak": peak, "unit": "TOP/s",
+            "frac": round(issued / peak, 4), "format": rec["mfma"],
             "traffic": _traffic_from_profile(config, pop, "pruned"),
             "kernel": "decode_kernel<false> (the timed steps' kernel: row pruning, early exit)",
-            "basis": "int8 MFMA ops the kernel issues per launch (32x32x32 tiles x 2) over its HIP-event "
-                     "duration on the launch stream",
+            "basis": "MFMA ops the kernel issues per launch (fp4: 32x32x64 x 2 per instruction, int8: "
+                     "32x32x32 x 2) over its HIP-event duration on the launch stream, against the dense peak "
+                     "of the operand format",
             "kernel_ms": round(kern_ms, 4), "mfma_ops_per_launch": mfma_launch,
             "algo_ops_per_launch": algo_launch,
             "algo_equivalent_tops": round(algo_launch / (kern_ms * 1e-3) / 1e12, 2),
             "full_shape": full}
-    _say("%s timed decode %.3f ms/launch, issued MFMA %.1f TOP/s (%.1f%% of int8 peak), algorithmic-equivalent "
-         "%.1f TOP/s" % (config, kern_ms, issued, 100 * issued / INT8_PEAK_TOPS, roof["algo_equivalent_tops"]))
+    _say("%s timed decode %.3f ms/launch, issued MFMA %.1f TOP/s (%.1f%% of the %s peak), algorithmic-equivalent "
+         "%.1f TOP/s" % (config, kern_ms, issued, 100 * issued / peak, rec["mfma"], roof["algo_equivalent_tops"]))
     return roof
 
 
@@ -425,6 +429,8 @@ def main():
     ap.add_argument("--only-timed", action="store_true",
                     help="stop after the timed steps (for profiler runs: every decode launch is then "
                          "a warm-up or timed one)")
+    ap.add_argument("--mfma", default="fp4", choices=["fp4", "int8"],
+                    help="operand format of the collision correlation's MFMAs (same results)")
     ap.add_argument("--dense", action="store_true",
                     help="with --only-timed: time the full-shape decode instead (every placement row, no "
                          "early exit; profiler runs of the roofline kernel)")
@@ -476,7 +482,7 @@ def main():
     if args.dense and not args.only_timed:
         ap.error("--dense needs --only-timed (the full-shape decode is not the headline)")
     rec, state = _timed_decode(args.config, pop, args.steps, args.warmup, args.seed, dev,
-                               autotune_reps=reps, only_timed=args.only_timed, dense=args.dense)
+                               autotune_reps=reps, only_timed=args.only_timed, dense=args.dense, mfma=args.mfma)
     if args.only_timed:
         state["dec"].close()
         print(json.dumps({"config": rec["workload"], "value": round(rec["value"], 2),
@@ -526,7 +532,7 @@ def main():
     extra = None
     if not args.no_extra and args.config != "C2":
         r2, s2 = _timed_decode("C2", SYNTHETIC_CONFIGS["C2"]["pop"], 50, 5, args.seed, dev,
-                               autotune_reps=3 if reps else 0)
+                               autotune_reps=3 if reps else 0, mfma=args.mfma)
         s2["dec"].close()
         roof2 = _roofline(r2, s2, 50, 5, 1, dev)
         extra = {"workload": r2["workload"], "value": round(r2["value"], 2),
@@ -567,7 +573,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8",
+        "dtype": args.mfma,
         "data": "synthetic",
         "config": {"workload": rec["workload"], "population": pop, "parts": rec["parts"],
                    "machines": rec["machines"]},

@@ -204,18 +204,19 @@ __device__ __forceinline__ v4i bfrag_from_bits(unsigned lo, unsigned hi, int bs)
 // fp4 (e2m1) fragment of the 32x32x64 MFMA: a lane supplies 32 k-values, the
 // 32 cells of one band or mask word, in VGPRs 0-3 (8 nibbles each, low nibble
 // first; VGPRs 4-7 are unused by this format).  As for the int8 fragments, A
-// and B share one bijection between (VGPR v, nibble) and the word's bits: the
-// nibbles of byte j of VGPR v take bits (8j + v, 8j + v + 5) for v < 3 and
-// (8j + 3, 8j + 4) for v = 3, a set cell becoming 0.5 (low nibble, code 0001) or
-// 1.0 (high nibble, code 0010).  Products are 0, 0.25, 0.5 or 1 and a count is
-// at most h * w <= 2^24, so the float32 accumulation is exact and "count == 0"
-// is the same test as with integer counts.
+// and B share one bijection between (VGPR v, nibble i) and the word's bits:
+// VGPR v < 3 keeps bit 4i + v of the word in place (e2m1 codes 1, 2, 4: 0.5,
+// 1.0, 2.0), VGPR 3 takes bit 4i + 3 one place down (code 4; bit 3 of a
+// nibble is the sign).  Three ANDs and one shift-and-AND per fragment.  Every
+// product is 0 or positive, so a float32 sum of them is zero exactly when no
+// mask cell meets an occupied cell, whatever the mask's size: "count == 0" is
+// the same test as with integer counts.
 __device__ __forceinline__ v8i nib_frag(unsigned w) {
     v8i r;
-    r[0] = (int)(w & 0x21212121u);
-    r[1] = (int)((w >> 1) & 0x21212121u);
-    r[2] = (int)((w >> 2) & 0x21212121u);
-    r[3] = (int)(((w >> 3) & 0x01010101u) | ((w << 1) & 0x20202020u));
+    r[0] = (int)(w & 0x11111111u);
+    r[1] = (int)(w & 0x22222222u);
+    r[2] = (int)(w & 0x44444444u);
+    r[3] = (int)((w >> 1) & 0x44444444u);
     r[4] = 0; r[5] = 0; r[6] = 0; r[7] = 0;
     return r;
 }
