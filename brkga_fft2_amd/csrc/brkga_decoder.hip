@@ -883,9 +883,14 @@ __device__ __forceinline__ void load_densities(const DevProblem &P, int rb, int 
 // to decide whether the bin is tried).  The candidate top rows, descending,
 // go to s_cand[r * hmax ...] and their count to s_ncand[r]; the caller
 // synchronises.  Dense mode lists every row of the valid region (same results).
+// Rows above the bin's top-most occupied row (minrow) are empty, so a top row
+// y < minrow - h + 1 keeps the whole part in empty rows and always passes: only
+// the top rows from there on are tested, and the others follow them in the
+// list (they are the highest, so the list stays descending).
 template <bool DENSE>
-__device__ __forceinline__ void vacancy_filter(const DevProblem &P, int rb, int nr, int H, int W, const short *vrow,
-                                               const int *s_den, short *s_cand, int hmax, int *s_ncand) {
+__device__ __forceinline__ void vacancy_filter(const DevProblem &P, int rb, int nr, int H, int W, int minrow,
+                                               const short *vrow, const int *s_den, short *s_cand, int hmax,
+                                               int *s_ncand) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (wave < nr) {
         const int rg = rb + wave;
@@ -897,13 +902,14 @@ __device__ __forceinline__ void vacancy_filter(const DevProblem &P, int rb, int 
         int nc_ = 0;
         if (h <= H && w <= W) {
             const int ny = H - h + 1;
-            const int nk = (ny + 63) >> 6;
+            const int yf = DENSE ? 0 : min(ny, max(0, uni(minrow) - h + 1));   // top rows below yf pass
+            const int nk = (ny - yf + 63) >> 6;
             lds_cs16 *vr = (lds_cs16 *)vrow;
             lds_ci32 *dn = (lds_ci32 *)den;
             // chunks of at most 8 words (512 top rows), the highest chunk first
             // so that the list stays descending; one chunk when H - h < 512
             for (int c = (nk - 1) >> 3; c >= 0; c--) {
-                const int y0 = c << 9, nkc = min(8, nk - (c << 3));
+                const int y0 = yf + (c << 9), nkc = min(8, nk - (c << 3));
                 short *cc = cand + nc_;
                 switch (nkc) {
                     case 1: nc_ += vacancy_rotation<1>(vr, dn, h, ny - y0, cc, y0); break;
@@ -916,6 +922,8 @@ __device__ __forceinline__ void vacancy_filter(const DevProblem &P, int rb, int 
                     default: nc_ += vacancy_rotation<8>(vr, dn, h, ny - y0, cc, y0); break;
                 }
             }
+            for (int k = lane; k < yf; k += 64) cand[nc_ + k] = (short)(yf - 1 - k);
+            nc_ += yf;
             if (DENSE && nc_ > 0) {   // full valid region, bottom-up
                 nc_ = H - h + 1;
                 for (int k = lane; k < nc_; k += 64) cand[k] = (short)(H - h - k);
@@ -1177,7 +1185,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                     __syncthreads();
                 }
                 // ---- vacancy filter, per candidate top row (wave r: rotation r) ----
-                vacancy_filter<DENSE>(P, rb, nr, H, W, vrow, s_den, s_cand, hmax, s_ncand);
+                vacancy_filter<DENSE>(P, rb, nr, H, W, b_minrow[b], vrow, s_den, s_cand, hmax, s_ncand);
                 __syncthreads();
                 if (tid == 0) ph[1] += tick() - tp;
                 // ---- collision tests, one rotation at a time (whole workgroup) ----
@@ -1313,7 +1321,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P
     load_densities(P, rb, nr, s_den);
     for (int y = tid; y < H; y += WG_THREADS) s_vac[y] = (short)vac[y];
     __syncthreads();
-    vacancy_filter<false>(P, rb, nr, H, W, s_vac, s_den, s_cand, hmax, s_ncand);
+    vacancy_filter<false>(P, rb, nr, H, W, H - enc, s_vac, s_den, s_cand, hmax, s_ncand);
     __syncthreads();
     unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int bx = 0, by = 0;
