@@ -190,9 +190,10 @@ def _roofline(rec, state, steps, warmup, roofline_steps, dev):
         dd.close()
         dense_ms = ds["kernel_ms"] / roofline_steps
         achieved = ds["algo_ops"] / roofline_steps / (dense_ms * 1e-3) / 1e12
-        full = {"kernel": "decode_kernel<true> (full shape: every placement row, no early exit)",
-                "basis": "algorithmic int8 ops of the tests, 2*h*w*(H-h+1)*(W-w+1) per test, "
-                         "over the launch's HIP-event duration",
+        full = {"kernel": "decode_kernel<true, %s> (full shape: every placement row, no early exit)"
+                          % ("true" if rec["mfma"] == "fp4" else "false"),
+                "basis": "algorithmic ops of the tests, 2*h*w*(H-h+1)*(W-w+1) per test (the dense valid-region "
+                         "correlation), over the launch's HIP-event duration, against the operand format's peak",
                 "achieved": round(achieved, 2), "frac": round(achieved / peak, 4),
                 "kernel_ms": round(dense_ms, 4), "algo_ops_per_launch": ds["algo_ops"] / roofline_steps,
                 "issued_mfma_tops": round(ds["mfma_ops"] / roofline_steps / (dense_ms * 1e-3) / 1e12, 2),
@@ -206,7 +207,8 @@ def _roofline(rec, state, steps, warmup, roofline_steps, dev):
     roof = {"bound": "mfma", "achieved": round(issued, 2), "peak": peak, "unit": "TOP/s",
             "frac": round(issued / peak, 4), "format": rec["mfma"],
             "traffic": _traffic_from_profile(config, pop, "pruned"),
-            "kernel": "decode_kernel<false> (the timed steps' kernel: row pruning, early exit)",
+            "kernel": "decode_kernel<false, %s> (the timed steps' kernel: row pruning, early exit)"
+                      % ("true" if rec["mfma"] == "fp4" else "false"),
             "basis": "MFMA ops the kernel issues per launch (fp4: 32x32x64 x 2 per instruction, int8: "
                      "32x32x32 x 2) over its HIP-event duration on the launch stream, against the dense peak "
                      "of the operand format",
