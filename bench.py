@@ -442,9 +442,11 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the extra C2 record when the headline is another configuration")
     ap.add_argument("--no-ttt", action="store_true", help="skip the time-to-target comparisons")
-    ap.add_argument("--ttt-configs", default="C2,C3", help="instances of the time-to-target comparisons")
+    ap.add_argument("--ttt-configs", default="C2,C3,C5", help="instances of the time-to-target comparisons")
     ap.add_argument("--ttt-generations", type=int, default=100)
     ap.add_argument("--ttt-seeds", type=int, default=10)
+    ap.add_argument("--ttt-c5-generations", type=int, default=100)
+    ap.add_argument("--ttt-c5-seeds", type=int, default=3)
     ap.add_argument("--ttt-out", default="", help="write the full time-to-target records (JSON) under this prefix")
     ap.add_argument("--autotune", type=int, default=-1, metavar="REPS",
                     help="tune the tile table before the timed steps with REPS timings per trial "
@@ -561,7 +563,9 @@ def main():
     if not args.no_ttt:
         ttt = {}
         for c in [c for c in args.ttt_configs.split(",") if c]:
-            ttt[c] = _time_to_target(c, args.seed, args.ttt_generations, args.ttt_seeds,
+            big = c == "C5"   # a C5 generation decodes 1024 chromosomes of 200 parts: fewer, shorter runs
+            ttt[c] = _time_to_target(c, args.seed, args.ttt_c5_generations if big else args.ttt_generations,
+                                     args.ttt_c5_seeds if big else args.ttt_seeds,
                                      (args.ttt_out + "_%s.json" % c) if args.ttt_out else "")
 
     line = {

@@ -100,7 +100,7 @@ struct DevProblem {
     const int *rot_nc, *rot_nc4;   // Toeplitz chunks of 32 / 64 columns: ceil((w + 31) / 32), ceil((w + 31) / 64)
     const long long *mask_off, *dens_off, *qbits_off;
     const unsigned char *masks;
-    const int *dens;
+    const int *dens;         // per rotation: (longest 1-run << 16) | mask row, densest rows first
     const unsigned *qbits;   // per rotation: h rows of 2 nc4 + 2 dwords, mask bits after 32 zero bits, zero padded
     const int *plate_h, *plate_w;
     const double *setup, *proc_time, *pth;
@@ -813,8 +813,11 @@ __global__ __launch_bounds__(ORDER_BUCKETS) void order_scatter_kernel(const int 
 // wave: bit l of word k says top row y = y0 + 64k + l passes, i.e. every grid
 // row y+a (a < h) has a free run >= the mask row's longest 1-run
 // (numba_utils.py:131-157 applies the same test to decide whether the bin is
-// tried).  For mask row a, lane l tests grid row y0 + 64k + a + l, so the
-// ballot is already aligned with the candidate word.  Four mask rows per step
+// tried).  den holds (density << 16) | a for the rotation's mask rows a, the
+// densest first, so that a bin with no room fails within the first steps (the
+// result is an AND over the rows, whatever their order).  For mask row a,
+// lane l tests grid row y0 + 64k + a + l, so the ballot is already aligned
+// with the candidate word.  Four mask rows per step
 // keep 4 x NK reads in flight; the NK <= 8 words are all indexed by constants
 // so that they stay in scalar registers and the loop exit stays uniform.
 // ny: valid top rows from y0 on (H - h + 1 - y0).  Lanes whose top row is
@@ -836,8 +839,9 @@ __device__ __forceinline__ int vacancy_rotation(lds_cs16 *vrow, lds_ci32 *den, i
         int aj[4], d[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            aj[j] = min(a + j, h - 1);
-            d[j] = __builtin_amdgcn_readfirstlane(den[aj[j]]);
+            const int e = __builtin_amdgcn_readfirstlane(den[min(a + j, h - 1)]);
+            d[j] = e >> 16;
+            aj[j] = e & 0xffff;
         }
 #pragma unroll
         for (int j = 0; j < 4; j++)
@@ -864,7 +868,8 @@ __device__ __forceinline__ int vacancy_rotation(lds_cs16 *vrow, lds_ci32 *den, i
     return nc_;
 }
 
-// Densities (longest 1-run per mask row) of rotations rb..rb+nr-1 into
+// Densities (longest 1-run per mask row, packed with the row as
+// (density << 16) | row, densest rows first) of rotations rb..rb+nr-1 into
 // s_den, rotation after rotation (rotation r at the sum of the heights of
 // rotations rb..rb+r-1).  brkga_create appends the densities in rotation
 // order and a type's rotations are consecutive, so this is one contiguous
@@ -1571,10 +1576,15 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
         const int rh = p->rot_h[r], rw = p->rot_w[r];
         const uint8_t *mk = p->masks + p->mask_off[r];
         dens_off[r] = (long long)dens.size();
-        for (int a = 0; a < rh; a++) {
-            int best = 0, cur = 0;
-            for (int b = 0; b < rw; b++) { cur = mk[a * rw + b] ? cur + 1 : 0; best = std::max(best, cur); }
-            dens.push_back(best);
+        {   // (density << 16) | row, densest rows first: the filter's early exit comes sooner
+            std::vector<int> dr(rh);
+            for (int a = 0; a < rh; a++) {
+                int best = 0, cur = 0;
+                for (int b = 0; b < rw; b++) { cur = mk[a * rw + b] ? cur + 1 : 0; best = std::max(best, cur); }
+                dr[a] = (best << 16) | a;   // best <= rw <= 4096 and a < rh <= 4096
+            }
+            std::stable_sort(dr.begin(), dr.end(), [](int x, int y) { return (x >> 16) > (y >> 16); });
+            dens.insert(dens.end(), dr.begin(), dr.end());
         }
         nc[r] = (rw + 31 + 31) / 32;    // 32-column chunks a 32-placement tile spans (int8)
         nc4[r] = (rw + 31 + 63) / 64;   // 64-column chunks (fp4)

@@ -157,8 +157,6 @@ __global__ void splitk_reduce_kernel(const float *ws, int S, int M, int N, float
     C[(t / N) * ldc + t % N] = v;
 }
 
-// Features of each chromosome (one workgroup per row): keys, then area and
-// height of the parts ordered by ascending sequence key (ties by slot).
 // Machine of a machine key: the first threshold bucket holding it (placement.py:121-127).
 __device__ __forceinline__ int machine_bucket(double mv, const double *thr, int M) {
     if (M <= 1 || mv <= thr[0]) return 0;
@@ -167,26 +165,55 @@ __device__ __forceinline__ int machine_bucket(double mv, const double *thr, int 
     return M - 1;
 }
 
+// Features of each chromosome (one workgroup per row).  Slot features (4n):
+// the keys as the decoder reads them (sequence position / n, machine bucket /
+// max(M-1, 1)), then area and height of the parts ordered by ascending
+// sequence key (ties by slot), each times ss.  With per-machine terms mt
+// ([M][3][n]) four aggregates per machine m follow: the sums, in slot order,
+// of mt[m][0][j] and mt[m][1][j] over the slots j of machine m, their count
+// times M / n, and the largest mt[m][2][j] among them.  Columns F..Fp-1 are 0.
 __global__ __launch_bounds__(256) void featurize_kernel(const float *keys, int B, int n, int M,
-                                                        const double *thr, int Fp, const float *slot_area,
-                                                        const float *slot_height, bf16_t *x) {
-    extern __shared__ float sk[];   // [n]
+                                                        const double *thr, int F, int Fp, float ss,
+                                                        const float *slot_area, const float *slot_height,
+                                                        const float *mt, bf16_t *x) {
+    extern __shared__ float sk[];   // [n] sequence keys, then [n] machine buckets
+    int *sb = (int *)(sk + n);
     const long long b = blockIdx.x;
     const float *kr = keys + b * 2LL * n;
     bf16_t *xr = x + b * Fp;
-    for (int j = threadIdx.x; j < n; j += 256) sk[j] = kr[j];
-    for (int j = 4 * n + threadIdx.x; j < Fp; j += 256) xr[j] = 0;
+    for (int j = threadIdx.x; j < n; j += 256) {
+        sk[j] = kr[j];
+        sb[j] = machine_bucket((double)kr[n + j], thr, M);
+    }
+    for (int j = F + threadIdx.x; j < Fp; j += 256) xr[j] = 0;
     __syncthreads();
     const float inv_den = M > 1 ? (float)(M - 1) : 1.0f;
     for (int j = threadIdx.x; j < n; j += 256) {
         const float kj = sk[j];
         int rank = 0;
         for (int i = 0; i < n; i++) rank += (sk[i] < kj) || (sk[i] == kj && i < j);
-        // the keys as the decoder reads them: sequence position and machine bucket
-        xr[j] = f2bf((float)rank / (float)n);
-        xr[n + j] = f2bf((float)machine_bucket((double)kr[n + j], thr, M) / inv_den);
-        xr[2 * n + rank] = f2bf(slot_area[j]);
-        xr[3 * n + rank] = f2bf(slot_height[j]);
+        xr[j] = f2bf(((float)rank / (float)n) * ss);
+        xr[n + j] = f2bf(((float)sb[j] / inv_den) * ss);
+        xr[2 * n + rank] = f2bf(slot_area[j] * ss);
+        xr[3 * n + rank] = f2bf(slot_height[j] * ss);
+    }
+    if (mt && (int)threadIdx.x < M) {   // one thread per machine, slots in order
+        const int m = threadIdx.x;
+        const float *t0 = mt + (long long)m * 3 * n, *t1 = t0 + n, *t2 = t1 + n;
+        float s0 = 0.0f, s1 = 0.0f, mx = 0.0f;
+        int cnt = 0;
+        for (int j = 0; j < n; j++)
+            if (sb[j] == m) {
+                s0 += t0[j];
+                s1 += t1[j];
+                mx = fmaxf(mx, t2[j]);
+                cnt++;
+            }
+        bf16_t *xa = xr + 4 * n + 4 * m;
+        xa[0] = f2bf(s0);
+        xa[1] = f2bf(s1);
+        xa[2] = f2bf((float)(cnt * M) / (float)n);
+        xa[3] = f2bf(mx);
     }
     (void)B;
 }
@@ -353,6 +380,8 @@ struct brkga_mlp {
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     bf16_t *pb = nullptr;   // bf16 copy of every parameter (GEMMs read W1, W2)
     float *slot_area = nullptr, *slot_height = nullptr;
+    float *mterms = nullptr;   // [M][3][n] per-machine terms (null: slot features only)
+    float ss = 1.0f;           // slot feature scale
     // activations (grown on demand)
     long long cap = 0;
     bf16_t *x = nullptr, *H1 = nullptr, *H2 = nullptr, *dH2 = nullptr, *dH1 = nullptr;
@@ -489,11 +518,12 @@ static int backward(brkga_mlp *net, long long B, hipStream_t s) {
 
 extern "C" int brkga_mlp_create(int32_t n_parts, int32_t n_machines, const double *thresholds,
                                 int32_t hidden, const float *slot_area, const float *slot_height,
-                                uint64_t seed, float lr, int32_t device, brkga_mlp **out) {
-    if (!out || n_parts < 1 || n_machines < 1 || (n_machines > 1 && !thresholds) || hidden < 32 ||
-        hidden % 32 || !slot_area || !slot_height) {
-        g_mlp_error = "bad argument (n_parts >= 1, n_machines >= 1 with n_machines-1 thresholds, "
-                      "hidden a positive multiple of 32, feature arrays)";
+                                const float *machine_terms, uint64_t seed, float lr, int32_t device,
+                                brkga_mlp **out) {
+    if (!out || n_parts < 1 || n_parts > 8192 || n_machines < 1 || n_machines > 256 ||
+        (n_machines > 1 && !thresholds) || hidden < 32 || hidden % 32 || !slot_area || !slot_height) {
+        g_mlp_error = "bad argument (1 <= n_parts <= 8192, 1 <= n_machines <= 256 with n_machines-1 "
+                      "thresholds, hidden a positive multiple of 32, feature arrays)";
         return BRKGA_EINVAL;
     }
     for (int m = 1; m + 1 < n_machines; m++)
@@ -505,7 +535,11 @@ extern "C" int brkga_mlp_create(int32_t n_parts, int32_t n_machines, const doubl
     net->device = device;
     net->n = n_parts;
     net->M = n_machines;
-    net->F = 4 * n_parts;
+    // with per-machine aggregates the slot features are scaled down by 8: the
+    // network then leans on the aggregates first and overfits the 4n slot
+    // columns less (measured on 4-machine instances)
+    net->F = 4 * n_parts + (machine_terms ? 4 * n_machines : 0);
+    net->ss = machine_terms ? 0.125f : 1.0f;
     net->Fp = (net->F + GEMM_KC - 1) / GEMM_KC * GEMM_KC;
     net->Hd = hidden;
     net->lr = lr > 0 ? lr : 1e-3f;
@@ -537,6 +571,15 @@ extern "C" int brkga_mlp_create(int32_t n_parts, int32_t n_machines, const doubl
     if ((rc = mlp_alloc(net, (size_t)net->np_ * 2, &q))) return bail(rc); net->pb = (bf16_t *)q;
     if ((rc = mlp_alloc(net, (size_t)n_parts * 4, &q))) return bail(rc); net->slot_area = (float *)q;
     if ((rc = mlp_alloc(net, (size_t)n_parts * 4, &q))) return bail(rc); net->slot_height = (float *)q;
+    if (machine_terms) {
+        if ((rc = mlp_alloc(net, (size_t)n_machines * 3 * n_parts * 4, &q))) return bail(rc);
+        net->mterms = (float *)q;
+        if (hipMemcpy(net->mterms, machine_terms, (size_t)n_machines * 3 * n_parts * 4, hipMemcpyHostToDevice) !=
+            hipSuccess) {
+            g_mlp_error = "device initialisation failed";
+            return bail(BRKGA_EDEVICE);
+        }
+    }
     if ((rc = mlp_alloc(net, 16, &q))) return bail(rc); net->loss = (float *)q;
     if ((rc = mlp_alloc(net, (size_t)GEMM_SPLIT_MAX * std::max(Fp, Hd) * Hd * 4, &q))) return bail(rc);
     net->ws = (float *)q;
@@ -563,8 +606,9 @@ extern "C" int brkga_mlp_create(int32_t n_parts, int32_t n_machines, const doubl
 static int mlp_features(brkga_mlp *net, const float *d_keys, long long B, hipStream_t s) {
     int rc = mlp_reserve(net, B);
     if (rc) return rc;
-    hipLaunchKernelGGL(featurize_kernel, dim3((unsigned)B), dim3(256), (size_t)net->n * 4, s, d_keys, (int)B, net->n,
-                       net->M, net->thr, net->Fp, net->slot_area, net->slot_height, net->x);
+    hipLaunchKernelGGL(featurize_kernel, dim3((unsigned)B), dim3(256), (size_t)net->n * 8, s, d_keys, (int)B, net->n,
+                       net->M, net->thr, net->F, net->Fp, net->ss, net->slot_area, net->slot_height, net->mterms,
+                       net->x);
     MLPCHK(hipGetLastError());
     return 0;
 }

@@ -1,8 +1,13 @@
 """Surrogate-screened BRKGA: a BF16 MLP ranks oversized offspring pools.
 
-The network (``brkga_surrogate.hip``: 4n -> 512 -> 512 -> 1, BF16 weights and
-activations, FP32 accumulation, Adam) is trained online on the decoder's
-exact makespans.  Each generation after a warm-up:
+The network (``brkga_surrogate.hip``: 4n + 4M -> 512 -> 512 -> 1, BF16
+weights and activations, FP32 accumulation, Adam) is trained online on the
+decoder's exact makespans.  Its inputs are the keys as the decoder reads them
+and the parts' areas and heights in sequence order (4n, scaled by 1/8), and
+per machine the processing time, plate-relative area, count and tallest part
+the chromosome sends to it (4M): a machine's makespan is mostly its load plus
+setup per batch, so on several machines these aggregates carry most of the
+ranking signal.  Each generation after a warm-up:
 
 1. the population is sorted by exact fitness (``brkga_sort_fitness``);
 2. a pool of ``pool`` candidates is bred -- mutants and biased-crossover
@@ -34,7 +39,7 @@ _P = ctypes.c_void_p
 def _lib():
     lib = load_library()
     if not hasattr(lib, "_mlp_bound"):
-        lib.brkga_mlp_create.argtypes = [ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int32, _P, _P,
+        lib.brkga_mlp_create.argtypes = [ctypes.c_int32, ctypes.c_int32, _P, ctypes.c_int32, _P, _P, _P,
                                          ctypes.c_uint64, ctypes.c_float, ctypes.c_int32, ctypes.POINTER(_P)]
         lib.brkga_mlp_predict.argtypes = [_P, _P, ctypes.c_int64, _P, _P]
         lib.brkga_mlp_gradients.argtypes = [_P, _P, _P, ctypes.c_int64, _P, _P]
@@ -70,6 +75,30 @@ def slot_features(problem, instance_parts, machine=0):
             (height / max(height.max(), 1e-300)).astype(np.float32))
 
 
+def machine_terms(problem, instance_parts, n_machines):
+    """Per-machine terms of the aggregate features, float32 ``[M][3][n]``: the
+    processing time of slot j's part on machine m, its area over m's plate
+    area, and its height-driven processing time on m.  The first two are
+    scaled so that an even split of the parts over the machines sums to about
+    1 per machine, the third by its largest value.  A machine's makespan is the
+    sum over its batches of processing time, setup and the batch's largest
+    height-driven time (``placement.py:86-89``), so its load and its area
+    relative to the plate (about the number of batches it needs) are what a
+    chromosome's makespan mostly depends on."""
+    parts = [int(p) for p in instance_parts]
+    M = int(n_machines)
+    pt = np.array([[problem.machines[m].parts[p].proc_time for p in parts] for m in range(M)], np.float64)
+    pth = np.array([[problem.machines[m].parts[p].proc_time_height for p in parts] for m in range(M)],
+                   np.float64)
+    area = np.array([problem.parts[p].area for p in parts], np.float64)
+    plate = np.array([problem.machines[m].bin_area for m in range(M)], np.float64)
+    t = np.zeros((M, 3, len(parts)), np.float64)
+    t[:, 0] = pt / max(pt.mean(axis=0).sum() / M, 1e-300)
+    t[:, 1] = (area[None, :] / plate[:, None]) / max((area.sum() / plate.mean()) / M, 1e-300)
+    t[:, 2] = pth / max(pth.max(), 1e-300)
+    return t.astype(np.float32)
+
+
 def _stream_ptr(stream):
     return ctypes.c_void_p(stream.cuda_stream)
 
@@ -78,18 +107,24 @@ class MLPSurrogate:
     """ctypes handle of one surrogate network on one GPU."""
 
     def __init__(self, n_parts, slot_area, slot_height, thresholds=(), hidden=512, seed=0, lr=1e-3,
-                 device=0):
+                 device=0, machine_terms=None):
         self._lib = _lib()
         sa = np.ascontiguousarray(slot_area, np.float32)
         sh = np.ascontiguousarray(slot_height, np.float32)
         thr = np.ascontiguousarray(thresholds, np.float64).reshape(-1)
         self.thresholds = thr.copy()
         self.n_machines = thr.size + 1
+        mt = None
+        if machine_terms is not None:
+            mt = np.ascontiguousarray(machine_terms, np.float32)
+            if mt.shape != (self.n_machines, 3, int(n_parts)):
+                raise ValueError("machine_terms must be [n_machines][3][n_parts]")
+        self.machine_terms = mt
         h = _P()
         rc = self._lib.brkga_mlp_create(int(n_parts), int(self.n_machines),
                                         thr.ctypes.data if thr.size else None, int(hidden),
-                                        sa.ctypes.data, sh.ctypes.data, int(seed), float(lr), int(device),
-                                        ctypes.byref(h))
+                                        sa.ctypes.data, sh.ctypes.data, mt.ctypes.data if mt is not None else None,
+                                        int(seed), float(lr), int(device), ctypes.byref(h))
         if rc != 0:
             raise DecoderError(rc, "brkga_mlp_create: " + self._lib.brkga_mlp_last_error().decode())
         self._h = h
@@ -189,7 +224,7 @@ class SurrogateBRKGA:
     def __init__(self, decoder, problem, instance_parts, num_individuals=1280, elite_frac=0.2,
                  mutant_frac=0.15, elite_prob=0.7, pool=4096, warmup=5, seed=0,
                  use_surrogate=True, hidden=512, lr=1e-3, train_steps=16, batch=1024,
-                 buffer=16384, fallback_rho=0.2, audit_every=0, target="rank"):
+                 buffer=16384, fallback_rho=0.2, audit_every=0, target="rank", aggregates=True):
         import torch
         self.torch = torch
         self.dec = decoder
@@ -216,8 +251,9 @@ class SurrogateBRKGA:
         self.stream = torch.cuda.current_stream(self.dev)
         self._lib = _lib()
         sa, sh = slot_features(problem, instance_parts)
+        mt = machine_terms(problem, instance_parts, decoder.n_machines) if aggregates else None
         self.net = MLPSurrogate(self.n, sa, sh, decoder.thresholds, hidden=hidden, seed=seed, lr=lr,
-                                device=decoder.device) if self.use_surrogate else None
+                                device=decoder.device, machine_terms=mt) if self.use_surrogate else None
         g = torch.Generator(device="cpu").manual_seed(self.seed)
         self.pop = torch.rand((self.N, self.G2), generator=g, dtype=torch.float32).to(self.dev)
         self.fit = torch.empty(self.N, dtype=torch.float64, device=self.dev)
@@ -401,7 +437,9 @@ def time_to_target(decoder, problem, instance_parts, generations=100, seeds=10, 
     w.close()
     torch.cuda.synchronize()
     runs = {"plain": [], "surrogate": []}
+    dec_stats = {}
     for kind in runs:
+        s0 = decoder.stats()
         for seed in range(seeds):
             ga = SurrogateBRKGA(decoder, problem, instance_parts, seed=seed,
                                 use_surrogate=(kind == "surrogate"), **kw)
@@ -410,6 +448,8 @@ def time_to_target(decoder, problem, instance_parts, generations=100, seeds=10, 
                                "spearman": list(ga.log["spearman"]),
                                "screened": list(ga.log["screened"])})
             ga.close()
+        s1 = decoder.stats()
+        dec_stats[kind] = {k: s1[k] - s0[k] for k in ("kernel_ms", "mfma_ops", "contexts")}
     half = max(1, generations // 2)
     target = float(np.median([r["best"][half - 1] for r in runs["plain"][:target_seeds]]))
 
@@ -418,7 +458,7 @@ def time_to_target(decoder, problem, instance_parts, generations=100, seeds=10, 
         hit = np.nonzero(b <= target)[0]
         return float(r["time"][hit[0]]) if hit.size else float("inf")
 
-    out = {"target": target, "generations": generations, "seeds": seeds, "target_seeds": target_seeds}
+    out = {"target": target, "generations": generations, "seeds": seeds, "target_seeds": min(target_seeds, seeds)}
     for kind, rs in runs.items():
         t = [ttt(r) for r in rs]
         med = float(np.median(t))
@@ -427,6 +467,10 @@ def time_to_target(decoder, problem, instance_parts, generations=100, seeds=10, 
                      "reached": int(np.sum(np.isfinite(t))),
                      "ms_per_generation": 1e3 * float(np.mean([r["time"][-1] / generations for r in rs])),
                      "final_best_mean": float(np.mean([r["best"][-1] for r in rs]))}
+        ds = dec_stats[kind]
+        # the decode kernel's share of a generation and its issued MFMA rate
+        out[kind]["decode_kernel_ms_per_generation"] = ds["kernel_ms"] / max(1, seeds * generations)
+        out[kind]["decode_issued_mfma_tops"] = ds["mfma_ops"] / max(ds["kernel_ms"], 1e-9) / 1e9
     sp = [x for r in runs["surrogate"] for x, sc in zip(r["spearman"], r["screened"])
           if sc and math.isfinite(x)]
     out["surrogate"]["spearman_decoded_mean"] = float(np.mean(sp)) if sp else None

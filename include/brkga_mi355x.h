@@ -273,20 +273,30 @@ int brkga_breed_pool(const float *d_population, const int32_t *d_order, int32_t 
  * Surrogate: BF16 MLP with FP32 accumulation predicting a chromosome's
  * (normalised) makespan, trained online on exact decodes.  No counterpart in
  * the reference; it only ranks oversized offspring pools so that fewer
- * chromosomes need an exact decode.  Input features of a chromosome (4n): its
- * 2n keys as the decoder reads them -- the sequence position of each slot
- * divided by n, and the machine bucket of each slot's machine key
- * (thresholds as in brkga_problem) divided by max(n_machines-1, 1) -- then
+ * chromosomes need an exact decode.  Input features of a chromosome: 4n slot
+ * features -- its 2n keys as the decoder reads them (the sequence position of
+ * each slot divided by n, and the machine bucket of each slot's machine key,
+ * thresholds as in brkga_problem, divided by max(n_machines-1, 1)), then
  * slot_area and slot_height of its parts in ascending sequence-key order
- * (ties by slot).  Layers: 4n -> hidden -> hidden -> 1, ReLU.  Parameters
- * (float32 master copy) are laid out W1 [Fp][hidden] (rows >= 4n are zero),
- * b1 [hidden], W2 [hidden][hidden], b2 [hidden], w3 [hidden], b3 [1], with Fp
- * = 4n rounded up to 32.  d_* pointers are device memory.
+ * (ties by slot).  With machine_terms ([n_machines][3][n_parts] float32,
+ * nullable) the slot features are multiplied by 1/8 and 4 per machine m
+ * follow: the sums, in slot order, of machine_terms[m][0][j] and
+ * machine_terms[m][1][j] over the slots j the chromosome sends to machine m,
+ * their count times n_machines / n_parts, and the largest
+ * machine_terms[m][2][j] among them (0 for none); the Python layer passes the
+ * processing time, the area over the plate area and the height-driven
+ * processing time of each part on each machine, scaled to O(1).  Layers: F ->
+ * hidden -> hidden -> 1, ReLU, F = 4n (+ 4 n_machines).  Parameters (float32
+ * master copy) are laid out W1 [Fp][hidden] (rows >= F are zero), b1 [hidden],
+ * W2 [hidden][hidden], b2 [hidden], w3 [hidden], b3 [1], with Fp = F rounded
+ * up to 64.  1 <= n_parts <= 8192, n_machines <= 256.  d_* pointers are device
+ * memory.
  */
 typedef struct brkga_mlp brkga_mlp;
 int brkga_mlp_create(int32_t n_parts, int32_t n_machines, const double *thresholds,
                      int32_t hidden, const float *slot_area, const float *slot_height,
-                     uint64_t seed, float learning_rate, int32_t device, brkga_mlp **out);
+                     const float *machine_terms, uint64_t seed, float learning_rate, int32_t device,
+                     brkga_mlp **out);
 /* d_keys [B][2n] float32 -> d_pred [B] float32 */
 int brkga_mlp_predict(brkga_mlp *net, const float *d_keys, int64_t B, float *d_pred, void *stream);
 /* Mean-squared-error gradient of one batch into d_grad [n_params] (nullable). */

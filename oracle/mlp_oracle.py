@@ -36,23 +36,38 @@ def machine_bucket(mv, thresholds):
     return np.searchsorted(thr, mv, side="left")
 
 
-def features(keys, slot_area, slot_height, fp, thresholds=()):
+def features(keys, slot_area, slot_height, fp, thresholds=(), machine_terms=None):
     """[B, fp] features: sequence position / n and machine bucket / max(M-1, 1)
-    per slot, then area/height in ascending sequence-key order (ties by slot)."""
+    per slot, then area/height in ascending sequence-key order (ties by slot).
+    With ``machine_terms`` ([M][3][n] float32) those 4n columns are scaled by
+    1/8 and four aggregates per machine m follow: float32 sums, in slot order,
+    of terms 0 and 1 over the slots in bucket m, their count times M / n, and
+    the largest term 2 among them (0 for none)."""
     keys = np.asarray(keys, dtype=np.float32)
     B, g2 = keys.shape
     n = g2 // 2
     M = np.asarray(thresholds).size + 1
+    mt = None if machine_terms is None else np.asarray(machine_terms, np.float32)
+    ss = np.float32(0.125 if mt is not None else 1.0)
     x = np.zeros((B, fp), np.float64)
     for b in range(B):
         order = np.lexsort((np.arange(n), keys[b, :n]))   # stable by slot on ties
         rank = np.empty(n, np.int64)
         rank[order] = np.arange(n)
-        x[b, :n] = bf16(rank.astype(np.float32) / np.float32(n))
+        x[b, :n] = bf16((rank.astype(np.float32) / np.float32(n)) * ss)
         mach = machine_bucket(keys[b, n:].astype(np.float64), thresholds)
-        x[b, n:2 * n] = bf16(mach.astype(np.float32) / np.float32(max(M - 1, 1)))
-        x[b, 2 * n:3 * n] = bf16(np.asarray(slot_area, np.float32)[order])
-        x[b, 3 * n:4 * n] = bf16(np.asarray(slot_height, np.float32)[order])
+        x[b, n:2 * n] = bf16((mach.astype(np.float32) / np.float32(max(M - 1, 1))) * ss)
+        x[b, 2 * n:3 * n] = bf16(np.asarray(slot_area, np.float32)[order] * ss)
+        x[b, 3 * n:4 * n] = bf16(np.asarray(slot_height, np.float32)[order] * ss)
+        if mt is not None:
+            for m in range(M):
+                on = mach == m
+                # running float32 sums (cumsum adds in slot order; a 0 term adds nothing)
+                s0 = np.cumsum(np.where(on, mt[m, 0], np.float32(0)), dtype=np.float32)[-1]
+                s1 = np.cumsum(np.where(on, mt[m, 1], np.float32(0)), dtype=np.float32)[-1]
+                mx = mt[m, 2][on].max() if on.any() else np.float32(0)
+                cnt = np.float32(int(on.sum()) * M) / np.float32(n)
+                x[b, 4 * n + 4 * m:4 * n + 4 * m + 4] = bf16(np.array([s0, s1, cnt, mx], np.float32))
     return x
 
 

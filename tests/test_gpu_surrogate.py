@@ -19,30 +19,39 @@ def _say(msg):
 THR = np.array([1.0 / 3.0, 2.0 / 3.0])   # three machines
 
 
-def _net(n=50, hidden=512, seed=0):
+def _net(n=50, hidden=512, seed=0, terms=False):
     from brkga_fft2_amd.surrogate import MLPSurrogate
     rng = np.random.default_rng(seed)
     area = rng.random(n).astype(np.float32)
     height = rng.random(n).astype(np.float32)
-    return MLPSurrogate(n, area, height, THR, hidden=hidden, seed=seed), area, height
+    mt = (rng.random((THR.size + 1, 3, n)) * 3.0 / n).astype(np.float32) if terms else None
+    net = MLPSurrogate(n, area, height, THR, hidden=hidden, seed=seed, machine_terms=mt)
+    return net, area, height
 
 
-def test_features_bit_exact():
-    _say("mlp features")
+@pytest.mark.parametrize("terms", [False, True])
+def test_features_bit_exact(terms):
+    _say("mlp features terms=%s" % terms)
     import torch
-    net, area, height = _net()
+    net, area, height = _net(terms=terms)
+    assert net.fp == 256 and (net.machine_terms is not None) == terms   # 4n (+ 4M) rounded up to 64
     keys = np.random.default_rng(1).random((37, 100)).astype(np.float32)
     keys[:, 50:52] = THR.astype(np.float32)   # near the thresholds
     keys[3, :50] = keys[3, 7]                  # all sequence keys tied
+    keys[5, 50:] = 0.9                         # every slot on the last machine: the others get zeros
     x = net.features(torch.from_numpy(keys).cuda()).cpu().numpy().view(np.uint16)
     got = (x.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
-    want = mo.features(keys, area, height, net.fp, THR)
+    want = mo.features(keys, area, height, net.fp, THR, machine_terms=net.machine_terms)
     np.testing.assert_array_equal(got, want)
+    if terms:
+        assert np.all(got[5, 200:208] == 0) and got[5, 210] == 3.0   # count 50 * 3 / 50
+        assert np.all(got[:, 212:] == 0)
     net.close()
 
 
-@pytest.mark.parametrize("B,same_sign", [(64, False), (200, False), (333, False), (1024, False), (1024, True)])
-def test_forward_and_gradients_match_oracle(B, same_sign):
+@pytest.mark.parametrize("B,same_sign,terms", [(64, False, False), (200, False, True), (333, False, False),
+                                               (1024, False, True), (1024, True, False)])
+def test_forward_and_gradients_match_oracle(B, same_sign, terms):
     """Element-wise tolerance (mlp_oracle.gradient_tolerance): |GPU - float64| <=
     (K 2^-23 + 2^-6) S + T per gradient element, S the sum of |products| the
     element adds and T one product (float32 accumulation of K terms, one-ulp
@@ -51,9 +60,9 @@ def test_forward_and_gradients_match_oracle(B, same_sign):
     weight gradients; with same_sign the targets lie below every prediction, so
     the output-side gradients add products of one sign and a lost split-k
     partial exceeds the bound (test_mlp_oracle pins that on the CPU)."""
-    _say("mlp gradients B=%d same_sign=%s" % (B, same_sign))
+    _say("mlp gradients B=%d same_sign=%s terms=%s" % (B, same_sign, terms))
     import torch
-    net, area, height = _net(seed=B)
+    net, area, height = _net(seed=B, terms=terms)
     rng = np.random.default_rng(B)
     p = net.get_params().astype(np.float64)
     o_b1 = net.fp * net.hidden
@@ -63,7 +72,7 @@ def test_forward_and_gradients_match_oracle(B, same_sign):
     keys = rng.random((B, 100)).astype(np.float32)
     dk = torch.from_numpy(keys).cuda()
     y = net.predict(dk).cpu().numpy().astype(np.float64)
-    x = mo.features(keys, area, height, net.fp, THR)
+    x = mo.features(keys, area, height, net.fp, THR, machine_terms=net.machine_terms)
     y_ref, _, _ = mo.forward(x, p, net.fp, net.hidden)
     assert np.max(np.abs(y - y_ref)) <= 2e-2 * max(1.0, np.max(np.abs(y_ref)))
     if same_sign:
@@ -106,14 +115,16 @@ def test_training_fits_a_batch():
     net.close()
 
 
-def test_screening_gains_over_plain_selection():
+@pytest.mark.parametrize("config", ["C2", "C3"])
+def test_screening_gains_over_plain_selection(config):
     """After online training on exact decodes, the network's picks from a fresh pool decode
-    to better makespans than the pool's first rows (what plain BRKGA decodes)."""
-    _say("surrogate gain")
+    to better makespans than the pool's first rows (what plain BRKGA decodes); C3 has four
+    machines with different plates, where the per-machine aggregates carry the ranking."""
+    _say("surrogate gain %s" % config)
     import torch
     from brkga_fft2_amd.native import Decoder
     from brkga_fft2_amd.surrogate import SurrogateBRKGA, spearman
-    prob, n, M, packed = syn_instance("C2", seed=4)
+    prob, n, M, packed = syn_instance(config, seed=4)
     dec = Decoder(packed, max_bins=n)
     ga = SurrogateBRKGA(dec, prob, prob.instance_parts, num_individuals=512, pool=2048, warmup=3,
                         seed=7, train_steps=32, batch=1024)

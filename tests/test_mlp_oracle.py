@@ -54,6 +54,46 @@ def test_features_layout():
     assert np.all(mo.features(keys, area, height, 16)[0, 3:6] == 0)
 
 
+def test_machine_aggregate_features_layout():
+    """Hand-worked aggregates: slots 0..3 on machines (by key) 1, 0, 1, 1 of two."""
+    keys = np.array([[0.4, 0.3, 0.2, 0.1, 0.9, 0.5, 0.6, 0.75]], np.float32)   # n = 4; 0.5 is machine 0
+    area = np.array([1.0, 2.0, 3.0, 4.0], np.float32)
+    height = np.array([0.5, 0.25, 1.0, 0.75], np.float32)
+    t = np.zeros((2, 3, 4), np.float32)
+    t[0, 0] = [1, 2, 4, 8]          # only slot 1 is on machine 0
+    t[0, 1] = [0.5, 0.25, 0.5, 0.5]
+    t[0, 2] = [9, 3, 9, 9]
+    t[1, 0] = [1, 2, 4, 8]          # slots 0, 2, 3: 1 + 4 + 8
+    t[1, 1] = [0.5, 0.25, 0.125, 1.0]
+    t[1, 2] = [0.5, 9, 0.25, 0.75]
+    x = mo.features(keys, area, height, 32, thresholds=[0.5], machine_terms=t)
+    # slot columns scaled by 1/8; sequence order slots 3, 2, 1, 0
+    assert x[0, 0:4].tolist() == [0.75 / 8, 0.5 / 8, 0.25 / 8, 0.0]
+    assert x[0, 4:8].tolist() == [1 / 8, 0.0, 1 / 8, 1 / 8]
+    assert x[0, 8:12].tolist() == [4 / 8, 3 / 8, 2 / 8, 1 / 8]
+    assert x[0, 12:16].tolist() == [0.75 / 8, 1 / 8, 0.25 / 8, 0.5 / 8]
+    # machine 0: sums 2 and 0.25, count 1 * 2 / 4, tallest 3; machine 1: 13, 1.625, 3 * 2 / 4, 0.75
+    assert x[0, 16:20].tolist() == [2.0, 0.25, 0.5, 3.0]
+    assert x[0, 20:24].tolist() == [13.0, 1.625, 1.5, 0.75]
+    assert np.all(x[0, 24:] == 0)
+    # a machine with no slots: zeros
+    keys[0, 4:] = 0.9
+    x = mo.features(keys, area, height, 32, thresholds=[0.5], machine_terms=t)
+    assert x[0, 16:20].tolist() == [0.0, 0.0, 0.0, 0.0] and x[0, 22] == 2.0
+
+
+def test_machine_terms_scale_to_even_split():
+    """machine_terms scales processing time and plate-relative area so that the
+    parts spread evenly over identical machines sum to 1 per machine."""
+    from brkga_fft2_amd.problem import synthetic_problem
+    from brkga_fft2_amd.surrogate import machine_terms
+    prob, n, M = synthetic_problem("C2", seed=1)
+    t = machine_terms(prob, prob.instance_parts, M).astype(np.float64)
+    assert t.shape == (M, 3, n)
+    assert np.allclose(t[:, 0].sum(axis=1), M) and np.allclose(t[:, 1].sum(axis=1), M)
+    assert t[:, 2].max() == 1.0
+
+
 def test_machine_bucket_matches_decoder_assignment():
     """Buckets agree with the machine the C oracle places every part on,
     including keys exactly on a threshold (they belong to the lower bucket)."""
