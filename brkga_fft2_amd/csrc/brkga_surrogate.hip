@@ -169,9 +169,10 @@ __device__ __forceinline__ int machine_bucket(double mv, const double *thr, int 
 // the keys as the decoder reads them (sequence position / n, machine bucket /
 // max(M-1, 1)), then area and height of the parts ordered by ascending
 // sequence key (ties by slot), each times ss.  With per-machine terms mt
-// ([M][3][n]) four aggregates per machine m follow: the sums, in slot order,
-// of mt[m][0][j] and mt[m][1][j] over the slots j of machine m, their count
-// times M / n, and the largest mt[m][2][j] among them.  Columns F..Fp-1 are 0.
+// ([M][3][n]) four aggregates per machine m follow: the float32 sums of
+// mt[m][0][j] and mt[m][1][j] over the slots j of machine m (64 strided lane
+// sums, then a butterfly), their count times M / n, and the largest
+// mt[m][2][j] among them.  Columns F..Fp-1 are 0.
 __global__ __launch_bounds__(256) void featurize_kernel(const float *keys, int B, int n, int M,
                                                         const double *thr, int F, int Fp, float ss,
                                                         const float *slot_area, const float *slot_height,
@@ -197,23 +198,37 @@ __global__ __launch_bounds__(256) void featurize_kernel(const float *keys, int B
         xr[2 * n + rank] = f2bf(slot_area[j] * ss);
         xr[3 * n + rank] = f2bf(slot_height[j] * ss);
     }
-    if (mt && (int)threadIdx.x < M) {   // one thread per machine, slots in order
-        const int m = threadIdx.x;
-        const float *t0 = mt + (long long)m * 3 * n, *t1 = t0 + n, *t2 = t1 + n;
-        float s0 = 0.0f, s1 = 0.0f, mx = 0.0f;
-        int cnt = 0;
-        for (int j = 0; j < n; j++)
-            if (sb[j] == m) {
-                s0 += t0[j];
-                s1 += t1[j];
-                mx = fmaxf(mx, t2[j]);
-                cnt++;
+    if (mt) {
+        // wave w takes machines w, w + 4, ...: lane l adds the terms of slots
+        // l, l + 64, ... in that order, then the 64 lane sums are added by a
+        // butterfly (xor 32, 16, ..., 1); the count and the maximum are exact
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int m = wave; m < M; m += 4) {
+            const float *t0 = mt + (long long)m * 3 * n, *t1 = t0 + n, *t2 = t1 + n;
+            float s0 = 0.0f, s1 = 0.0f, mx = 0.0f;
+            int cnt = 0;
+            for (int j = lane; j < n; j += 64) {
+                const bool on = sb[j] == m;   // adding 0 to a sum of non-negative terms changes nothing
+                s0 += on ? t0[j] : 0.0f;
+                s1 += on ? t1[j] : 0.0f;
+                mx = on ? fmaxf(mx, t2[j]) : mx;
+                cnt += on;
             }
-        bf16_t *xa = xr + 4 * n + 4 * m;
-        xa[0] = f2bf(s0);
-        xa[1] = f2bf(s1);
-        xa[2] = f2bf((float)(cnt * M) / (float)n);
-        xa[3] = f2bf(mx);
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                s0 += __shfl_xor(s0, off, 64);
+                s1 += __shfl_xor(s1, off, 64);
+                mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+                cnt += __shfl_xor(cnt, off, 64);
+            }
+            if (lane == 0) {
+                bf16_t *xa = xr + 4 * n + 4 * m;
+                xa[0] = f2bf(s0);
+                xa[1] = f2bf(s1);
+                xa[2] = f2bf((float)(cnt * M) / (float)n);
+                xa[3] = f2bf(mx);
+            }
+        }
     }
     (void)B;
 }
@@ -520,9 +535,9 @@ extern "C" int brkga_mlp_create(int32_t n_parts, int32_t n_machines, const doubl
                                 int32_t hidden, const float *slot_area, const float *slot_height,
                                 const float *machine_terms, uint64_t seed, float lr, int32_t device,
                                 brkga_mlp **out) {
-    if (!out || n_parts < 1 || n_parts > 8192 || n_machines < 1 || n_machines > 256 ||
+    if (!out || n_parts < 1 || n_parts > 4096 || n_machines < 1 || n_machines > 256 ||
         (n_machines > 1 && !thresholds) || hidden < 32 || hidden % 32 || !slot_area || !slot_height) {
-        g_mlp_error = "bad argument (1 <= n_parts <= 8192, 1 <= n_machines <= 256 with n_machines-1 "
+        g_mlp_error = "bad argument (1 <= n_parts <= 4096, 1 <= n_machines <= 256 with n_machines-1 "
                       "thresholds, hidden a positive multiple of 32, feature arrays)";
         return BRKGA_EINVAL;
     }

@@ -280,8 +280,9 @@ int brkga_breed_pool(const float *d_population, const int32_t *d_order, int32_t 
  * slot_area and slot_height of its parts in ascending sequence-key order
  * (ties by slot).  With machine_terms ([n_machines][3][n_parts] float32,
  * nullable) the slot features are multiplied by 1/8 and 4 per machine m
- * follow: the sums, in slot order, of machine_terms[m][0][j] and
- * machine_terms[m][1][j] over the slots j the chromosome sends to machine m,
+ * follow: the float32 sums of machine_terms[m][0][j] and machine_terms[m][1][j]
+ * over the slots j the chromosome sends to machine m (64 lane sums of slots
+ * l, l + 64, ..., then a butterfly over xor 32, 16, ..., 1),
  * their count times n_machines / n_parts, and the largest
  * machine_terms[m][2][j] among them (0 for none); the Python layer passes the
  * processing time, the area over the plate area and the height-driven
@@ -289,7 +290,7 @@ int brkga_breed_pool(const float *d_population, const int32_t *d_order, int32_t 
  * hidden -> hidden -> 1, ReLU, F = 4n (+ 4 n_machines).  Parameters (float32
  * master copy) are laid out W1 [Fp][hidden] (rows >= F are zero), b1 [hidden],
  * W2 [hidden][hidden], b2 [hidden], w3 [hidden], b3 [1], with Fp = F rounded
- * up to 64.  1 <= n_parts <= 8192, n_machines <= 256.  d_* pointers are device
+ * up to 64.  1 <= n_parts <= 4096, n_machines <= 256.  d_* pointers are device
  * memory.
  */
 typedef struct brkga_mlp brkga_mlp;

@@ -36,13 +36,30 @@ def machine_bucket(mv, thresholds):
     return np.searchsorted(thr, mv, side="left")
 
 
+def lane_tree_sum(v):
+    """float32 sum in the kernel's order: lane l (of 64) adds v[l], v[l + 64],
+    ... from 0, then a butterfly adds lane l and lane l ^ off for off = 32, 16,
+    ..., 1 (lane 0's value is the result)."""
+    v = np.asarray(v, np.float32)
+    k = -(-v.size // 64)
+    pad = np.zeros(64 * max(k, 1), np.float32)
+    pad[:v.size] = v
+    part = np.zeros(64, np.float32)
+    for row in pad.reshape(-1, 64):
+        part = part + row
+    idx = np.arange(64)
+    for off in (32, 16, 8, 4, 2, 1):
+        part = part + part[idx ^ off]
+    return part[0]
+
+
 def features(keys, slot_area, slot_height, fp, thresholds=(), machine_terms=None):
     """[B, fp] features: sequence position / n and machine bucket / max(M-1, 1)
     per slot, then area/height in ascending sequence-key order (ties by slot).
     With ``machine_terms`` ([M][3][n] float32) those 4n columns are scaled by
-    1/8 and four aggregates per machine m follow: float32 sums, in slot order,
-    of terms 0 and 1 over the slots in bucket m, their count times M / n, and
-    the largest term 2 among them (0 for none)."""
+    1/8 and four aggregates per machine m follow: float32 sums of terms 0 and 1
+    over the slots in bucket m (``lane_tree_sum``), their count times M / n,
+    and the largest term 2 among them (0 for none)."""
     keys = np.asarray(keys, dtype=np.float32)
     B, g2 = keys.shape
     n = g2 // 2
@@ -62,9 +79,8 @@ def features(keys, slot_area, slot_height, fp, thresholds=(), machine_terms=None
         if mt is not None:
             for m in range(M):
                 on = mach == m
-                # running float32 sums (cumsum adds in slot order; a 0 term adds nothing)
-                s0 = np.cumsum(np.where(on, mt[m, 0], np.float32(0)), dtype=np.float32)[-1]
-                s1 = np.cumsum(np.where(on, mt[m, 1], np.float32(0)), dtype=np.float32)[-1]
+                s0 = lane_tree_sum(np.where(on, mt[m, 0], np.float32(0)))
+                s1 = lane_tree_sum(np.where(on, mt[m, 1], np.float32(0)))
                 mx = mt[m, 2][on].max() if on.any() else np.float32(0)
                 cnt = np.float32(int(on.sum()) * M) / np.float32(n)
                 x[b, 4 * n + 4 * m:4 * n + 4 * m + 4] = bf16(np.array([s0, s1, cnt, mx], np.float32))

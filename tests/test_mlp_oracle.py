@@ -155,3 +155,23 @@ def test_gradient_tolerance_accepts_float32_and_rejects_a_dropped_chunk():
         sl = slice(offs[k], offs[k + 1])
         frac = np.mean(np.abs(gd[sl] - g[sl]) > tol[sl])
         assert frac > 0.5, (name, frac)   # elements with no product in the dropped rows stay put
+
+
+def test_lane_tree_sum_order():
+    """The kernel's summation order: 64 strided lane sums, then a butterfly.
+    Exactly representable terms give the plain sum; a case where float32
+    rounding depends on the order tells it apart from the sequential sum."""
+    v = np.arange(200, dtype=np.float32) * np.float32(0.25)
+    assert mo.lane_tree_sum(v) == np.float32(v.astype(np.float64).sum())
+    big = np.zeros(65, np.float32)
+    big[0], big[64], big[1] = 2.0 ** 24, 1.0, 1.0   # lane 0 adds 2^24 + 1 (rounds away), lane 1 holds 1
+    # sequential: ((2^24 + 1) + 1) rounds twice to 2^24; lanes: (2^24 + 1) -> 2^24, then + 1 -> 2^24 too
+    assert mo.lane_tree_sum(big) == np.float32(2.0 ** 24)
+    small = np.zeros(66, np.float32)
+    small[0], small[1], small[64], small[65] = 2.0 ** 24, 1.0, 1.0, 1.0
+    # lanes: lane 0 = 2^24 + 1 -> 2^24; lane 1 = 1 + 1 = 2; butterfly 2^24 + 2 exact
+    assert mo.lane_tree_sum(small) == np.float32(2.0 ** 24 + 2)
+    seq = np.float32(0)
+    for x in small:
+        seq = np.float32(seq + x)
+    assert seq == np.float32(2.0 ** 24)   # the slot-order sum loses both ones
