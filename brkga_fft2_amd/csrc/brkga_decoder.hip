@@ -106,9 +106,10 @@ struct DevProblem {
     const double *setup, *proc_time, *pth;
     const unsigned char *fits;   // [M][n_types] rotation 0 fits the plate in some orientation
     // Tile shapes of the collision passes per (machine, Toeplitz chunk count
-    // nc = ceil((w+31)/32), clamped to TUNE_NC-1): [M][TUNE_NC][2] = {cap on x-tiles
-    // per pass, cap on x-tiles per staged LDS window}.  Any values give the
-    // same results; only the speed changes.
+    // nc = ceil((w+31)/32), clamped to TUNE_NC-1): [M][TUNE_NC][TILE_ENTRY] =
+    // {cap on x-tiles per pass, cap on x-tiles per staged LDS window, mask rows
+    // of the screening pass (0: none), unused}.  Any values give the same
+    // results; only the speed changes.
     const unsigned char *tiles;
     const int *type_cost;   // [n_types] expected work of placing one part: its mask's bounding-box area
 };
@@ -123,7 +124,7 @@ struct DevWork {
     int q_off;               // LDS offset of the staged mask bits
     int vc_bins;             // bins whose vacancy vector is cached in LDS
     int band_off, band_words;  // LDS band for staged grid bit rows (32-bit words)
-    int zw_off;                // LDS zero masks: [2 passes][4 waves][8 tiles * 16] u64
+    int zw_off;                // LDS zero masks: [2 passes][4 waves][8 tiles * 16] u64, then 4 u64 screen masks
     int bins_off;              // LDS bin bookkeeping: area, pt, pth [n_parts] f64, minrow [n_parts] i32
 };
 
@@ -404,6 +405,8 @@ struct Fold { int loff, rlog, segx, skip; };
 // the B fragments and the A fragments in flight fit the register budget.
 #define MAX_NC 12
 #define TUNE_NC BRKGA_TILE_CLASSES
+#define TILE_ENTRY 4        // bytes per tile-table entry
+#define SCREEN_DEFAULT 6    // mask rows of the screening pass unless set otherwise
 __host__ __device__ constexpr int pass_tiles(int nc) { return nc <= 5 ? 8 : (nc <= 8 ? 4 : (nc <= MAX_NC ? 2 : 1)); }
 
 // One pass of TP tiles: correlate, combine the waves' zero masks, decode.
@@ -422,28 +425,150 @@ __device__ __forceinline__ int one_pass(lds_cu32 *band, int stride, const Fold &
     return decode_zero<TP>(zp, xbase + (t0 << 5), xmax, nrows, f.rlog, f.segx);
 }
 
-// All passes over one staged x-window (tiles [0, tl) of each segment), run
-// by the whole workgroup.  The window is cut into passes of 8/4/2/1 tiles; a
-// remainder of 6 or 7 tiles takes one 8-tile pass and a remainder of 3 one
-// 4-tile pass (one or two tiles of MFMAs cost less than the extra pass's
-// zero-mask exchange and barrier).  Tiles past the staged window read
-// whatever the band row holds there (at most two words past the staged ones,
-// inside the row's stride); the caller's xmax ends at the window's last
-// column, so their counts are never decoded.  Returns the window's minimum
-// key; with `early`, stops once candidate 0 has a zero left of every tile
-// still to come (a key below xbase + 32 * t0; in a folded group the later
-// passes of segment 0 come before the current ones of the other segments).
-// mfma counts the MFMA instructions issued.
+// Screening pass of one wave over the tiles [t0, t0 + TW) of a staged window,
+// with the S mask rows a_i = skip + (h - 1 - skip) * i / (S - 1) only (evenly
+// spaced over the rows that meet occupied grid rows): the same MFMA
+// correlation as corr_pass, with the wave's own tiles instead of a share of
+// the rows.  Every product is non-negative, so the count over a subset of
+// the rows is zero wherever the full count is: a tile whose subset counts
+// have no zero in any accumulator row (candidate or fold segment) has no
+// placement, and the full passes skip it.  Returns bit t for tile t0 + t (t
+// < tcnt) when some subset count of that tile is zero, uniform in the wave.
+// Tiles from tcnt on are computed from clamped band words (wlast: the last
+// word index the window's tiles read, before the lane's k-group offset) and
+// not reported.  Zero counts past the valid region only set bits needlessly.
+template <int TW, int NC, bool F4>
+__device__ __forceinline__ unsigned screen_pass(lds_cu32 *band, int stride, int loff, int skip, int h, int S,
+                                                lds_cu32 *qbits, int qs, int t0, int tcnt, int wlast) {
+    constexpr int NA = F4 ? TW + 2 * NC - 1 : TW + NC - 1;
+    constexpr int NQ = F4 ? 2 * NC : NC + 1;
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 5;
+    const int ks = g << 2;
+    const int bs = F4 ? 32 - (lane & 31) : 32 - (lane & 31) + ks;
+    typedef typename TileAcc<F4>::T AT;
+    AT acc[TW];
+#pragma unroll
+    for (int t = 0; t < TW; t++) acc[t] = (AT){};
+    lds_cu32 *base = band + loff + (F4 ? g : 0);
+    lds_cu32 *qb = qbits + (F4 ? g : 0);
+    for (int i = 0; i < S; i++) {
+        const int a = skip + (S > 1 ? ((h - 1 - skip) * i) / (S - 1) : 0);
+        unsigned qv[NQ];
+#pragma unroll
+        for (int c = 0; c < NQ; c++) qv[c] = qb[a * qs + c];
+        typedef typename std::conditional<F4, v8i, v4i>::type FT;
+        FT B[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            if constexpr (F4) B[c] = nib_frag((unsigned)((((unsigned long long)qv[2 * c + 1] << 32) | qv[2 * c]) >> bs));
+            else B[c] = bfrag_from_bits(qv[c], qv[c + 1], bs);
+        }
+        lds_cu32 *rc = base + a * stride;
+        unsigned Aw[NA];
+#pragma unroll
+        for (int j = 0; j < NA; j++) Aw[j] = rc[min(t0 + j, wlast)];
+#pragma unroll
+        for (int j = 0; j < NA; j++) {
+            if constexpr (F4) {
+                const v8i A = nib_frag(Aw[j]);
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    const int t = j - 2 * c;
+                    if (t >= 0 && t < TW) acc[t] = mfma_f4(A, B[c], acc[t]);
+                }
+            } else {
+                const v4i A = frag_from_word(Aw[j] >> ks);
+#pragma unroll
+                for (int c = 0; c < NC; c++) {
+                    const int t = j - c;
+                    if (t >= 0 && t < TW) acc[t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A, B[c], acc[t], 0, 0, 0);
+                }
+            }
+        }
+    }
+    unsigned bits = 0;
+#pragma unroll
+    for (int t = 0; t < TW; t++) {
+        if (t < tcnt) {
+            unsigned long long z = 0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) z |= __ballot(acc[t][i] == 0);
+            if (z) bits |= 1u << t;
+        }
+    }
+    return bits;
+}
+
+// Tiles per screening step of one wave: the accumulators of a step and the
+// B fragments of a row stay within half the full pass's register budget.
+__host__ __device__ constexpr int screen_tiles(int nc) { return nc <= 5 ? 4 : (nc <= 8 ? 2 : 1); }
+
+// Screen of a staged window (tiles [0, tl) of each segment) with S mask rows,
+// run by the whole workgroup: the waves take consecutive quarters of the
+// tiles, and the OR of their bits (through s_bits, 4 x u64 of LDS) is the
+// mask of tiles the full passes must correlate.  mfma counts the wave's MFMA
+// instructions.
+template <int NC, bool F4>
+__device__ __attribute__((always_inline)) unsigned long long screen_window(lds_cu32 *band, int stride, const Fold &f,
+                                                                           int h, int S, lds_cu32 *qbits, int qs, int tl,
+                                                                           lds_u64 *s_bits, unsigned long long &mfma) {
+    constexpr int TW = screen_tiles(NC);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int q = (tl + 3) >> 2;
+    const int tb = min(tl, wave * q), te = min(tl, tb + q);
+    const int wlast = F4 ? tl + 2 * NC - 3 : tl + NC - 2;
+    unsigned long long bits = 0;
+    for (int t0 = tb; t0 < te; t0 += TW) {
+        bits |= (unsigned long long)screen_pass<TW, NC, F4>(band, stride, f.loff, f.skip, h, S, qbits, qs, t0,
+                                                            min(TW, te - t0), wlast) << t0;
+        mfma += (unsigned long long)TW * NC * S;
+    }
+    if ((threadIdx.x & 63) == 0) s_bits[wave] = bits;
+    __syncthreads();
+    return s_bits[0] | s_bits[1] | s_bits[2] | s_bits[3];
+}
+
+template <bool F4>
+__device__ __forceinline__ unsigned long long screen_dispatch(int nc, lds_cu32 *band, int stride, const Fold &f, int h,
+                                                              int S, lds_cu32 *qbits, int qs, int tl, lds_u64 *s_bits,
+                                                              unsigned long long &mfma) {
+#define SCASE(K) case K: return screen_window<K, F4>(band, stride, f, h, S, qbits, qs, tl, s_bits, mfma);
+    switch (nc) {
+        SCASE(1) SCASE(2) SCASE(3) SCASE(4) SCASE(5) SCASE(6) SCASE(7) SCASE(8) SCASE(9) SCASE(10) SCASE(11) SCASE(12)
+        default:
+            return tl >= 64 ? ~0ull : (1ull << tl) - 1ull;
+    }
+#undef SCASE
+}
+
+// The passes over one staged x-window (tiles [0, tl) of each segment), run
+// by the whole workgroup, over the tiles set in `need` (bit t: tile t; the
+// others are known to hold no placement).  Each run of consecutive needed
+// tiles is cut into passes of 8/4/2/1 tiles; a remainder of 6 or 7 tiles
+// takes one 8-tile pass and a remainder of 3 one 4-tile pass (one or two
+// tiles of MFMAs cost less than the extra pass's zero-mask exchange and
+// barrier), so a pass may also cover up to two tiles after its run.  Tiles
+// past the staged window read whatever the band row holds there (at most two
+// words past the staged ones, inside the row's stride); the caller's xmax
+// ends at the window's last column, so their counts are never decoded.
+// Returns the window's minimum key; with `early`, stops once candidate 0 has
+// a zero left of every needed tile still to come (a key below xbase + 32 *
+// t_end; in a folded group the later passes of segment 0 come before the
+// current ones of the other segments).  mfma counts the MFMA instructions
+// issued.
 template <int NC, bool F4>
 __device__ __attribute__((always_inline)) int window_passes(lds_cu32 *band, int stride, const Fold &f, int h,
-                             lds_cu32 *qbits, int qs, int tl, int xbase, int xmax, int nrows,
+                             lds_cu32 *qbits, int qs, unsigned long long need, int xbase, int xmax, int nrows,
                              lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
     constexpr int TPM = pass_tiles(NC);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int a0 = f.skip + (((h - f.skip) * wave) >> 2), a1 = f.skip + (((h - f.skip) * (wave + 1)) >> 2);
     int key = KEY_NONE, parity = 0;
-    for (int t0 = 0; t0 < tl;) {
-        const int rem = tl - t0;
+    while (need) {
+        const int t0 = (int)__builtin_ctzll(need);
+        const unsigned long long rest = ~(need >> t0);
+        const int rem = rest ? (int)__builtin_ctzll(rest) : 64 - t0;   // needed tiles from t0 on, in a run
         int k, tp;
         if (TPM >= 8 && tpcap >= 8 && rem >= 6) {
             k = one_pass<(TPM >= 8 ? 8 : 1), NC, F4>(band, stride, f, a0, a1, qbits, qs, t0, xbase, xmax, nrows, s_zw, parity);
@@ -460,8 +585,9 @@ __device__ __attribute__((always_inline)) int window_passes(lds_cu32 *band, int 
         }
         mfma += (unsigned long long)tp * (a1 - a0) * NC;
         key = k < key ? k : key;
-        t0 += tp;
-        if (early && key < xbase + (t0 << 5)) break;
+        const int te = t0 + tp;
+        need = te >= 64 ? 0ull : need & (~0ull << te);
+        if (early && key < xbase + (te << 5)) break;
     }
     return key;
 }
@@ -507,9 +633,9 @@ __device__ int window_passes_generic(lds_cu32 *band, int stride, const Fold &f, 
 
 template <bool F4>
 __device__ __forceinline__ int window_dispatch(int nc, lds_cu32 *band, int stride, const Fold &f, int h,
-                               lds_cu32 *qbits, int qs, int tl, int xbase, int xmax, int nrows,
-                               lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
-#define CASE(K) case K: return window_passes<K, F4>(band, stride, f, h, qbits, qs, tl, xbase, xmax, nrows, s_zw, mfma, tpcap, early);
+                               lds_cu32 *qbits, int qs, int tl, unsigned long long need, int xbase, int xmax,
+                               int nrows, lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
+#define CASE(K) case K: return window_passes<K, F4>(band, stride, f, h, qbits, qs, need, xbase, xmax, nrows, s_zw, mfma, tpcap, early);
     switch (nc) {
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
         default:
@@ -588,7 +714,7 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid
                              int cw, int qs, const unsigned *qbits_g, const short *cand, int ncand, int r0,
                              lds_u32 *band, int band_words, lds_u32 *s_q, lds_u64 *s_zw,
                              unsigned long long &groups, unsigned long long &mfma, unsigned long long *ph,
-                             int tpcap, int wcap) {
+                             int tpcap, int wcap, int screen) {
     const int tid = threadIdx.x, lane = tid & 63;
     // mask bits of the rotation -> LDS (read by every wave for every group)
     wg_copy<unsigned>(h * qs, [&](int k) { return qbits_g[k]; },
@@ -645,8 +771,15 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid
             f.loff = (cand[g0 + min(m & ((1 << f.rlog) - 1), nrows - 1)] - ylo) * stride + (tw0 - c0) +
                      (f.segx >> 5);
             f.skip = skip;
-            const int k = window_dispatch<F4>(nc, band, stride, f, h, s_q, qs, tf, tw0 << 5,
-                                          min(W - w, ((tw0 + tl) << 5) - 1), nrows, s_zw, mfma, tpcap, !DENSE);
+            // screening pass (not in the dense mode): the tiles whose counts over
+            // `screen` evenly spaced mask rows have no zero hold no placement
+            unsigned long long need = tf >= 64 ? ~0ull : (1ull << tf) - 1ull;
+            if (!DENSE && screen > 0 && h - skip > 2 * screen)
+                need = screen_dispatch<F4>(nc, band, stride, f, h, screen, s_q, qs, tf, s_zw + 2 * ZW_PASS, mfma);
+            const int k = need ? window_dispatch<F4>(nc, band, stride, f, h, s_q, qs, tf, need, tw0 << 5,
+                                                     min(W - w, ((tw0 + tl) << 5) - 1), nrows, s_zw, mfma, tpcap,
+                                                     !DENSE)
+                               : KEY_NONE;
             gkey = k < gkey ? k : gkey;
             if (tid == 0) ph[3] += tick() - t1;
             if (!DENSE && gkey < XKEY) break;   // candidate 0 has a zero: nothing further left can win
@@ -993,11 +1126,11 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
         c_tests++;
         c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
         const int ncr = P.rot_nc[rg], nc4 = P.rot_nc4[rg];
-        const unsigned char *tc = P.tiles + ((long long)mach * TUNE_NC + min(ncr, TUNE_NC - 1)) * 2;
+        const unsigned char *tc = P.tiles + ((long long)mach * TUNE_NC + min(ncr, TUNE_NC - 1)) * TILE_ENTRY;
         const int key = test_rotation<DENSE, F4>(grid, Wk.pw, W, h, w, F4 ? nc4 : ncr, F4 ? 2 * nc4 : ncr,
                                                  2 * nc4 + 2, P.qbits + P.qbits_off[rg], cand, ncand, H - enc,
                                              band, Wk.band_words, s_q, s_zw, c_groups, c_mfma, ph,
-                                             uni(tc[0]), uni(tc[1]));
+                                             uni(tc[0]), uni(tc[1]), uni(tc[2]));
         if (key == KEY_NONE) continue;
         // density (new enclosure length) -> row -> col (binClassNew.py:101-117)
         const int y = uni(cand[key / XKEY]), x = key % XKEY, row = y + h - 1;
@@ -1089,7 +1222,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
     short *s_vcache = s_vac + hmax;                         // [vc_bins][hmax] vacancy of bins < vc_bins
     lds_u32 *s_q = (lds_u32 *)(smem + Wk.q_off);            // [hrot * (max nc + 1)]
     lds_u32 *band = (lds_u32 *)(smem + Wk.band_off);        // [band_words]
-    lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);          // [2][4][ZW_WAVE] zero masks
+    lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);          // [2][4][ZW_WAVE] zero masks, 4 screen masks
     __shared__ int s_ctx, s_cnt, s_off, s_infeasible, s_nb;
     __shared__ int s_ncand[MAX_ROT];
 
@@ -1544,6 +1677,10 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
         return fail(nullptr, BRKGA_EINVAL, "mfma_format must be BRKGA_MFMA_FP4 (0) or BRKGA_MFMA_INT8 (1)");
     }
     h->f4 = opt.mfma_format == BRKGA_MFMA_FP4;
+    if (opt.screen_rows < -1 || opt.screen_rows > BRKGA_MAX_SCREEN_ROWS) {
+        delete h;
+        return fail(nullptr, BRKGA_EINVAL, "screen_rows must be -1 (off), 0 (default) or 1..16");
+    }
     if (!(opt.tiles_per_pass == 0 || opt.tiles_per_pass == 1 || opt.tiles_per_pass == 2 ||
           opt.tiles_per_pass == 4 || opt.tiles_per_pass == 8)) {
         delete h;
@@ -1653,13 +1790,17 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
         (rc = upload(h, p->proc_time_height, (size_t)M * T, &d.pth)) ||
         (rc = upload(h, fits.data(), fits.size(), &d.fits)))
         return bail(rc);
-    // tile table: per-width default pass size (or the option's cap), full windows
-    h->tiles.assign((size_t)M * TUNE_NC * 2, 0);
+    // tile table: per-width default pass size (or the option's cap), full
+    // windows, the option's screening rows
+    const int screen = opt.screen_rows < 0 ? 0 : (opt.screen_rows ? opt.screen_rows : SCREEN_DEFAULT);
+    h->tiles.assign((size_t)M * TUNE_NC * TILE_ENTRY, 0);
     for (int m = 0; m < M; m++)
         for (int c = 0; c < TUNE_NC; c++) {
             const int def = pass_tiles(c < 1 ? 1 : c);
-            h->tiles[((size_t)m * TUNE_NC + c) * 2] = (unsigned char)(opt.tiles_per_pass ? std::min(def, opt.tiles_per_pass) : def);
-            h->tiles[((size_t)m * TUNE_NC + c) * 2 + 1] = (unsigned char)WINDOW_CAP;
+            unsigned char *e = &h->tiles[((size_t)m * TUNE_NC + c) * TILE_ENTRY];
+            e[0] = (unsigned char)(opt.tiles_per_pass ? std::min(def, opt.tiles_per_pass) : def);
+            e[1] = (unsigned char)WINDOW_CAP;
+            e[2] = (unsigned char)screen;
         }
     if ((rc = upload(h, h->tiles.data(), h->tiles.size(), &d.tiles))) return bail(rc);
     // context cost for the decode order: bounding-box area of rotation 0
@@ -1696,7 +1837,7 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     w.q_off = (int)fixed;
     fixed += (qwords * 4 + 15) & ~(size_t)15;
     w.zw_off = (int)fixed;
-    fixed += (size_t)2 * ZW_PASS * sizeof(unsigned long long);
+    fixed += (size_t)(2 * ZW_PASS + 4) * sizeof(unsigned long long);   // + the screen's tile masks
     w.bins_off = (int)fixed;
     fixed += ((size_t)p->n_parts * (3 * sizeof(double) + sizeof(int)) + 15) & ~(size_t)15;
     // band sizes in bytes: the smallest that works (one row group of one tile,
@@ -2002,12 +2143,12 @@ extern "C" int brkga_set_tiles(brkga_handle *h, const int32_t *tiles_per_pass, c
                 const int v = tiles_per_pass[i];
                 if (!(v == 0 || v == 1 || v == 2 || v == 4 || v == 8))
                     return fail(h, BRKGA_EINVAL, "tiles_per_pass entries must be 0, 1, 2, 4 or 8");
-                t[i * 2] = (unsigned char)(v ? std::min(v, pass_tiles(c < 1 ? 1 : c)) : pass_tiles(c < 1 ? 1 : c));
+                t[i * TILE_ENTRY] = (unsigned char)(v ? std::min(v, pass_tiles(c < 1 ? 1 : c)) : pass_tiles(c < 1 ? 1 : c));
             }
             if (window_tiles) {
                 const int v = window_tiles[i];
                 if (v < 0 || v > WINDOW_CAP) return fail(h, BRKGA_EINVAL, "window_tiles entries must be in [0, 64]");
-                t[i * 2 + 1] = (unsigned char)(v ? v : WINDOW_CAP);
+                t[i * TILE_ENTRY + 1] = (unsigned char)(v ? v : WINDOW_CAP);
             }
         }
     HIPCHK(h, hipSetDevice(h->device));
@@ -2021,9 +2162,33 @@ extern "C" int brkga_get_tiles(const brkga_handle *h, int32_t *tiles_per_pass, i
     if (!h) return BRKGA_EINVAL;
     const size_t n = (size_t)h->prob.n_machines * TUNE_NC;
     for (size_t i = 0; i < n; i++) {
-        if (tiles_per_pass) tiles_per_pass[i] = h->tiles[i * 2];
-        if (window_tiles) window_tiles[i] = h->tiles[i * 2 + 1];
+        if (tiles_per_pass) tiles_per_pass[i] = h->tiles[i * TILE_ENTRY];
+        if (window_tiles) window_tiles[i] = h->tiles[i * TILE_ENTRY + 1];
     }
+    return BRKGA_OK;
+}
+
+extern "C" int brkga_set_screen(brkga_handle *h, const int32_t *screen_rows) {
+    if (!h) return fail(nullptr, BRKGA_EINVAL, "handle is NULL");
+    if (!screen_rows) return fail(h, BRKGA_EINVAL, "screen_rows is NULL");
+    const size_t n = (size_t)h->prob.n_machines * TUNE_NC;
+    std::vector<unsigned char> t = h->tiles;
+    for (size_t i = 0; i < n; i++) {
+        const int v = screen_rows[i];
+        if (v < 0 || v > BRKGA_MAX_SCREEN_ROWS) return fail(h, BRKGA_EINVAL, "screen_rows entries must be in [0, 16]");
+        t[i * TILE_ENTRY + 2] = (unsigned char)v;
+    }
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipDeviceSynchronize());   // no launch may be reading the table
+    HIPCHK(h, hipMemcpy(h->d_tiles, t.data(), t.size(), hipMemcpyHostToDevice));
+    h->tiles = t;
+    return BRKGA_OK;
+}
+
+extern "C" int brkga_get_screen(const brkga_handle *h, int32_t *screen_rows) {
+    if (!h || !screen_rows) return BRKGA_EINVAL;
+    const size_t n = (size_t)h->prob.n_machines * TUNE_NC;
+    for (size_t i = 0; i < n; i++) screen_rows[i] = h->tiles[i * TILE_ENTRY + 2];
     return BRKGA_OK;
 }
 

@@ -34,7 +34,7 @@ class BrkgaOptions(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("wgs_per_cu", ctypes.c_int32),
                 ("max_bins", ctypes.c_int32), ("dense", ctypes.c_int32),
                 ("tiles_per_pass", ctypes.c_int32), ("mfma_format", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("screen_rows", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 MFMA_FORMATS = {"fp4": 0, "int8": 1}   # BRKGA_MFMA_FP4, BRKGA_MFMA_INT8
@@ -68,7 +68,8 @@ class BrkgaGaParams(ctypes.Structure):
 TILE_CLASSES = 16   # BRKGA_TILE_CLASSES: mask width classes nc = ceil((w + 31) / 32)
 
 EXPORTED = ["brkga_create", "brkga_evaluate", "brkga_evaluate_device", "brkga_get_stats",
-            "brkga_reset_stats", "brkga_set_tiles", "brkga_get_tiles", "brkga_initial_solution",
+            "brkga_reset_stats", "brkga_set_tiles", "brkga_get_tiles", "brkga_set_screen",
+            "brkga_get_screen", "brkga_initial_solution",
             "brkga_evaluate_device_noalloc",
             "brkga_last_error",
             "brkga_destroy",
@@ -116,6 +117,10 @@ def load_library(path=LIB_PATH):
     lib.brkga_set_tiles.restype = ctypes.c_int
     lib.brkga_get_tiles.argtypes = [_P, _P, _P]
     lib.brkga_get_tiles.restype = ctypes.c_int
+    lib.brkga_set_screen.argtypes = [_P, _P]
+    lib.brkga_set_screen.restype = ctypes.c_int
+    lib.brkga_get_screen.argtypes = [_P, _P]
+    lib.brkga_get_screen.restype = ctypes.c_int
     lib.brkga_initial_solution.argtypes = [_P, _P, _P, _P, _P, _P, _P]
     lib.brkga_initial_solution.restype = ctypes.c_int
     lib.brkga_last_error.argtypes = [_P]
@@ -156,7 +161,7 @@ class Decoder:
     """One decoder handle: a problem uploaded to one GPU."""
 
     def __init__(self, packed, device=0, dense=False, tiles_per_pass=0, wgs_per_cu=0,
-                 max_bins=0, mfma="fp4"):
+                 max_bins=0, mfma="fp4", screen_rows=0):
         self._lib = load_library()
         self.n_parts = int(packed["n_parts"])
         self.n_machines = int(packed["n_machines"])
@@ -173,7 +178,7 @@ class Decoder:
             raise ValueError("mfma must be one of %s" % sorted(MFMA_FORMATS))
         self.mfma = mfma
         opt = BrkgaOptions(int(device), int(wgs_per_cu), int(max_bins), int(bool(dense)),
-                           int(tiles_per_pass), MFMA_FORMATS[mfma])
+                           int(tiles_per_pass), MFMA_FORMATS[mfma], int(screen_rows))
         h = ctypes.c_void_p()
         rc = self._lib.brkga_create(ctypes.byref(prob), ctypes.byref(opt), ctypes.byref(h))
         if rc != 0:
@@ -316,6 +321,19 @@ class Decoder:
         t, tp = arr(tiles_per_pass)
         w, wp = arr(window_tiles)
         self._check(self._lib.brkga_set_tiles(self._h, tp, wp), "brkga_set_tiles")
+
+    def get_screen(self):
+        """Screening rows per tile class, int32 ``[n_machines, TILE_CLASSES]`` (0: none)."""
+        s = np.zeros((self.n_machines, TILE_CLASSES), np.int32)
+        self._check(self._lib.brkga_get_screen(self._h, s.ctypes.data), "brkga_get_screen")
+        return s
+
+    def set_screen(self, screen_rows):
+        """Set the screening rows (a scalar or ``[n_machines, TILE_CLASSES]``, 0..16); results
+        never change."""
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(screen_rows, np.int32),
+                                                 (self.n_machines, TILE_CLASSES)))
+        self._check(self._lib.brkga_set_screen(self._h, s.ctypes.data), "brkga_set_screen")
 
     def close(self):
         if getattr(self, "_h", None):

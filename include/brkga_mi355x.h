@@ -93,7 +93,10 @@ typedef struct brkga_options {
     int32_t mfma_format;     /* operands of the collision correlation's MFMAs:
                                 BRKGA_MFMA_FP4 (0, default) or BRKGA_MFMA_INT8 (1); the
                                 counts are exact either way and the results identical */
-    int32_t reserved[2];
+    int32_t screen_rows;     /* mask rows of the screening pass for every class of the tile
+                                table: 0 = default (6), -1 = no screening, 1..16 (see
+                                brkga_set_screen); same results either way */
+    int32_t reserved;
 } brkga_options;
 
 #define BRKGA_MFMA_FP4 0    /* e2m1 operands 0 / 0.5 / 1, float32 accumulation, K = 64 per MFMA */
@@ -193,6 +196,21 @@ int brkga_reset_stats(brkga_handle *h);
 #define BRKGA_TILE_CLASSES 16
 int brkga_set_tiles(brkga_handle *h, const int32_t *tiles_per_pass, const int32_t *window_tiles);
 int brkga_get_tiles(const brkga_handle *h, int32_t *tiles_per_pass, int32_t *window_tiles);
+
+/*
+ * Screening pass of the collision test (an autotuner knob; no counterpart in
+ * the reference).  Before the full correlation of a staged x-window, the
+ * MFMAs correlate only `screen_rows` evenly spaced mask rows over every tile;
+ * the overlap count over a subset of the rows is zero wherever the full count
+ * is, so a tile with no zero there holds no placement and the full passes skip
+ * it.  Table [n_machines][BRKGA_TILE_CLASSES] as for brkga_set_tiles, entries
+ * 0 (no screening) to BRKGA_MAX_SCREEN_ROWS; a group whose mask meets fewer
+ * than 2 * screen_rows occupied rows is not screened.  Every setting gives
+ * identical results.  brkga_set_screen waits for the device to be idle.
+ */
+#define BRKGA_MAX_SCREEN_ROWS 16
+int brkga_set_screen(brkga_handle *h, const int32_t *screen_rows);
+int brkga_get_screen(const brkga_handle *h, int32_t *screen_rows);
 
 /*
  * Greedy initial solution (replaces BRKGA_alg3.py:445-603 and the batch class

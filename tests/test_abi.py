@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
-def _create(packed):
+def _create(packed, opt=None):
     lib = native.load_library()
     keep = []
     prob = native.BrkgaProblem(packed["n_parts"], packed["n_machines"], packed["n_types"],
@@ -41,7 +41,8 @@ def _create(packed):
         keep.append(a)
         setattr(prob, name, a.ctypes.data if a.size else None)
     h = ctypes.c_void_p()
-    rc = lib.brkga_create(ctypes.byref(prob), None, ctypes.byref(h))
+    rc = lib.brkga_create(ctypes.byref(prob), ctypes.byref(opt) if opt is not None else None,
+                          ctypes.byref(h))
     return rc, lib.brkga_last_error(None).decode()
 
 
@@ -57,6 +58,20 @@ def test_create_rejects_bad_problems(mutate, msg):
     packed = dict(packed)
     mutate(packed)
     rc, err = _create(packed)
+    assert rc == -1
+    assert msg in err
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("screen_rows", 17, "screen_rows"), ("screen_rows", -2, "screen_rows"),
+    ("tiles_per_pass", 3, "tiles_per_pass"), ("mfma_format", 2, "mfma_format"),
+])
+def test_create_rejects_bad_options(field, value, msg):
+    """Options are checked before any device call (no GPU needed)."""
+    _, n, m, packed = syn_instance("C1")
+    opt = native.BrkgaOptions()
+    setattr(opt, field, value)
+    rc, err = _create(packed, opt)
     assert rc == -1
     assert msg in err
 

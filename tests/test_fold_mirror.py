@@ -5,6 +5,8 @@ csrc/brkga_decoder.hip): a group of nrows <= R = 2^rlog candidates fills the
 every segment at once and the pass loop may stop early
 once candidate 0 has a zero left of every tile still to come; each pass takes
 its minimum key by a scalar selection over the lanes' keys (pass_min_key).
+With screening (screen_window), the passes run over the runs of tiles whose
+counts over a subset of the mask rows have a zero (screen_need).
 This restates that schedule in Python on random zero patterns and checks it against the
 plain answer -- the smallest (candidate, x) with zero overlap -- so that a
 wrong segment layout, a gap between segments or a premature exit fails."""
@@ -52,16 +54,40 @@ def pass_tiles(rem, tpm, tpcap):
     return 1
 
 
-def window_keys(zero, nrows, tl, xbase, xmax, tpm, tpcap, early):
-    """zero[c][x]: candidate c has zero overlap at window column x < 32 * tl;
-    columns past the window (read by a rounded-up pass) hold garbage, which
-    is modelled as zeros that the window-end limit must discard."""
+def screen_need(szero, nrows, tl):
+    """screen_window: bit t of the mask when some accumulator row (candidate
+    or fold segment; rows past nrows repeat candidate nrows - 1) has a zero
+    subset count anywhere in tile t -- columns past the window or past xmax
+    included, since the screen reports every zero it computes."""
     rl, tf = fold_params(nrows, tl)
     segx = [min((m >> rl) * tf, tl - tf) * 32 for m in range(32)]
+    need = 0
+    for t in range(tf):
+        for m in range(32):
+            c = min(m & ((1 << rl) - 1), nrows - 1)
+            if any(szero(c, segx[m] + 32 * t + n) for n in range(32)):
+                need |= 1 << t
+                break
+    return need
+
+
+def window_keys(zero, nrows, tl, xbase, xmax, tpm, tpcap, early, need=None):
+    """zero[c][x]: candidate c has zero overlap at window column x < 32 * tl;
+    columns past the window (read by a rounded-up pass) hold garbage, which
+    is modelled as zeros that the window-end limit must discard.  need: the
+    screen's tile mask (None: every tile); window_passes runs the passes over
+    each run of needed tiles in order."""
+    rl, tf = fold_params(nrows, tl)
+    if need is None:
+        need = (1 << tf) - 1
+    segx = [min((m >> rl) * tf, tl - tf) * 32 for m in range(32)]
     xend = min(xmax, xbase + 32 * tl - 1)
-    key, t0 = NONE, 0
-    while t0 < tf:
-        rem = tf - t0
+    key = NONE
+    while need:
+        t0 = (need & -need).bit_length() - 1
+        rem = 0
+        while (need >> (t0 + rem)) & 1:
+            rem += 1
         tp = pass_tiles(rem, tpm, tpcap)
         lane_key = [NONE] * 32
         for m in range(32):   # lanes 0..31, one accumulator row each
@@ -77,8 +103,8 @@ def window_keys(zero, nrows, tl, xbase, xmax, tpm, tpcap, early):
         k = pass_min_key(lane_key, rl)
         assert k == min(lane_key)
         key = min(key, k)
-        t0 += tp
-        if early and key < xbase + 32 * t0:
+        need &= ~((1 << (t0 + tp)) - 1)
+        if early and key < xbase + 32 * (t0 + tp):
             break
     return key
 
@@ -100,6 +126,59 @@ def test_folded_schedule_finds_the_smallest_key():
         for early in (False, True):
             assert window_keys(zero, nrows, tl, xbase, xmax, tpm, tpcap, early) == want, \
                 (tl, nrows, xbase, xmax, tpm, tpcap, early)
+
+
+def _screened_case(rng):
+    tl = rng.randint(1, 12)
+    nrows = rng.randint(1, 32)
+    xbase = 32 * rng.randint(0, 3)
+    width = 32 * tl
+    xmax = xbase + rng.randint(0, width + 64)
+    p = rng.choice([0.0, 0.002, 0.01, 0.05])
+    zero = [[rng.random() < p for _ in range(width)] for _ in range(nrows)]
+    # a subset of the mask rows leaves every zero of the full count and more
+    extra = rng.choice([0.0, 0.001, 0.02, 0.2])
+    sz = [[(x < width and zero[c][x]) or rng.random() < extra for x in range(width + 96)]
+          for c in range(nrows)]
+    want = min([c * XKEY + xbase + x for c in range(nrows) for x in range(width)
+                if zero[c][x] and xbase + x <= xmax], default=NONE)
+    return tl, nrows, xbase, xmax, zero, sz, want
+
+
+def test_screened_schedule_finds_the_smallest_key():
+    """Passes over the screen's runs of needed tiles (with the early exit) give
+    the plain answer for any screen pattern that keeps the full count's zeros."""
+    rng = random.Random(11)
+    for _ in range(1500):
+        tl, nrows, xbase, xmax, zero, sz, want = _screened_case(rng)
+        need = screen_need(lambda c, x: sz[c][x], nrows, tl)
+        tpm = rng.choice([8, 4, 2, 1])
+        tpcap = rng.choice([8, 4, 2, 1])
+        for early in (False, True):
+            assert window_keys(zero, nrows, tl, xbase, xmax, tpm, tpcap, early, need) == want, \
+                (tl, nrows, xbase, xmax, tpm, tpcap, early, need)
+
+
+def test_screen_that_loses_a_zero_is_caught():
+    """A screen that dropped the true answer's tile (a wrong row subset or a
+    lost bit) changes the key on some case: the check above can fail."""
+    rng = random.Random(5)
+    caught = 0
+    for _ in range(300):
+        tl, nrows, xbase, xmax, zero, sz, want = _screened_case(rng)
+        if want == NONE:
+            continue
+        rl, tf = fold_params(nrows, tl)
+        need = screen_need(lambda c, x: sz[c][x], nrows, tl)
+        # drop every tile that holds the answer's column in some segment
+        x = want % XKEY - xbase
+        segs = {min(q * tf, tl - tf) for q in range(32 >> rl)}
+        for s0 in segs:
+            if 32 * s0 <= x < 32 * (s0 + tf):
+                need &= ~(1 << ((x - 32 * s0) // 32))
+        got = window_keys(zero, nrows, tl, xbase, xmax, 8, 8, True, need)
+        caught += got != want
+    assert caught > 50
 
 
 def test_segments_cover_the_window_without_leaving_it():

@@ -135,10 +135,44 @@ def test_random_tile_tables_agree(cfg):
         t = rng.choice([0, 1, 2, 4, 8], size=(M, TILE_CLASSES))
         w = rng.choice([0, 1, 3, 4, 7, 16, 64], size=(M, TILE_CLASSES))
         dec.set_tiles(t, w)
+        dec.set_screen(rng.choice([0, 2, 6, 9], size=(M, TILE_CLASSES)))
         ms, log = dec.evaluate(keys, with_log=True)
         np.testing.assert_array_equal(ms, ref_ms)
         assert feasible_logs_equal(ref_ms, log, ref_log) is None
     dec.close()
+
+
+@pytest.mark.parametrize("cfg,n_chrom,mfma", [("C2", 24, "fp4"), ("C3", 12, "fp4"), ("C5", 6, "fp4"),
+                                               ("C5", 4, "int8")])
+def test_screen_settings_agree(cfg, n_chrom, mfma):
+    """The screening pass only skips tiles that cannot hold a placement: no
+    screening, every row count up to 16, and random per-class tables all give
+    the oracle's placements, in both operand formats."""
+    _say("screen settings %s N=%d %s" % (cfg, n_chrom, mfma))
+    from brkga_fft2_amd.native import TILE_CLASSES
+    prob, n, M, packed = syn_instance(cfg)
+    keys = syn_keys(cfg, n_chrom, n, seed=17)
+    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=oracle_threads(), with_log=True)
+    mfma_counts = {}
+    for rows in (-1, 1, 2, 6, 16):
+        dec = _decoder(packed, mfma=mfma, screen_rows=rows)
+        want = 0 if rows < 0 else rows
+        assert (dec.get_screen() == want).all()
+        ms, log = dec.evaluate(keys, with_log=True)
+        np.testing.assert_array_equal(ms, ref_ms)
+        assert feasible_logs_equal(ref_ms, log, ref_log) is None
+        mfma_counts[rows] = dec.stats()["mfma_ops"]
+        if rows == 6:
+            rng = np.random.default_rng(9)
+            for trial in range(2):
+                dec.set_screen(rng.integers(0, 17, size=(M, TILE_CLASSES)))
+                ms, log = dec.evaluate(keys, with_log=True)
+                np.testing.assert_array_equal(ms, ref_ms)
+                assert feasible_logs_equal(ref_ms, log, ref_log) is None
+        dec.close()
+    # the screen removes MFMA work on the large masks of C5
+    if cfg == "C5":
+        assert mfma_counts[6] < 0.5 * mfma_counts[-1]
 
 
 def test_autotune_keeps_results():
