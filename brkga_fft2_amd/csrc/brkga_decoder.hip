@@ -426,8 +426,9 @@ __device__ __forceinline__ int one_pass(lds_cu32 *band, int stride, const Fold &
 }
 
 // Screening pass of one wave over the tiles [t0, t0 + TW) of a staged window,
-// with the S mask rows a_i = skip + (h - 1 - skip) * i / (S - 1) only (evenly
-// spaced over the rows that meet occupied grid rows): the same MFMA
+// with the S mask rows a_i = skip + floor(i * step / 2^16) only, step =
+// floor(2^16 (h - 1 - skip) / (S - 1)) (evenly spaced over the rows [skip, h)
+// that meet occupied grid rows; any subset would do): the same MFMA
 // correlation as corr_pass, with the wave's own tiles instead of a share of
 // the rows.  Every product is non-negative, so the count over a subset of
 // the rows is zero wherever the full count is: a tile whose subset counts
@@ -438,8 +439,8 @@ __device__ __forceinline__ int one_pass(lds_cu32 *band, int stride, const Fold &
 // word index the window's tiles read, before the lane's k-group offset) and
 // not reported.  Zero counts past the valid region only set bits needlessly.
 template <int TW, int NC, bool F4>
-__device__ __forceinline__ unsigned screen_pass(lds_cu32 *band, int stride, int loff, int skip, int h, int S,
-                                                lds_cu32 *qbits, int qs, int t0, int tcnt, int wlast) {
+__device__ __forceinline__ unsigned screen_pass(lds_cu32 *band, int stride, int loff, int skip, unsigned long long step,
+                                                int S, lds_cu32 *qbits, int qs, int t0, int tcnt, int wlast) {
     constexpr int NA = F4 ? TW + 2 * NC - 1 : TW + NC - 1;
     constexpr int NQ = F4 ? 2 * NC : NC + 1;
     const int lane = threadIdx.x & 63;
@@ -452,11 +453,28 @@ __device__ __forceinline__ unsigned screen_pass(lds_cu32 *band, int stride, int 
     for (int t = 0; t < TW; t++) acc[t] = (AT){};
     lds_cu32 *base = band + loff + (F4 ? g : 0);
     lds_cu32 *qb = qbits + (F4 ? g : 0);
-    for (int i = 0; i < S; i++) {
-        const int a = skip + (S > 1 ? ((h - 1 - skip) * i) / (S - 1) : 0);
-        unsigned qv[NQ];
+    // row i's mask and band words are loaded while row i - 1 correlates
+    unsigned qn[NQ], An[NA];
+    {
+        const int a = skip;
 #pragma unroll
-        for (int c = 0; c < NQ; c++) qv[c] = qb[a * qs + c];
+        for (int c = 0; c < NQ; c++) qn[c] = qb[a * qs + c];
+#pragma unroll
+        for (int j = 0; j < NA; j++) An[j] = base[a * stride + min(t0 + j, wlast)];
+    }
+    for (int i = 0; i < S; i++) {
+        unsigned qv[NQ], Aw[NA];
+#pragma unroll
+        for (int c = 0; c < NQ; c++) qv[c] = qn[c];
+#pragma unroll
+        for (int j = 0; j < NA; j++) Aw[j] = An[j];
+        if (i + 1 < S) {
+            const int a = skip + (int)((step * (unsigned)(i + 1)) >> 16);
+#pragma unroll
+            for (int c = 0; c < NQ; c++) qn[c] = qb[a * qs + c];
+#pragma unroll
+            for (int j = 0; j < NA; j++) An[j] = base[a * stride + min(t0 + j, wlast)];
+        }
         typedef typename std::conditional<F4, v8i, v4i>::type FT;
         FT B[NC];
 #pragma unroll
@@ -464,10 +482,6 @@ __device__ __forceinline__ unsigned screen_pass(lds_cu32 *band, int stride, int 
             if constexpr (F4) B[c] = nib_frag((unsigned)((((unsigned long long)qv[2 * c + 1] << 32) | qv[2 * c]) >> bs));
             else B[c] = bfrag_from_bits(qv[c], qv[c + 1], bs);
         }
-        lds_cu32 *rc = base + a * stride;
-        unsigned Aw[NA];
-#pragma unroll
-        for (int j = 0; j < NA; j++) Aw[j] = rc[min(t0 + j, wlast)];
 #pragma unroll
         for (int j = 0; j < NA; j++) {
             if constexpr (F4) {
@@ -500,8 +514,8 @@ __device__ __forceinline__ unsigned screen_pass(lds_cu32 *band, int stride, int 
     return bits;
 }
 
-// Tiles per screening step of one wave: the accumulators of a step and the
-// B fragments of a row stay within half the full pass's register budget.
+// Tiles per screening step of one wave at most: the accumulators of a step and
+// the B fragments of a row stay within half the full pass's register budget.
 __host__ __device__ constexpr int screen_tiles(int nc) { return nc <= 5 ? 4 : (nc <= 8 ? 2 : 1); }
 
 // Screen of a staged window (tiles [0, tl) of each segment) with S mask rows,
@@ -518,11 +532,30 @@ __device__ __attribute__((always_inline)) unsigned long long screen_window(lds_c
     const int q = (tl + 3) >> 2;
     const int tb = min(tl, wave * q), te = min(tl, tb + q);
     const int wlast = F4 ? tl + 2 * NC - 3 : tl + NC - 2;
+    const unsigned long long step = S > 1 ? ((unsigned long long)(h - 1 - f.skip) << 16) / (unsigned)(S - 1) : 0ull;
     unsigned long long bits = 0;
-    for (int t0 = tb; t0 < te; t0 += TW) {
-        bits |= (unsigned long long)screen_pass<TW, NC, F4>(band, stride, f.loff, f.skip, h, S, qbits, qs, t0,
-                                                            min(TW, te - t0), wlast) << t0;
-        mfma += (unsigned long long)TW * NC * S;
+    for (int t0 = tb; t0 < te;) {
+        // a step takes the wave's remaining tiles up to TW, so no tile is
+        // computed only to be dropped
+        const int r = te - t0;
+        unsigned b;
+        int tw;
+        if (TW >= 4 && r >= 4) {
+            b = screen_pass<(TW >= 4 ? 4 : 1), NC, F4>(band, stride, f.loff, f.skip, step, S, qbits, qs, t0, 4, wlast);
+            tw = 4;
+        } else if (TW >= 3 && r == 3) {
+            b = screen_pass<(TW >= 3 ? 3 : 1), NC, F4>(band, stride, f.loff, f.skip, step, S, qbits, qs, t0, 3, wlast);
+            tw = 3;
+        } else if (TW >= 2 && r >= 2) {
+            b = screen_pass<(TW >= 2 ? 2 : 1), NC, F4>(band, stride, f.loff, f.skip, step, S, qbits, qs, t0, 2, wlast);
+            tw = 2;
+        } else {
+            b = screen_pass<1, NC, F4>(band, stride, f.loff, f.skip, step, S, qbits, qs, t0, 1, wlast);
+            tw = 1;
+        }
+        bits |= (unsigned long long)b << t0;
+        mfma += (unsigned long long)tw * NC * S;
+        t0 += tw;
     }
     if ((threadIdx.x & 63) == 0) s_bits[wave] = bits;
     __syncthreads();
@@ -774,8 +807,10 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid
             // screening pass (not in the dense mode): the tiles whose counts over
             // `screen` evenly spaced mask rows have no zero hold no placement
             unsigned long long need = tf >= 64 ? ~0ull : (1ull << tf) - 1ull;
-            if (!DENSE && screen > 0 && h - skip > 2 * screen)
+            if (!DENSE && screen > 0 && h - skip > 2 * screen) {
                 need = screen_dispatch<F4>(nc, band, stride, f, h, screen, s_q, qs, tf, s_zw + 2 * ZW_PASS, mfma);
+                if (tid == 0) { const unsigned long long t = tick(); ph[4] += t - t1; t1 = t; }
+            }
             const int k = need ? window_dispatch<F4>(nc, band, stride, f, h, s_q, qs, tf, need, tw0 << 5,
                                                      min(W - w, ((tw0 + tl) << 5) - 1), nrows, s_zw, mfma, tpcap,
                                                      !DENSE)
@@ -795,9 +830,10 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid
 
 // Longest free run (longest run of zero cells) of `rows` grid bit rows in LDS
 // (row r at band + r * rstride words); numba_utils.py:161-188 computes the same
-// value sequentially per row.  L = 2^lg lanes per row (the smallest power of
-// two >= ceil(W / 32), at most 64), lane j scanning the wpl = ceil(ceil(W/32) /
-// L) words [j*wpl, j*wpl + wpl) (cells past W count as occupied): per word the
+// value sequentially per row.  L = 2^lg lanes per row (at most 64, chosen
+// per call from the row count and the row width), lane j scanning the wpl =
+// ceil(ceil(W/32) / L) words [j*wpl, j*wpl + wpl) (cells past W count as
+// occupied): per word the
 // free cells at its start (ctz of the occupied bits) and end (clz) and its
 // longest inner run (longest_free_in_word); the lane keeps its free cells at
 // the segment start (pre) and end (suf) and its longest run (best), and lg
@@ -826,8 +862,15 @@ template <typename Put>
 __device__ __forceinline__ void rows_free_run(lds_cu32 *band, int rstride, int rows, int W, Put put) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int nw = (W + 31) >> 5;   // words of a row
-    int lg = 0;
-    while ((1 << lg) < nw && lg < 6) lg++;
+    // lanes per row: the fewest rounds over the rows, then the cheapest round
+    // (a word costs about 4 units per lane, a butterfly step about 10: two
+    // dependent cross-lane shuffles)
+    int lg = 0, cost = 1 << 30;
+    for (int l = 0; l <= 6 && (1 << l) < 2 * nw; l++) {
+        const int rounds = (rows + (256 >> l) - 1) / (256 >> l);
+        const int c = rounds * (4 * ((nw + (1 << l) - 1) >> l) + 10 * l);
+        if (c < cost) { cost = c; lg = l; }
+    }
     const int L = 1 << lg, wpl = (nw + L - 1) >> lg, rpw = 64 >> lg;
     const int sub = lane & (L - 1);
     for (int r0 = wave * rpw; r0 < rows; r0 += 4 * rpw) {
@@ -968,22 +1011,30 @@ __device__ __forceinline__ int vacancy_rotation(lds_cs16 *vrow, lds_ci32 *den, i
         cw[k] = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
     }
     lds_cs16 *vl = vrow + y0 + lane;
-    for (int a = 0; a < h; a += 4) {
-        int aj[4], d[4];
+    bool live = true;
+    for (int a0 = 0; a0 < h && live; a0 += 64) {
+        // the densities of mask rows a0 .. a0+63, one per lane: each step
+        // reads its four from a register (readlane) instead of a round trip
+        // through LDS before its grid reads can start
+        const int dl = den[min(a0 + lane, h - 1)];
+        const int a1 = min(h, a0 + 64);
+        for (int a = a0; a < a1; a += 4) {
+            int aj[4], d[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int e = __builtin_amdgcn_readfirstlane(den[min(a + j, h - 1)]);
-            d[j] = e >> 16;
-            aj[j] = e & 0xffff;
+            for (int j = 0; j < 4; j++) {
+                const int e = __builtin_amdgcn_readlane(dl, min(a + j, h - 1) - a0);
+                d[j] = e >> 16;
+                aj[j] = e & 0xffff;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int k = 0; k < NK; k++) cw[k] &= __ballot(vl[aj[j] + (k << 6)] >= d[j]);
+            unsigned long long any = 0;
+#pragma unroll
+            for (int k = 0; k < NK; k++) any |= cw[k];
+            if (!any) { live = false; break; }
         }
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int k = 0; k < NK; k++) cw[k] &= __ballot(vl[aj[j] + (k << 6)] >= d[j]);
-        unsigned long long any = 0;
-#pragma unroll
-        for (int k = 0; k < NK; k++) any |= cw[k];
-        if (!any) break;
     }
     // compaction in scalar form: bit-reversed, word k gives lane l the row
     // y = 64k + 63 - l (inverse ballot) and its rank among the passing rows

@@ -55,7 +55,7 @@ class BrkgaStats(ctypes.Structure):
 
 MAX_MACHINES = 64   # BRKGA_MAX_MACHINES
 
-PHASES = ["sequence", "vacancy_filter", "band_staging", "correlation", "group_reduce",
+PHASES = ["sequence", "vacancy_filter", "band_staging", "correlation", "screen",
           "stamp_update", "new_bin", "makespan"]
 
 

@@ -117,7 +117,7 @@ typedef struct brkga_stats {
     double last_kernel_ms;   /* decode kernel time of the last call */
     /* Where a workgroup's time goes, summed over workgroups, in microseconds of
      * the GPU's constant-rate wall clock: [0] sequence decode, [1] vacancy
-     * filter, [2] band staging, [3] correlation, [4] group reduction,
+     * filter, [2] band staging, [3] correlation (full passes), [4] screening pass,
      * [5] stamp + free-run update, [6] new-bin clearing, [7] makespan. */
     double phase_us[8];
     double context_us_max;   /* longest single context */

@@ -183,12 +183,23 @@ def longest_free_in_word(o):
     return L
 
 
-def bit_row_scan(row):
+def choose_lg(rows, nw):
+    """rows_free_run's lanes per row, 2^lg: the cheapest of rounds x (4 per word
+    + 10 per butterfly step) for lg with 2^lg < 2 nw (at most 6)."""
+    lg, cost = 0, 1 << 30
+    for l in range(7):
+        if (1 << l) >= 2 * nw:
+            break
+        rounds = (rows + (256 >> l) - 1) // (256 >> l)
+        c = rounds * (4 * ((nw + (1 << l) - 1) >> l) + 10 * l)
+        if c < cost:
+            cost, lg = c, l
+    return lg
+
+
+def bit_row_scan(row, lg):
     W = len(row)
     nw = (W + 31) // 32
-    lg = 0
-    while (1 << lg) < nw and lg < 6:
-        lg += 1
     L = 1 << lg
     wpl = (nw + L - 1) >> lg
     lanes = []
@@ -261,9 +272,16 @@ def test_bit_row_scan_matches_plain_scan():
             a = rng.randrange(W)
             b = rng.randrange(a, W + 1)
             row[a:b] = [0] * (b - a)
-        assert bit_row_scan(row) == plain_scan(row), W
+        nw = (W + 31) // 32
+        lgs = [l for l in range(7) if (1 << l) < 2 * nw]
+        assert choose_lg(rng.randint(1, 600), nw) in lgs
+        for lg in lgs:   # every lane count the row count can select
+            assert bit_row_scan(row, lg) == plain_scan(row), (W, lg)
     for W in (1, 64, 300, 512, 4096):   # all free, all occupied
-        assert bit_row_scan([0] * W) == W and bit_row_scan([1] * W) == 0
+        for lg in (0, choose_lg(120, (W + 31) // 32)):
+            assert bit_row_scan([0] * W, lg) == W and bit_row_scan([1] * W, lg) == 0
+    # a 120-row C5 stamp on 500-cell rows takes one round of two lanes per row
+    assert choose_lg(120, 16) == 1 and choose_lg(1, 16) == 2
 
 
 # ---- fp4 (e2m1) operands: 32x32x64 MFMA, lane (row/col = l % 32, g = l // 32)
