@@ -111,10 +111,13 @@ def _timed_decode(config, pop, steps, warmup, seed, dev, autotune_reps=0, only_t
     if autotune_reps:
         from brkga_fft2_amd import autotune
         t_def, w_def = dec.get_tiles()
+        s_def = dec.get_screen()
         rec = autotune.autotune(dec, packed, keys[0], reps=autotune_reps)
         tuned = {k: rec[k] for k in ("default_ms", "tuned_ms", "trials")}
         tuned["changed_entries"] = int(np.sum(np.asarray(rec["tiles_per_pass"]) != t_def) +
-                                       np.sum(np.asarray(rec["window_tiles"]) != w_def))
+                                       np.sum(np.asarray(rec["window_tiles"]) != w_def) +
+                                       np.sum(np.asarray(rec["screen_rows"]) != s_def))
+        tuned["changed"] = autotune.changed_entries(packed, (t_def, w_def, s_def), rec)
         _say("%s autotune %d trials, default %.3f ms -> tuned %.3f ms, %d table entries changed" %
              (config, rec["trials"], rec["default_ms"], rec["tuned_ms"], tuned["changed_entries"]))
     for s in range(warmup):
@@ -481,7 +484,7 @@ def main():
     cfg = SYNTHETIC_CONFIGS[args.config]
     pop = args.pop or cfg["pop"]
     dev = torch.device("cuda:0")
-    reps = args.autotune if args.autotune >= 0 else (0 if args.config == "C5" else 3)
+    reps = args.autotune if args.autotune >= 0 else (2 if args.config == "C5" else 3)
     _say("config %s pop=%d steps=%d warmup=%d" % (args.config, pop, args.steps, args.warmup))
     if args.dense and not args.only_timed:
         ap.error("--dense needs --only-timed (the full-shape decode is not the headline)")

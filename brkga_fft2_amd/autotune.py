@@ -1,17 +1,22 @@
 """Tile autotuner for the decoder's collision passes.
 
 The exact correlation runs in passes of up to 8 MFMA x-tiles (32 placements
-each) over LDS windows of up to 64 x-tiles (``csrc/brkga_decoder.hip``).  The
+each) over LDS windows of up to 64 x-tiles (``csrc/brkga_decoder.hip``),
+after a screening pass that correlates a few evenly spaced mask rows over
+every tile and lets the full passes skip the tiles with no zero count.  The
 best pass and window sizes depend on the mask width (how many 32-column
-Toeplitz chunks, ceil((w + 31) / 32), feed each tile, hence register pressure) and on the plate width (how
-many tiles a row has).  ``autotune`` times the decoder on a sample population
-for each (plate size, mask width class) entry of the tile table and keeps the
-fastest setting, coordinate by coordinate; ``apply_cached`` reloads a result
-from a JSON cache keyed by the instance's shape signature and the GPU name.
+Toeplitz chunks, ceil((w + 31) / 32), feed each tile, hence register
+pressure) and on the plate width (how many tiles a row has); the best number
+of screening rows on how cluttered the plates get (more rows reject more
+tiles, and cost more).  ``autotune`` times the decoder on a sample
+population for each (plate size, mask width class) entry of the tile table
+and keeps the fastest setting, coordinate by coordinate; ``apply_cached``
+reloads a result from a JSON cache keyed by the instance's shape signature
+and the GPU name.
 
 Every table entry gives bit-identical results (the tiling only changes which
-MFMAs compute the same exact counts); ``autotune`` re-checks that on the
-sample before returning.
+MFMAs compute the same exact counts, the screen only skips tiles that hold
+no placement); ``autotune`` re-checks that on the sample before returning.
 """
 import hashlib
 import json
@@ -23,6 +28,7 @@ from .native import TILE_CLASSES
 
 TILE_CHOICES = (1, 2, 4, 8)
 WINDOW_CHOICES = (4, 8, 16, 32, 64)
+SCREEN_CHOICES = (0, 3, 4, 5, 6, 8, 10)
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "brkga_mi355x", "tiles.json")
 
 
@@ -73,16 +79,17 @@ def _time(dec, keys, out, stream, reps):
 
 
 def autotune(dec, packed, keys, reps=3, min_gain=0.02, cache=None, tiles=TILE_CHOICES,
-             windows=WINDOW_CHOICES, log=None):
+             windows=WINDOW_CHOICES, screens=SCREEN_CHOICES, log=None):
     """Tune ``dec``'s tile table on the device population ``keys``; returns a record.
 
-    Coordinate descent over (plate size, width class, {tiles per pass, window
-    tiles}): a candidate replaces the current value when its median kernel
-    time is at least ``min_gain`` lower than the best so far and, measured
-    again, than the incumbent re-measured beside it (kernel times vary by a
-    few percent between launches).  The tuned table stays set on
-    ``dec``; with ``cache`` (a JSON path) the record is stored under the
-    instance signature.
+    Coordinate descent over the knobs {screening rows, tiles per pass, window
+    tiles}: first each knob for every width class of a plate size at once,
+    then per (plate size, width class).  A candidate replaces the current
+    value when its median kernel time is at least ``min_gain`` lower than the
+    best so far and, measured again, than the incumbent re-measured beside it
+    (kernel times vary by a few percent between launches).  The tuned table
+    stays set on ``dec``; with ``cache`` (a JSON path) the record is stored
+    under the instance signature.
     """
     import torch
     stream = torch.cuda.current_stream(keys.device)
@@ -90,42 +97,68 @@ def autotune(dec, packed, keys, reps=3, min_gain=0.02, cache=None, tiles=TILE_CH
     out = torch.empty(n, dtype=torch.float64, device=keys.device)
     ref = torch.empty(n, dtype=torch.float64, device=keys.device)
     t_tab, w_tab = dec.get_tiles()
+    s_tab = dec.get_screen()
+    tabs = (t_tab, w_tab, s_tab)
+    names = ("tiles", "window", "screen")
+
+    def apply():
+        dec.set_tiles(t_tab, w_tab)
+        dec.set_screen(s_tab)
+
+    def value(which, nc, v):
+        # a pass size above the width's register budget means that budget
+        return min(v, _default_tiles(nc)) if which == 0 else v
+
     dec.evaluate_device(keys, ref, stream=stream)   # also warms the kernel up
     torch.cuda.synchronize()
-    base_ms = best_ms = _time(dec, keys, out, stream, reps)
-    trials = 0
-    for plate, ncs in sorted(tile_classes(packed).items()):
-        ms_rows = _machines_of(packed, plate)
+    base_ms = _time(dec, keys, out, stream, reps)
+    state = {"best": base_ms, "trials": 0}
+
+    def attempt(which, rows, ncs, v, label):
+        """Set knob `which` to v for classes ncs of machines rows; keep it if faster."""
+        tab = tabs[which]
+        old = {nc: int(tab[rows[0], nc]) for nc in ncs}
+        new = {nc: value(which, nc, v) for nc in ncs}
+        if new == old:
+            return
         for nc in ncs:
-            for which, choices in ((0, tiles), (1, windows)):
-                tab = t_tab if which == 0 else w_tab
+            tab[rows, nc] = new[nc]
+        apply()
+        ms = _time(dec, keys, out, stream, reps)
+        state["trials"] += 1
+        if log:
+            log("autotune: %s %s %d -> %.3f ms (best %.3f)" % (label, names[which], v, ms, state["best"]))
+        if ms < state["best"] * (1.0 - min_gain):
+            # confirm against the incumbent, measured again right now
+            for nc in ncs:
+                tab[rows, nc] = old[nc]
+            apply()
+            inc = _time(dec, keys, out, stream, reps)
+            for nc in ncs:
+                tab[rows, nc] = new[nc]
+            apply()
+            ms = _time(dec, keys, out, stream, reps)
+            state["trials"] += 2
+            if ms < inc * (1.0 - min_gain):
+                state["best"] = ms
+                return
+        for nc in ncs:
+            tab[rows, nc] = old[nc]
+        apply()
+
+    knobs = ((2, screens), (0, tiles), (1, windows))
+    for plate, ncs in sorted(tile_classes(packed).items()):
+        rows = _machines_of(packed, plate)
+        label = "plate %dx%d" % plate
+        for which, choices in knobs:          # every class of the plate at once
+            for v in choices:
+                attempt(which, rows, ncs, v, label + " all classes")
+        for nc in ncs:                        # then class by class
+            for which, choices in knobs:
                 for v in choices:
                     if which == 0 and v > _default_tiles(nc):
-                        continue   # the width's register budget caps the pass size
-                    cur = int(tab[ms_rows[0], nc])
-                    if v == cur:
                         continue
-                    tab[ms_rows, nc] = v
-                    dec.set_tiles(t_tab, w_tab)
-                    ms = _time(dec, keys, out, stream, reps)
-                    trials += 1
-                    if log:
-                        log("autotune: plate %dx%d nc %d %s %d -> %.3f ms (best %.3f)" %
-                            (plate[0], plate[1], nc, "tiles" if which == 0 else "window", v, ms, best_ms))
-                    if ms < best_ms * (1.0 - min_gain):
-                        # confirm against the incumbent, measured again right now
-                        tab[ms_rows, nc] = cur
-                        dec.set_tiles(t_tab, w_tab)
-                        inc = _time(dec, keys, out, stream, reps)
-                        tab[ms_rows, nc] = v
-                        dec.set_tiles(t_tab, w_tab)
-                        ms = _time(dec, keys, out, stream, reps)
-                        trials += 2
-                        if ms < inc * (1.0 - min_gain):
-                            best_ms = ms
-                            continue
-                    tab[ms_rows, nc] = cur
-                dec.set_tiles(t_tab, w_tab)
+                    attempt(which, rows, [nc], v, label + " nc %d" % nc)
     dec.evaluate_device(keys, out, stream=stream)
     torch.cuda.synchronize()
     if not torch.equal(out, ref):
@@ -133,11 +166,25 @@ def autotune(dec, packed, keys, reps=3, min_gain=0.02, cache=None, tiles=TILE_CH
     final_ms = _time(dec, keys, out, stream, reps)
     name = torch.cuda.get_device_name(keys.device)
     rec = {"signature": signature(packed, n, name), "device": name, "population": int(n),
-           "default_ms": base_ms, "tuned_ms": final_ms, "trials": trials,
-           "tiles_per_pass": t_tab.tolist(), "window_tiles": w_tab.tolist()}
+           "default_ms": base_ms, "tuned_ms": final_ms, "trials": state["trials"],
+           "tiles_per_pass": t_tab.tolist(), "window_tiles": w_tab.tolist(), "screen_rows": s_tab.tolist()}
     if cache:
         save(cache, rec)
     return rec
+
+
+def changed_entries(packed, defaults, rec):
+    """The entries ``rec`` moved from ``defaults`` (tiles, windows, screens), as
+    ``[plate, width class, knob, default, tuned]`` for the classes the instance uses."""
+    out = []
+    tuned = (rec["tiles_per_pass"], rec["window_tiles"], rec["screen_rows"])
+    for plate, ncs in sorted(tile_classes(packed).items()):
+        m = _machines_of(packed, plate)[0]
+        for nc in ncs:
+            for name, d, t in zip(("tiles", "window", "screen"), defaults, tuned):
+                if int(d[m][nc]) != int(t[m][nc]):
+                    out.append(["%dx%d" % plate, int(nc), name, int(d[m][nc]), int(t[m][nc])])
+    return out
 
 
 def load(cache):
@@ -166,4 +213,6 @@ def apply_cached(dec, packed, population, cache=DEFAULT_CACHE):
     if rec is None:
         return None
     dec.set_tiles(rec["tiles_per_pass"], rec["window_tiles"])
+    if "screen_rows" in rec:
+        dec.set_screen(rec["screen_rows"])
     return rec

@@ -749,9 +749,12 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid
                              unsigned long long &groups, unsigned long long &mfma, unsigned long long *ph,
                              int tpcap, int wcap, int screen) {
     const int tid = threadIdx.x, lane = tid & 63;
-    // mask bits of the rotation -> LDS (read by every wave for every group)
+    // mask bits of the rotation -> LDS (read by every wave for every group);
+    // timed with the band staging, whose first barrier they share
+    unsigned long long tq = tid == 0 ? tick() : 0;
     wg_copy<unsigned>(h * qs, [&](int k) { return qbits_g[k]; },
                       [&](int k, unsigned v) { s_q[k] = v; });
+    if (tid == 0) ph[2] += tick() - tq;
     const int T = (W - w + 1 + 31) >> 5;
     int stride, rcap;
     const int TW = band_window(T, cw, h, band_words, wcap, &stride, &rcap);
@@ -1000,6 +1003,9 @@ __global__ __launch_bounds__(ORDER_BUCKETS) void order_scatter_kernel(const int 
 // past H - h read rows up to H + 62 (inside the LDS allocation: the layout
 // pads the vacancy arrays) and are cleared by the initial mask.  The passing
 // top rows go to cand in descending order; returns their count.
+#ifndef VAC_STEP
+#define VAC_STEP 4   // mask rows per step of the vacancy filter (their reads are in flight together)
+#endif
 template <int NK>
 __device__ __forceinline__ int vacancy_rotation(lds_cs16 *vrow, lds_ci32 *den, int h, int ny, short *cand,
                                                 int y0) {
@@ -1018,16 +1024,16 @@ __device__ __forceinline__ int vacancy_rotation(lds_cs16 *vrow, lds_ci32 *den, i
         // through LDS before its grid reads can start
         const int dl = den[min(a0 + lane, h - 1)];
         const int a1 = min(h, a0 + 64);
-        for (int a = a0; a < a1; a += 4) {
-            int aj[4], d[4];
+        for (int a = a0; a < a1; a += VAC_STEP) {
+            int aj[VAC_STEP], d[VAC_STEP];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < VAC_STEP; j++) {
                 const int e = __builtin_amdgcn_readlane(dl, min(a + j, h - 1) - a0);
                 d[j] = e >> 16;
                 aj[j] = e & 0xffff;
             }
 #pragma unroll
-            for (int j = 0; j < 4; j++)
+            for (int j = 0; j < VAC_STEP; j++)
 #pragma unroll
                 for (int k = 0; k < NK; k++) cw[k] &= __ballot(vl[aj[j] + (k << 6)] >= d[j]);
             unsigned long long any = 0;

@@ -191,6 +191,25 @@ def test_autotune_keeps_results():
     dec.close()
 
 
+def test_autotune_recovers_the_screen():
+    """From a table with screening off, the plate-wide round turns it back on:
+    on C5's large masks that is the difference between correlating every tile
+    in full and skipping most of them."""
+    _say("autotune C5 from screen off")
+    import torch
+    from brkga_fft2_amd import autotune
+    prob, n, M, packed = syn_instance("C5")
+    keys = syn_keys("C5", 256, n, seed=4)
+    dec = _decoder(packed, screen_rows=-1)
+    assert (dec.get_screen() == 0).all()
+    dk = torch.from_numpy(keys).cuda()
+    rec = autotune.autotune(dec, packed, dk, reps=1, tiles=(8,), windows=(64,), screens=(0, 6))
+    used = sorted({autotune.chunk_class(w) for w in packed["rot_w"]})
+    assert (np.asarray(rec["screen_rows"])[:, used] > 0).any()
+    assert rec["tuned_ms"] < 0.8 * rec["default_ms"], rec
+    dec.close()
+
+
 def test_reference_golden_p75m2_0():
     _say("golden P75M2-0")
     prob, n, M, packed = ref_instance("P75M2-0")
