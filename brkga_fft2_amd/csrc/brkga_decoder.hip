@@ -78,8 +78,9 @@ typedef LDSP const int lds_ci32;
 #define GROUP_ROWS 32
 #define LDS_RESERVE 2048    // static LDS of the kernel + slack
 #define XKEY 8192           // key = m * XKEY + x ; plates must have W < XKEY
-#define ZW_WAVE (8 * 16)    // zero masks per wave and pass: up to 8 tiles x 16 accumulators
-#define ZW_PASS (4 * ZW_WAVE)
+// Zero masks per wave and pass: up to 8 tiles (4 at three workgroups per CU)
+// x 16 accumulators; a pass of the four waves holds 4 x that.
+__host__ __device__ constexpr int zw_wave(int wpc) { return (wpc >= 3 ? 4 : 8) * 16; }
 #define KEY_NONE 0x7fffffff
 
 enum {
@@ -135,6 +136,7 @@ struct LaunchArgs {
     double *ctx_ms;          // [n_chrom * M]
     int *log;                // nullable [n_chrom][n_parts][6]
     const int *order;        // nullable [n_chrom * M]: contexts in decode order
+    int wpc;                 // workgroups per CU of the launch's occupancy
 };
 
 // Constant-rate (100 MHz on MI300-class parts) wall clock, read by thread 0
@@ -349,7 +351,7 @@ __device__ __forceinline__ void corr_pass(lds_cu32 *band, int stride, int loff, 
 // half (m/4)%2.  Lanes 0..31 each scan one row m: candidate m & (2^rlog - 1)
 // at x offset segx (the lane's segment of a folded group, 0 unfolded); every
 // wave computes the result.
-template <int TP>
+template <int TP, int ZW>
 __device__ __forceinline__ int decode_zero(lds_u64 *zp, int xbase, int xmax, int nrows, int rlog, int segx) {
     const int lane = threadIdx.x & 63;
     const int m = lane & 31;
@@ -359,7 +361,7 @@ __device__ __forceinline__ int decode_zero(lds_u64 *zp, int xbase, int xmax, int
 #pragma unroll
     for (int t = 0; t < TP; t++) {   // all 4 x TP reads in flight together
         const int e = t * 16 + i;
-        const unsigned long long z = zp[e] & zp[ZW_WAVE + e] & zp[2 * ZW_WAVE + e] & zp[3 * ZW_WAVE + e];
+        const unsigned long long z = zp[e] & zp[ZW + e] & zp[2 * ZW + e] & zp[3 * ZW + e];
         bits[t] = (unsigned)(z >> half);
     }
     int key = KEY_NONE;
@@ -402,27 +404,32 @@ __device__ __forceinline__ int decode_zero(lds_u64 *zp, int xbase, int xmax, int
 struct Fold { int loff, rlog, segx, skip; };
 
 // Tiles per pass shrink as the chunk count grows so that the accumulators,
-// the B fragments and the A fragments in flight fit the register budget.
+// the B fragments and the A fragments in flight fit the register budget: 256
+// VGPRs at WPC = 2 workgroups (two waves) per SIMD, 168 at WPC = 3.  After
+// the screening pass most runs of tiles to correlate are short, and 4-tile
+// passes decode as fast as 8-tile ones.
 #define MAX_NC 12
 #define TUNE_NC BRKGA_TILE_CLASSES
 #define TILE_ENTRY 4        // bytes per tile-table entry
 #define SCREEN_DEFAULT 6    // mask rows of the screening pass unless set otherwise
-__host__ __device__ constexpr int pass_tiles(int nc) { return nc <= 5 ? 8 : (nc <= 8 ? 4 : (nc <= MAX_NC ? 2 : 1)); }
+__host__ __device__ constexpr int pass_tiles(int nc, int wpc = 2) {
+    return wpc >= 3 ? (nc <= 4 ? 4 : (nc <= 8 ? 2 : 1)) : (nc <= 5 ? 8 : (nc <= 8 ? 4 : (nc <= MAX_NC ? 2 : 1)));
+}
 
 // One pass of TP tiles: correlate, combine the waves' zero masks, decode.
 // Passes alternate between two mask buffers, so one barrier per pass is
 // enough: a wave can only write buffer p%2 again (pass p+2) after every wave
 // has passed the barrier of pass p+1, i.e. has finished decoding pass p.
-template <int TP, int NC, bool F4>
+template <int TP, int NC, bool F4, int ZW>
 __device__ __forceinline__ int one_pass(lds_cu32 *band, int stride, const Fold &f, int a0, int a1,
                                         lds_cu32 *qbits, int qs, int t0, int xbase, int xmax, int nrows,
                                         lds_u64 *s_zw, int &parity) {
-    lds_u64 *zp = s_zw + parity * ZW_PASS;
+    lds_u64 *zp = s_zw + parity * 4 * ZW;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    corr_pass<TP, NC, F4>(band, stride, f.loff, a0, a1, qbits, qs, t0, TP, zp + wave * ZW_WAVE);
+    corr_pass<TP, NC, F4>(band, stride, f.loff, a0, a1, qbits, qs, t0, TP, zp + wave * ZW);
     __syncthreads();
     parity ^= 1;
-    return decode_zero<TP>(zp, xbase + (t0 << 5), xmax, nrows, f.rlog, f.segx);
+    return decode_zero<TP, ZW>(zp, xbase + (t0 << 5), xmax, nrows, f.rlog, f.segx);
 }
 
 // Screening pass of one wave over the tiles [t0, t0 + TW) of a staged window,
@@ -516,18 +523,20 @@ __device__ __forceinline__ unsigned screen_pass(lds_cu32 *band, int stride, int 
 
 // Tiles per screening step of one wave at most: the accumulators of a step and
 // the B fragments of a row stay within half the full pass's register budget.
-__host__ __device__ constexpr int screen_tiles(int nc) { return nc <= 5 ? 4 : (nc <= 8 ? 2 : 1); }
+__host__ __device__ constexpr int screen_tiles(int nc, int wpc) {
+    return wpc >= 3 ? (nc <= 3 ? 4 : (nc <= 6 ? 2 : 1)) : (nc <= 5 ? 4 : (nc <= 8 ? 2 : 1));
+}
 
 // Screen of a staged window (tiles [0, tl) of each segment) with S mask rows,
 // run by the whole workgroup: the waves take consecutive quarters of the
 // tiles, and the OR of their bits (through s_bits, 4 x u64 of LDS) is the
 // mask of tiles the full passes must correlate.  mfma counts the wave's MFMA
 // instructions.
-template <int NC, bool F4>
+template <int NC, bool F4, int WPC>
 __device__ __attribute__((always_inline)) unsigned long long screen_window(lds_cu32 *band, int stride, const Fold &f,
                                                                            int h, int S, lds_cu32 *qbits, int qs, int tl,
                                                                            lds_u64 *s_bits, unsigned long long &mfma) {
-    constexpr int TW = screen_tiles(NC);
+    constexpr int TW = screen_tiles(NC, WPC);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int q = (tl + 3) >> 2;
     const int tb = min(tl, wave * q), te = min(tl, tb + q);
@@ -562,11 +571,11 @@ __device__ __attribute__((always_inline)) unsigned long long screen_window(lds_c
     return s_bits[0] | s_bits[1] | s_bits[2] | s_bits[3];
 }
 
-template <bool F4>
+template <bool F4, int WPC>
 __device__ __forceinline__ unsigned long long screen_dispatch(int nc, lds_cu32 *band, int stride, const Fold &f, int h,
                                                               int S, lds_cu32 *qbits, int qs, int tl, lds_u64 *s_bits,
                                                               unsigned long long &mfma) {
-#define SCASE(K) case K: return screen_window<K, F4>(band, stride, f, h, S, qbits, qs, tl, s_bits, mfma);
+#define SCASE(K) case K: return screen_window<K, F4, WPC>(band, stride, f, h, S, qbits, qs, tl, s_bits, mfma);
     switch (nc) {
         SCASE(1) SCASE(2) SCASE(3) SCASE(4) SCASE(5) SCASE(6) SCASE(7) SCASE(8) SCASE(9) SCASE(10) SCASE(11) SCASE(12)
         default:
@@ -590,11 +599,11 @@ __device__ __forceinline__ unsigned long long screen_dispatch(int nc, lds_cu32 *
 // t_end; in a folded group the later passes of segment 0 come before the
 // current ones of the other segments).  mfma counts the MFMA instructions
 // issued.
-template <int NC, bool F4>
+template <int NC, bool F4, int WPC>
 __device__ __attribute__((always_inline)) int window_passes(lds_cu32 *band, int stride, const Fold &f, int h,
                              lds_cu32 *qbits, int qs, unsigned long long need, int xbase, int xmax, int nrows,
                              lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
-    constexpr int TPM = pass_tiles(NC);
+    constexpr int TPM = pass_tiles(NC, WPC);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int a0 = f.skip + (((h - f.skip) * wave) >> 2), a1 = f.skip + (((h - f.skip) * (wave + 1)) >> 2);
     int key = KEY_NONE, parity = 0;
@@ -604,16 +613,16 @@ __device__ __attribute__((always_inline)) int window_passes(lds_cu32 *band, int 
         const int rem = rest ? (int)__builtin_ctzll(rest) : 64 - t0;   // needed tiles from t0 on, in a run
         int k, tp;
         if (TPM >= 8 && tpcap >= 8 && rem >= 6) {
-            k = one_pass<(TPM >= 8 ? 8 : 1), NC, F4>(band, stride, f, a0, a1, qbits, qs, t0, xbase, xmax, nrows, s_zw, parity);
+            k = one_pass<(TPM >= 8 ? 8 : 1), NC, F4, zw_wave(WPC)>(band, stride, f, a0, a1, qbits, qs, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 8;
         } else if (TPM >= 4 && tpcap >= 4 && rem >= 3) {
-            k = one_pass<(TPM >= 4 ? 4 : 1), NC, F4>(band, stride, f, a0, a1, qbits, qs, t0, xbase, xmax, nrows, s_zw, parity);
+            k = one_pass<(TPM >= 4 ? 4 : 1), NC, F4, zw_wave(WPC)>(band, stride, f, a0, a1, qbits, qs, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 4;
         } else if (TPM >= 2 && tpcap >= 2 && rem >= 2) {
-            k = one_pass<(TPM >= 2 ? 2 : 1), NC, F4>(band, stride, f, a0, a1, qbits, qs, t0, xbase, xmax, nrows, s_zw, parity);
+            k = one_pass<(TPM >= 2 ? 2 : 1), NC, F4, zw_wave(WPC)>(band, stride, f, a0, a1, qbits, qs, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 2;
         } else {
-            k = one_pass<1, NC, F4>(band, stride, f, a0, a1, qbits, qs, t0, xbase, xmax, nrows, s_zw, parity);
+            k = one_pass<1, NC, F4, zw_wave(WPC)>(band, stride, f, a0, a1, qbits, qs, t0, xbase, xmax, nrows, s_zw, parity);
             tp = 1;
         }
         mfma += (unsigned long long)tp * (a1 - a0) * NC;
@@ -627,7 +636,7 @@ __device__ __attribute__((always_inline)) int window_passes(lds_cu32 *band, int 
 
 // Generic chunk count (NC above the compiled set): one tile per pass, runtime
 // chunk loop, one band word (int8) or two (fp4) per MFMA.
-template <bool F4>
+template <bool F4, int WPC>
 __device__ int window_passes_generic(lds_cu32 *band, int stride, const Fold &f, int h, int nc,
                                      lds_cu32 *qbits, int qs, int tl, int xbase, int xmax, int nrows,
                                      lds_u64 *s_zw, unsigned long long &mfma, bool early) {
@@ -653,26 +662,27 @@ __device__ int window_passes_generic(lds_cu32 *band, int stride, const Fold &f, 
             }
         }
         mfma += (unsigned long long)(a1 - a0) * nc;
-        lds_u64 *zp = s_zw + parity * ZW_PASS;
-        store_zero_masks<1>(acc, 1, zp + wave * ZW_WAVE);
+        constexpr int ZW = zw_wave(WPC);
+        lds_u64 *zp = s_zw + parity * 4 * ZW;
+        store_zero_masks<1>(acc, 1, zp + wave * ZW);
         __syncthreads();
         parity ^= 1;
-        const int k = decode_zero<1>(zp, xbase + (t0 << 5), xmax, nrows, f.rlog, f.segx);
+        const int k = decode_zero<1, ZW>(zp, xbase + (t0 << 5), xmax, nrows, f.rlog, f.segx);
         key = k < key ? k : key;
         if (early && key < xbase + ((t0 + 1) << 5)) break;
     }
     return key;
 }
 
-template <bool F4>
+template <bool F4, int WPC>
 __device__ __forceinline__ int window_dispatch(int nc, lds_cu32 *band, int stride, const Fold &f, int h,
                                lds_cu32 *qbits, int qs, int tl, unsigned long long need, int xbase, int xmax,
                                int nrows, lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
-#define CASE(K) case K: return window_passes<K, F4>(band, stride, f, h, qbits, qs, need, xbase, xmax, nrows, s_zw, mfma, tpcap, early);
+#define CASE(K) case K: return window_passes<K, F4, WPC>(band, stride, f, h, qbits, qs, need, xbase, xmax, nrows, s_zw, mfma, tpcap, early);
     switch (nc) {
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
         default:
-            return window_passes_generic<F4>(band, stride, f, h, nc, qbits, qs, tl, xbase, xmax, nrows, s_zw, mfma, early);
+            return window_passes_generic<F4, WPC>(band, stride, f, h, nc, qbits, qs, tl, xbase, xmax, nrows, s_zw, mfma, early);
     }
 #undef CASE
 }
@@ -742,7 +752,7 @@ __device__ __forceinline__ void wg_copy(int n, Load load, Store store) {
 // that only meet them are neither staged nor correlated, except in the dense
 // mode.  nc: Toeplitz chunks of the operand format (32 columns for int8, 64
 // for fp4), cw: the words they span, qs: mask words per row in qbits_g.
-template <bool DENSE, bool F4>
+template <bool DENSE, bool F4, int WPC>
 __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid, int gpw, int W, int h, int w, int nc,
                              int cw, int qs, const unsigned *qbits_g, const short *cand, int ncand, int r0,
                              lds_u32 *band, int band_words, lds_u32 *s_q, lds_u64 *s_zw,
@@ -811,10 +821,10 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid
             // `screen` evenly spaced mask rows have no zero hold no placement
             unsigned long long need = tf >= 64 ? ~0ull : (1ull << tf) - 1ull;
             if (!DENSE && screen > 0 && h - skip > 2 * screen) {
-                need = screen_dispatch<F4>(nc, band, stride, f, h, screen, s_q, qs, tf, s_zw + 2 * ZW_PASS, mfma);
+                need = screen_dispatch<F4, WPC>(nc, band, stride, f, h, screen, s_q, qs, tf, s_zw + 2 * 4 * zw_wave(WPC), mfma);
                 if (tid == 0) { const unsigned long long t = tick(); ph[4] += t - t1; t1 = t; }
             }
-            const int k = need ? window_dispatch<F4>(nc, band, stride, f, h, s_q, qs, tf, need, tw0 << 5,
+            const int k = need ? window_dispatch<F4, WPC>(nc, band, stride, f, h, s_q, qs, tf, need, tw0 << 5,
                                                      min(W - w, ((tw0 + tl) << 5) - 1), nrows, s_zw, mfma, tpcap,
                                                      !DENSE)
                                : KEY_NONE;
@@ -1161,7 +1171,7 @@ __device__ __forceinline__ int winning_prefix(const short *cand, int ncand, int 
 // mode tests them all), and equal placements go to the smaller rotation
 // index, so the order changes no choice.  Returns the rotation, or -1, with
 // its top-left (by, bx); uniform across the workgroup.
-template <bool DENSE, bool F4>
+template <bool DENSE, bool F4, int WPC>
 __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork &Wk, int mach, int rb, int nr, int rf,
                                               int H, int W, int enc, const unsigned *grid,
                                               const short *s_cand, const int *s_ncand, int hmax, lds_u32 *band,
@@ -1184,7 +1194,7 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
         c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
         const int ncr = P.rot_nc[rg], nc4 = P.rot_nc4[rg];
         const unsigned char *tc = P.tiles + ((long long)mach * TUNE_NC + min(ncr, TUNE_NC - 1)) * TILE_ENTRY;
-        const int key = test_rotation<DENSE, F4>(grid, Wk.pw, W, h, w, F4 ? nc4 : ncr, F4 ? 2 * nc4 : ncr,
+        const int key = test_rotation<DENSE, F4, WPC>(grid, Wk.pw, W, h, w, F4 ? nc4 : ncr, F4 ? 2 * nc4 : ncr,
                                                  2 * nc4 + 2, P.qbits + P.qbits_off[rg], cand, ncand, H - enc,
                                              band, Wk.band_words, s_q, s_zw, c_groups, c_mfma, ph,
                                              uni(tc[0]), uni(tc[1]), uni(tc[2]));
@@ -1262,9 +1272,10 @@ __device__ __forceinline__ void stamp_part(const DevProblem &P, const DevWork &W
 }
 
 // DENSE: every placement row, no early exit (the full-shape decode); F4: fp4
-// MFMA operands (else int8).  Every instance gives the same results.
-template <bool DENSE, bool F4>
-__global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, DevWork Wk, LaunchArgs L) {
+// MFMA operands (else int8); WPC: workgroups per CU the register budget is
+// compiled for (2: 256 VGPRs, 3: 168).  Every instance gives the same results.
+template <bool DENSE, bool F4, int WPC>
+__global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, DevWork Wk, LaunchArgs L) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = P.n_parts;
     const int M = P.n_machines;
@@ -1279,7 +1290,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
     short *s_vcache = s_vac + hmax;                         // [vc_bins][hmax] vacancy of bins < vc_bins
     lds_u32 *s_q = (lds_u32 *)(smem + Wk.q_off);            // [hrot * (max nc + 1)]
     lds_u32 *band = (lds_u32 *)(smem + Wk.band_off);        // [band_words]
-    lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);          // [2][4][ZW_WAVE] zero masks, 4 screen masks
+    lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);          // [2][4][zw_wave] zero masks, 4 screen masks
     __shared__ int s_ctx, s_cnt, s_off, s_infeasible, s_nb;
     __shared__ int s_ncand[MAX_ROT];
 
@@ -1303,13 +1314,20 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
     for (bool first = true;; first = false) {
         unsigned long long tc0 = tid == 0 ? tick() : 0, tp = tc0;
         // The first G = gridDim.x positions (G <= n_ctx) go one to each
-        // workgroup, later ones come from the queue.  Workgroups b and b + G/2
-        // of a full launch share a compute unit, so b < G/2 takes position b
-        // and b + G/2 position G - 1 - b: the heaviest context shares its CU
-        // with the lightest of the first round, which finishes early.
+        // workgroup, later ones come from the queue.  In a full launch the
+        // workgroups b, b + G/k, b + 2G/k, ... (k per CU) share a compute unit,
+        // so layer r = b / (G/k) takes its positions in snake order (ascending
+        // for even r, descending for odd r): the heaviest context of the first
+        // round shares its CU with the lightest of the next layer, which
+        // finishes early.
         if (tid == 0) {
-            const int G = (int)gridDim.x, b = (int)blockIdx.x;
-            s_ctx = first ? (b < G / 2 ? b : G / 2 + (G - 1 - b)) : G + (int)atomicAdd(Wk.queue, 1u);
+            const int G = (int)gridDim.x, b = (int)blockIdx.x, per = G / max(1, L.wpc);
+            int pos = b;
+            if (per > 0 && b < per * L.wpc) {
+                const int r = b / per, c = b - r * per;
+                pos = r * per + ((r & 1) ? per - 1 - c : c);
+            }
+            s_ctx = first ? pos : G + (int)atomicAdd(Wk.queue, 1u);
         }
         __syncthreads();
         const long long q = uni(s_ctx);
@@ -1385,7 +1403,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void decode_kernel(DevProblem P, Dev
                 if (tid == 0) ph[1] += tick() - tp;
                 // ---- collision tests, one rotation at a time (whole workgroup) ----
                 int bx = 0, by = 0;
-                const int best = best_placement<DENSE, F4>(P, Wk, mach, rb, nr, P.best_rot[ty], H, W, H - b_minrow[b], grid, s_cand,
+                const int best = best_placement<DENSE, F4, WPC>(P, Wk, mach, rb, nr, P.best_rot[ty], H, W, H - b_minrow[b], grid, s_cand,
                                                        s_ncand, hmax, band, s_q, s_zw, c_tests, c_algo, c_groups,
                                                        c_mfma, ph, by, bx);
                 if (best >= 0) {
@@ -1520,7 +1538,7 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P
     __syncthreads();
     unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int bx = 0, by = 0;
-    const int best = best_placement<false, F4>(P, Wk, m, rb, nr, P.best_rot[ty], H, W, enc, grid, s_cand, s_ncand, hmax, band, s_q,
+    const int best = best_placement<false, F4, 2>(P, Wk, m, rb, nr, P.best_rot[ty], H, W, enc, grid, s_cand, s_ncand, hmax, band, s_q,
                                            s_zw, c0, c1, c2, c3, ph, by, bx);
     if (tid == 0) {
         int *r = A.res + 4 * blockIdx.x;
@@ -1575,10 +1593,20 @@ __global__ void finalize_kernel(const double *ctx_ms, long long n_chrom, int M, 
 
 static thread_local std::string g_create_error;
 
+// LDS layout of one occupancy (workgroups per CU) of the decode kernel.
+struct LdsLayout {
+    int wpc = 0, n_wg = 0;        // n_wg = 0: this occupancy does not fit
+    int vc_bins = 0, q_off = 0, zw_off = 0, bins_off = 0, band_off = 0, band_words = 0;
+    size_t smem = 0;
+};
+
 struct brkga_handle {
     int device = 0;
     int n_cu = 0;
-    int n_wg = 0;
+    int n_wg = 0;          // workgroup slots of the workspace (the larger layout's grid)
+    int last_wg = 0;       // grid of the last decode launch
+    bool force3 = false;   // options.wgs_per_cu == 3
+    LdsLayout lay[2];      // [0]: two workgroups per CU (or the forced count), [1]: three
     int dense = 0;
     int f4 = 1;   // fp4 MFMA operands (else int8)
     std::vector<unsigned char> tiles;   // host copy of DevProblem::tiles
@@ -1884,19 +1912,23 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     int hrot = 1;
     for (int r = 0; r < R; r++) hrot = std::max(hrot, p->rot_h[r]);
     w.hrot = hrot;
-    w.vc_bins = std::min(mb, 16);
-    // (+64 rows: the vacancy filter reads up to 63 rows past a vacancy vector)
-    size_t fixed = (size_t)p->n_parts * (8 + 4 + 4) + (size_t)MAX_ROT * hrot * 4 +
-                   (size_t)MAX_ROT * hmax * 2 + (size_t)hmax * 2 + (size_t)w.vc_bins * hmax * 2 + 64 * 2;
-    fixed = (fixed + 15) & ~(size_t)15;
     size_t qwords = 1;
     for (int r = 0; r < R; r++) qwords = std::max(qwords, (size_t)p->rot_h[r] * (2 * nc4[r] + 2));
-    w.q_off = (int)fixed;
-    fixed += (qwords * 4 + 15) & ~(size_t)15;
-    w.zw_off = (int)fixed;
-    fixed += (size_t)(2 * ZW_PASS + 4) * sizeof(unsigned long long);   // + the screen's tile masks
-    w.bins_off = (int)fixed;
-    fixed += ((size_t)p->n_parts * (3 * sizeof(double) + sizeof(int)) + 15) & ~(size_t)15;
+    // fixed arrays for a vacancy cache of vc bins, offsets into w
+    auto layout = [&](int vc, int wpc) -> size_t {
+        w.vc_bins = vc;
+        // (+64 rows: the vacancy filter reads up to 63 rows past a vacancy vector)
+        size_t f = (size_t)p->n_parts * (8 + 4 + 4) + (size_t)MAX_ROT * hrot * 4 +
+                   (size_t)MAX_ROT * hmax * 2 + (size_t)hmax * 2 + (size_t)vc * hmax * 2 + 64 * 2;
+        f = (f + 15) & ~(size_t)15;
+        w.q_off = (int)f;
+        f += (qwords * 4 + 15) & ~(size_t)15;
+        w.zw_off = (int)f;
+        f += (size_t)(2 * 4 * zw_wave(wpc) + 4) * sizeof(unsigned long long);   // + the screen's tile masks
+        w.bins_off = (int)f;
+        f += ((size_t)p->n_parts * (3 * sizeof(double) + sizeof(int)) + 15) & ~(size_t)15;
+        return f;
+    };
     // band sizes in bytes: the smallest that works (one row group of one tile,
     // and one grid row for the stamp) and one holding a 32-row group of every
     // rotation with at least a 4-tile window (windows shrink at run time when
@@ -1912,28 +1944,72 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
                                        BAND_SLACK) * 4);
         }
     size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (size_t)prop.maxSharedMemoryPerMultiProcessor : 65536;
-    int wpc2 = opt.wgs_per_cu > 0 ? opt.wgs_per_cu : (fixed + need_4 + LDS_RESERVE <= lds_cu / 2 ? 2 : 1);
-    size_t per_wg = lds_cu / wpc2;
-    if (per_wg < fixed + need_min + LDS_RESERVE) {
+    // Two LDS layouts.  lay[0] (the WPC = 2 kernels, the full-shape decode and
+    // the greedy): two workgroups per CU when a 32-row group of every
+    // rotation fits a 4-tile band beside the fixed arrays, else one, with a
+    // vacancy cache of up to 16 bins.  lay[1] (the WPC = 3 kernels of the
+    // pruned decode): three per CU, the vacancy cache giving way first (down
+    // to 2 bins: later bins read their free runs from HBM); absent when that
+    // does not fit.  options.wgs_per_cu 1 or 2 forces lay[0] alone, 3 forces
+    // lay[1] for every pruned decode.
+    if (opt.wgs_per_cu < 0 || opt.wgs_per_cu > 3) {
+        h->err = "wgs_per_cu must be 0 (automatic), 1, 2 or 3";
+        return bail(BRKGA_EINVAL);
+    }
+    const int vc_max = std::min(mb, 16);
+    auto make = [&](int wpc, int vcb, size_t need, LdsLayout &L) -> bool {
+        const size_t fixed = layout(vcb, wpc);
+        const size_t per_wg = lds_cu / wpc;
+        if (per_wg < fixed + need + LDS_RESERVE) return false;
+        L.wpc = wpc;
+        L.n_wg = std::max(1, h->n_cu * wpc);
+        L.vc_bins = w.vc_bins; L.q_off = w.q_off; L.zw_off = w.zw_off; L.bins_off = w.bins_off;
+        L.band_off = (int)fixed;
+        L.band_words = (int)(((per_wg - fixed - LDS_RESERVE) & ~(size_t)15) / 4);
+        L.smem = fixed + (size_t)L.band_words * 4;
+        return true;
+    };
+    const int force = opt.wgs_per_cu;
+    const int w0 = force == 1 || force == 2 ? force : (make(2, vc_max, need_4, h->lay[0]) ? 2 : 1);
+    if (!make(w0, vc_max, need_min, h->lay[0])) {
         char b2[200];
+        const size_t fixed = layout(vc_max, w0), per_wg = lds_cu / w0;
         snprintf(b2, sizeof b2, "a part rotation needs %zu bytes of LDS band, %zu available per workgroup",
                  need_min, per_wg > fixed + LDS_RESERVE ? per_wg - fixed - LDS_RESERVE : (size_t)0);
         h->err = b2;
         return bail(BRKGA_EINVAL);
     }
-    w.band_off = (int)fixed;
-    w.band_words = (int)(((per_wg - fixed - LDS_RESERVE) & ~(size_t)15) / 4);
-    h->smem = fixed + (size_t)w.band_words * 4;
-    h->n_wg = std::max(1, h->n_cu * wpc2);   // persistent grid at the occupancy the LDS allows
+    h->lay[1] = LdsLayout{};
+    if (force == 0 || force == 3) {
+        for (int v = vc_max; v >= std::min(vc_max, 2) && !h->lay[1].n_wg; v--) make(3, v, need_4, h->lay[1]);
+        if (force == 3 && !h->lay[1].n_wg)
+            for (int v = vc_max; v >= std::min(vc_max, 2) && !h->lay[1].n_wg; v--) make(3, v, need_min, h->lay[1]);
+        if (force == 3 && !h->lay[1].n_wg) {
+            h->err = "three workgroups per CU do not fit the LDS for this problem";
+            return bail(BRKGA_EINVAL);
+        }
+    }
+    h->force3 = force == 3;
+    const LdsLayout &L0 = h->lay[0];
+    w.vc_bins = L0.vc_bins; w.q_off = L0.q_off; w.zw_off = L0.zw_off; w.bins_off = L0.bins_off;
+    w.band_off = L0.band_off; w.band_words = L0.band_words;
+    h->smem = L0.smem;
+    h->n_wg = std::max(L0.n_wg, h->lay[1].n_wg);   // workspace slots for the larger grid
     if ((rc = device_workspace(h, mb))) return bail(rc);
-    for (const void *kern : {(const void *)decode_kernel<false, false>, (const void *)decode_kernel<true, false>,
-                             (const void *)decode_kernel<false, true>, (const void *)decode_kernel<true, true>,
+    for (const void *kern : {(const void *)decode_kernel<false, false, 2>, (const void *)decode_kernel<true, false, 2>,
+                             (const void *)decode_kernel<false, true, 2>, (const void *)decode_kernel<true, true, 2>,
                              (const void *)greedy_test_kernel<false>, (const void *)greedy_test_kernel<true>,
                              (const void *)greedy_insert_kernel})
-        if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->smem) != hipSuccess) {
+        if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L0.smem) != hipSuccess) {
             h->err = "hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed";
             return bail(BRKGA_EDEVICE);
         }
+    if (h->lay[1].n_wg)
+        for (const void *kern : {(const void *)decode_kernel<false, false, 3>, (const void *)decode_kernel<false, true, 3>})
+            if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lay[1].smem) != hipSuccess) {
+                h->err = "hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed";
+                return bail(BRKGA_EDEVICE);
+            }
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) { h->err = "hipEventCreate failed"; return bail(BRKGA_EDEVICE); }
     if ((rc = scratch_address(h, h->device, &h->scratch))) return bail(rc);
     *out = h;
@@ -1994,7 +2070,17 @@ static int launch_decode(brkga_handle *h, const void *d_keys, int key_dtype, int
     DevWork wk = h->dw;
     wk.queue = cb.queue;
     const long long n_ctx = n_chrom * M;
-    const int grid = (int)std::min<long long>(h->n_wg, std::max<long long>(n_ctx, 1));
+    // three workgroups per CU for a pruned decode of at least two rounds of
+    // contexts at that occupancy (a single round finishes sooner at two, where
+    // each context runs faster); the layout fields follow the choice
+    const LdsLayout &L3 = h->lay[1];
+    const bool three = !h->dense && L3.n_wg > 0 && (h->force3 || n_ctx >= 2LL * L3.n_wg);
+    const LdsLayout &LL = three ? L3 : h->lay[0];
+    wk.vc_bins = LL.vc_bins; wk.q_off = LL.q_off; wk.zw_off = LL.zw_off; wk.bins_off = LL.bins_off;
+    wk.band_off = LL.band_off; wk.band_words = LL.band_words;
+    const int grid = (int)std::min<long long>(LL.n_wg, std::max<long long>(n_ctx, 1));
+    h->last_wg = grid;
+    L.wpc = LL.wpc;
     hipLaunchKernelGGL(order_cost_kernel, dim3((unsigned)((n_ctx + 3) / 4)), dim3(256), 0, s, h->dp, d_keys,
                        L.key_f64, (long long)n_chrom, h->cost_all, cb.bucket, cb.hist);
     HIPCHK(h, hipGetLastError());
@@ -2002,9 +2088,10 @@ static int launch_decode(brkga_handle *h, const void *d_keys, int key_dtype, int
                        cb.order, cb.queue);
     HIPCHK(h, hipGetLastError());
     if (timed) HIPCHK(h, hipEventRecord(h->ev0, s));
-    auto kern = h->f4 ? (h->dense ? decode_kernel<true, true> : decode_kernel<false, true>)
-                      : (h->dense ? decode_kernel<true, false> : decode_kernel<false, false>);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(WG_THREADS), h->smem, s, h->dp, wk, L);
+    auto kern = three ? (h->f4 ? decode_kernel<false, true, 3> : decode_kernel<false, false, 3>)
+                      : h->f4 ? (h->dense ? decode_kernel<true, true, 2> : decode_kernel<false, true, 2>)
+                              : (h->dense ? decode_kernel<true, false, 2> : decode_kernel<false, false, 2>);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(WG_THREADS), LL.smem, s, h->dp, wk, L);
     HIPCHK(h, hipGetLastError());
     if (timed) {
         HIPCHK(h, hipEventRecord(h->ev1, s));
@@ -2435,7 +2522,7 @@ extern "C" int brkga_get_stats(const brkga_handle *hc, brkga_stats *out) {
     out->context_us_sum = (double)c[ST_CTXSUM] * us_per_tick;
     out->growths = h->growths;
     out->max_bins = h->dw.max_bins;
-    out->n_workgroups = h->n_wg;
+    out->n_workgroups = h->last_wg ? h->last_wg : h->lay[0].n_wg;
     for (int m = 0; m < BRKGA_MAX_MACHINES; m++) {
         const bool used = m < h->prob.n_machines;
         out->machine_contexts[m] = used ? (int64_t)c[ST_MCTX + m] : 0;

@@ -82,8 +82,11 @@ typedef struct brkga_problem {
 
 typedef struct brkga_options {
     int32_t device;          /* HIP device ordinal */
-    int32_t wgs_per_cu;      /* persistent workgroups per compute unit (0 = 2 when the
-                                LDS budget allows, else 1) */
+    int32_t wgs_per_cu;      /* persistent workgroups per compute unit: 0 = automatic (a
+                                decode of at least two rounds of contexts at three per CU
+                                runs three per CU when the LDS allows, others two, or one
+                                when two do not fit); 1 or 2 = always that; 3 = three for
+                                every decode (BRKGA_EINVAL when it does not fit) */
     int32_t max_bins;        /* bins per context in the workspace (0 = min(n_parts, 32));
                                 brkga_evaluate grows it when a context needs more */
     int32_t dense;           /* 1: correlation over every placement row, no early exit
@@ -126,7 +129,7 @@ typedef struct brkga_stats {
      * per machine: peak bins per solution, total bins, solutions, growth events). */
     int64_t growths;         /* workspace growths: brkga_evaluate doubled max_bins after an overflow */
     int32_t max_bins;        /* bins per context the workspace holds now */
-    int32_t n_workgroups;    /* persistent workgroups of a decode launch */
+    int32_t n_workgroups;    /* persistent workgroups of the last decode launch */
     int64_t machine_contexts[BRKGA_MAX_MACHINES];    /* contexts decoded on machine m (feasible or not) */
     int64_t machine_bins[BRKGA_MAX_MACHINES];        /* bins the contexts of machine m opened */
     int32_t machine_peak_bins[BRKGA_MAX_MACHINES];   /* most bins one context of machine m opened */

@@ -175,6 +175,34 @@ def test_screen_settings_agree(cfg, n_chrom, mfma):
         assert mfma_counts[6] < 0.5 * mfma_counts[-1]
 
 
+@pytest.mark.parametrize("cfg,n_chrom", [("C2", 24), ("C3", 600), ("C5", 6)])
+def test_occupancy_settings_agree(cfg, n_chrom):
+    """The decode kernels compiled for two and for three workgroups per CU
+    (their own register budgets, pass sizes and LDS layouts), and one
+    workgroup per CU, give the oracle's placements; C3 at 600 chromosomes
+    (2400 contexts) is large enough for the automatic choice to take three."""
+    _say("occupancy settings %s N=%d" % (cfg, n_chrom))
+    prob, n, M, packed = syn_instance(cfg)
+    keys = syn_keys(cfg, n_chrom, n, seed=29)
+    ref_ms, ref_log, _ = oracle.decode(packed, keys, n_threads=oracle_threads(), with_log=True)
+    grids = {}
+    for wpc in (0, 1, 2, 3):
+        dec = _decoder(packed, wgs_per_cu=wpc)
+        ms, log = dec.evaluate(keys, with_log=True)
+        np.testing.assert_array_equal(ms, ref_ms)
+        assert feasible_logs_equal(ref_ms, log, ref_log) is None
+        grids[wpc] = dec.stats()["n_workgroups"]
+        dec.close()
+    import torch
+    n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+    n_ctx = n_chrom * M
+    for k in (1, 2, 3):
+        assert grids[k] == min(n_ctx, k * n_cu), (k, grids)
+    assert grids[0] == (grids[3] if n_ctx >= 2 * 3 * n_cu else grids[2]), grids
+    if cfg == "C3":
+        assert grids[0] == 3 * n_cu
+
+
 def test_autotune_keeps_results():
     _say("autotune C2")
     import torch
