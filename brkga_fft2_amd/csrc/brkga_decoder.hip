@@ -1016,6 +1016,57 @@ __global__ __launch_bounds__(ORDER_BUCKETS) void order_scatter_kernel(const int 
 #ifndef VAC_STEP
 #define VAC_STEP 4   // mask rows per step of the vacancy filter (their reads are in flight together)
 #endif
+// Mask rows [a, aend) (densest first) against the K candidate words widx[0..K)
+// (word k: top rows y0 + 64k .. + 63), cw[j] the passing bits of word widx[j].
+// Returns true once no candidate is left.  The densities of rows b .. b+63
+// (b a multiple of 64) are held one per lane, so a step's thresholds come from
+// registers (readlane) instead of a round trip through LDS.
+template <int K>
+__device__ __forceinline__ bool vacancy_rows(lds_cs16 *vl, lds_ci32 *den, int h, int a, int aend,
+                                             const int (&widx)[K], unsigned long long (&cw)[K]) {
+    const int lane = threadIdx.x & 63;
+    for (int a0 = a & ~63; a0 < aend; a0 += 64) {
+        const int dl = den[min(a0 + lane, h - 1)];
+        const int a1 = min(aend, a0 + 64);
+        for (int b = max(a, a0); b < a1; b += VAC_STEP) {
+            int aj[VAC_STEP], d[VAC_STEP];
+#pragma unroll
+            for (int j = 0; j < VAC_STEP; j++) {
+                const int e = __builtin_amdgcn_readlane(dl, min(b + j, aend - 1) - a0);
+                d[j] = e >> 16;
+                aj[j] = e & 0xffff;
+            }
+#pragma unroll
+            for (int j = 0; j < VAC_STEP; j++)
+#pragma unroll
+                for (int k = 0; k < K; k++) cw[k] &= __ballot(vl[aj[j] + (widx[k] << 6)] >= d[j]);
+            unsigned long long any = 0;
+#pragma unroll
+            for (int k = 0; k < K; k++) any |= cw[k];
+            if (!any) return true;
+        }
+    }
+    return false;
+}
+
+// The rows after the head over the first KK entries of a compacted word list.
+#define VAC_HEAD 8   // densest rows tested over every candidate word before the list is compacted
+#ifndef VAC_COMPACT
+#define VAC_COMPACT 5   // candidate words (of 64 top rows) from which the list is compacted
+#endif
+template <int NK, int KK>
+__device__ __forceinline__ void vacancy_rows_sub(lds_cs16 *vl, lds_ci32 *den, int h, const int (&widx)[NK],
+                                                 unsigned long long (&cc)[NK]) {
+    constexpr int KE = KK < NK ? KK : NK;   // (KK >= NK is never called; it only keeps the arrays in bounds)
+    int wi[KE];
+    unsigned long long c[KE];
+#pragma unroll
+    for (int j = 0; j < KE; j++) { wi[j] = widx[j]; c[j] = cc[j]; }
+    vacancy_rows<KE>(vl, den, h, VAC_HEAD, h, wi, c);
+#pragma unroll
+    for (int j = 0; j < KE; j++) cc[j] = c[j];
+}
+
 template <int NK>
 __device__ __forceinline__ int vacancy_rotation(lds_cs16 *vrow, lds_ci32 *den, int h, int ny, short *cand,
                                                 int y0) {
@@ -1027,29 +1078,49 @@ __device__ __forceinline__ int vacancy_rotation(lds_cs16 *vrow, lds_ci32 *den, i
         cw[k] = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
     }
     lds_cs16 *vl = vrow + y0 + lane;
-    bool live = true;
-    for (int a0 = 0; a0 < h && live; a0 += 64) {
-        // the densities of mask rows a0 .. a0+63, one per lane: each step
-        // reads its four from a register (readlane) instead of a round trip
-        // through LDS before its grid reads can start
-        const int dl = den[min(a0 + lane, h - 1)];
-        const int a1 = min(h, a0 + 64);
-        for (int a = a0; a < a1; a += VAC_STEP) {
-            int aj[VAC_STEP], d[VAC_STEP];
+    // The densest VAC_HEAD rows over every word first: they end most
+    // candidates.  The words with candidates left are then listed (padded to a
+    // power of two with a dead word, whose reads change nothing) and the other
+    // rows read only those.
+    int widx[NK];
 #pragma unroll
-            for (int j = 0; j < VAC_STEP; j++) {
-                const int e = __builtin_amdgcn_readlane(dl, min(a + j, h - 1) - a0);
-                d[j] = e >> 16;
-                aj[j] = e & 0xffff;
+    for (int k = 0; k < NK; k++) widx[k] = k;
+    if (NK < VAC_COMPACT) {   // few words: no list
+        vacancy_rows<NK>(vl, den, h, 0, h, widx, cw);
+    } else if (!vacancy_rows<NK>(vl, den, h, 0, min(h, VAC_HEAD), widx, cw) && h > VAC_HEAD) {
+        int live = 0, dead = -1;
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            if (cw[k]) widx[live++] = k;
+            else if (dead < 0) dead = k;
+        }
+        const int kk = live <= 1 ? 1 : (live <= 2 ? 2 : (live <= 4 ? 4 : NK));
+        if (kk < NK) {
+#pragma unroll
+            for (int j = 0; j < NK; j++)
+                if (j >= live) widx[j] = dead;   // dead >= 0: live < NK here
+            unsigned long long cc[NK];
+#pragma unroll
+            for (int j = 0; j < NK; j++) {
+                cc[j] = 0;
+#pragma unroll
+                for (int k = 0; k < NK; k++) if (widx[j] == k) cc[j] = cw[k];
+            }
+            switch (kk) {
+                case 1: vacancy_rows_sub<NK, 1>(vl, den, h, widx, cc); break;
+                case 2: vacancy_rows_sub<NK, 2>(vl, den, h, widx, cc); break;
+                default: vacancy_rows_sub<NK, 4>(vl, den, h, widx, cc); break;
             }
 #pragma unroll
-            for (int j = 0; j < VAC_STEP; j++)
+            for (int k = 0; k < NK; k++) cw[k] = 0;
 #pragma unroll
-                for (int k = 0; k < NK; k++) cw[k] &= __ballot(vl[aj[j] + (k << 6)] >= d[j]);
-            unsigned long long any = 0;
+            for (int j = 0; j < NK; j++)
 #pragma unroll
-            for (int k = 0; k < NK; k++) any |= cw[k];
-            if (!any) { live = false; break; }
+                for (int k = 0; k < NK; k++) if (j < live && widx[j] == k) cw[k] = cc[j];
+        } else {
+#pragma unroll
+            for (int k = 0; k < NK; k++) widx[k] = k;
+            vacancy_rows<NK>(vl, den, h, VAC_HEAD, h, widx, cw);
         }
     }
     // compaction in scalar form: bit-reversed, word k gives lane l the row

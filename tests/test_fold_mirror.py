@@ -7,6 +7,8 @@ once candidate 0 has a zero left of every tile still to come; each pass takes
 its minimum key by a scalar selection over the lanes' keys (pass_min_key).
 With screening (screen_window), the passes run over the runs of tiles whose
 counts over a subset of the mask rows have a zero (screen_need).
+The vacancy filter's compacted word list (vacancy_rotation) is restated
+at the end against the plain definition of a candidate top row.
 This restates that schedule in Python on random zero patterns and checks it against the
 plain answer -- the smallest (candidate, x) with zero overlap -- so that a
 wrong segment layout, a gap between segments or a premature exit fails."""
@@ -192,3 +194,73 @@ def test_segments_cover_the_window_without_leaving_it():
                 assert 0 <= s and s + tf <= tl
                 covered.update(range(s, s + tf))
             assert covered == set(range(tl))
+
+
+def vacancy_words(vac, den_sorted, ny, nk, head=8, compact=5):
+    """vacancy_rotation: candidate words of 64 top rows; the densest `head`
+    rows over every word, then (from `compact` words on) the words with
+    candidates left listed and padded to 1, 2 or 4 entries with a dead word
+    whose bits stay zero, and the other rows over the list only."""
+    cw = []
+    for k in range(nk):
+        left = ny - 64 * k
+        cw.append((1 << min(64, left)) - 1)
+
+    def rows(a0, a1, idx, c):
+        for a in range(a0, a1):
+            d, row = den_sorted[a]
+            for j, k in enumerate(idx):
+                bits = 0
+                for lane in range(64):
+                    if vac[64 * k + lane + row] >= d:
+                        bits |= 1 << lane
+                c[j] &= bits
+            if not any(c):
+                return True
+        return False
+
+    h = len(den_sorted)
+    idx = list(range(nk))
+    if nk < compact:
+        rows(0, h, idx, cw)
+        return cw
+    if rows(0, min(h, head), idx, cw) or h <= head:
+        return cw
+    live = [k for k in range(nk) if cw[k]]
+    dead = next(k for k in range(nk) if not cw[k]) if len(live) < nk else None
+    kk = 1 if len(live) <= 1 else (2 if len(live) <= 2 else (4 if len(live) <= 4 else nk))
+    if kk < nk:
+        lst = live + [dead] * (kk - len(live))
+        c = [cw[k] for k in lst]
+        rows(head, h, lst, c)
+        out = [0] * nk
+        for j, k in enumerate(lst[:len(live)]):
+            out[k] = c[j]
+        return out
+    rows(head, h, idx, cw)
+    return cw
+
+
+def test_vacancy_word_compaction_keeps_the_candidates():
+    """The compacted vacancy filter lists exactly the top rows y whose rows
+    y..y+h-1 all have a free run at least the mask row's density."""
+    rng = random.Random(13)
+    for _ in range(300):
+        h = rng.randint(1, 40)
+        ny = rng.randint(1, 64 * 8)
+        nk = (ny + 63) // 64
+        H = ny + h - 1
+        W = 100
+        vac = [rng.choice([W, W, rng.randint(0, W)]) for _ in range(H)]
+        # blocks of full rows so that some words die early and others live
+        for _ in range(rng.randint(0, 6)):
+            a = rng.randrange(H)
+            for r in range(a, min(H, a + rng.randint(1, 80))):
+                vac[r] = rng.randint(0, 20)
+        vac += [0] * (64 * nk + h)     # rows past H read as occupied (masked by cw)
+        den = [rng.randint(1, 60) for _ in range(h)]
+        den_sorted = sorted(((den[a], a) for a in range(h)), key=lambda t: -t[0])
+        want = [y for y in range(ny) if all(vac[y + a] >= den[a] for a in range(h))]
+        cw = vacancy_words(vac, den_sorted, ny, nk)
+        got = [64 * k + i for k in range(nk) for i in range(64) if (cw[k] >> i) & 1]
+        assert got == want
