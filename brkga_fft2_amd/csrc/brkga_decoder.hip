@@ -77,6 +77,11 @@ typedef LDSP const int lds_ci32;
 #define MAX_ROT 4
 #define GROUP_ROWS 32
 #define LDS_RESERVE 2048    // static LDS of the kernel + slack
+#ifndef VC3_MAX
+// Bins whose free runs stay in LDS at three workgroups per CU, at most: the
+// rest of the LDS goes to the band (wider windows, larger row groups)
+#define VC3_MAX 2
+#endif
 #define XKEY 8192           // key = m * XKEY + x ; plates must have W < XKEY
 // Zero masks per wave and pass: up to 8 tiles (4 at three workgroups per CU)
 // x 16 accumulators; a pass of the four waves holds 4 x that.
@@ -2052,7 +2057,7 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     }
     h->lay[1] = LdsLayout{};
     if (force == 0 || force == 3) {
-        for (int v = vc_max; v >= std::min(vc_max, 2) && !h->lay[1].n_wg; v--) make(3, v, need_4, h->lay[1]);
+        for (int v = std::min(vc_max, VC3_MAX); v >= std::min(vc_max, 2) && !h->lay[1].n_wg; v--) make(3, v, need_4, h->lay[1]);
         if (force == 3 && !h->lay[1].n_wg)
             for (int v = vc_max; v >= std::min(vc_max, 2) && !h->lay[1].n_wg; v--) make(3, v, need_min, h->lay[1]);
         if (force == 3 && !h->lay[1].n_wg) {
