@@ -131,7 +131,7 @@ struct DevWork {
     int vc_bins;             // bins whose vacancy vector is cached in LDS
     int band_off, band_words;  // LDS band for staged grid bit rows (32-bit words)
     int zw_off;                // LDS zero masks: [2 passes][4 waves][8 tiles * 16] u64, then 4 u64 screen masks
-    int bins_off;              // LDS bin bookkeeping: area, pt, pth [n_parts] f64, minrow [n_parts] i32
+    int bins_off;              // LDS bin bookkeeping: area, pt, pth [max_bins] f64, minrow [max_bins] i32
 };
 
 struct LaunchArgs {
@@ -1378,9 +1378,9 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
     int *vacs = Wk.vac + wg * (long long)Wk.max_bins * hmax;
     // bin bookkeeping of the running context (a context opens at most n bins)
     LDSP double *b_area = (LDSP double *)(smem + Wk.bins_off);
-    LDSP double *b_pt = b_area + n;
-    LDSP double *b_pth = b_pt + n;
-    LDSP int *b_minrow = (LDSP int *)(b_pth + n);
+    LDSP double *b_pt = b_area + Wk.max_bins;
+    LDSP double *b_pth = b_pt + Wk.max_bins;
+    LDSP int *b_minrow = (LDSP int *)(b_pth + Wk.max_bins);
 
     unsigned long long c_ctx = 0, c_place = 0, c_bins = 0, c_tests = 0, c_groups = 0, c_mfma = 0,
                        c_algo = 0, c_over = 0;
@@ -1669,6 +1669,11 @@ __global__ void finalize_kernel(const double *ctx_ms, long long n_chrom, int M, 
 
 static thread_local std::string g_create_error;
 
+struct LayoutInputs {
+    int n_parts = 0, hrot = 0, hmax = 0, force = 0;
+    size_t qwords = 0, need_min = 0, need_4 = 0, lds_cu = 0;
+};
+
 // LDS layout of one occupancy (workgroups per CU) of the decode kernel.
 struct LdsLayout {
     int wpc = 0, n_wg = 0;        // n_wg = 0: this occupancy does not fit
@@ -1682,6 +1687,7 @@ struct brkga_handle {
     int n_wg = 0;          // workgroup slots of the workspace (the larger layout's grid)
     int last_wg = 0;       // grid of the last decode launch
     bool force3 = false;   // options.wgs_per_cu == 3
+    LayoutInputs li;       // what the LDS layouts depend on besides the bin count
     LdsLayout lay[2];      // [0]: two workgroups per CU (or the forced count), [1]: three
     int dense = 0;
     int f4 = 1;   // fp4 MFMA operands (else int8)
@@ -1820,6 +1826,91 @@ static int device_workspace(brkga_handle *h, int max_bins) {
 }
 
 static int scratch_address(brkga_handle *h, int device, void **out);
+
+// The two LDS layouts of the decode kernels for a workspace of mb bins per
+// context (the bin bookkeeping holds mb bins), and the workspace slot count.
+// lay[0] (the WPC = 2 kernels, the full-shape decode and the greedy): two
+// workgroups per CU when a 32-row group of every rotation fits a 4-tile band
+// beside the fixed arrays, else one, with a vacancy cache of up to 16 bins.
+// lay[1] (the WPC = 3 kernels of the pruned decode): three per CU with a
+// vacancy cache of at most VC3_MAX bins (later bins read their free runs from
+// HBM); absent when that does not fit.  options.wgs_per_cu 1 or 2 forces
+// lay[0] alone, 3 forces lay[1] for every pruned decode.  Called at create
+// and again when the workspace grows.
+static int compute_layouts(brkga_handle *h, int mb) {
+    const LayoutInputs &li = h->li;
+    DevWork &w = h->dw;
+    // fixed arrays for a vacancy cache of vc bins, offsets into w
+    auto layout = [&](int vc, int wpc) -> size_t {
+        w.vc_bins = vc;
+        // (+64 rows: the vacancy filter reads up to 63 rows past a vacancy vector)
+        size_t f = (size_t)li.n_parts * (8 + 4 + 4) + (size_t)MAX_ROT * li.hrot * 4 +
+                   (size_t)MAX_ROT * li.hmax * 2 + (size_t)li.hmax * 2 + (size_t)vc * li.hmax * 2 + 64 * 2;
+        f = (f + 15) & ~(size_t)15;
+        w.q_off = (int)f;
+        f += (li.qwords * 4 + 15) & ~(size_t)15;
+        w.zw_off = (int)f;
+        f += (size_t)(2 * 4 * zw_wave(wpc) + 4) * sizeof(unsigned long long);   // + the screen's tile masks
+        w.bins_off = (int)f;
+        f += ((size_t)mb * (3 * sizeof(double) + sizeof(int)) + 15) & ~(size_t)15;
+        return f;
+    };
+    const int vc_max = std::min(mb, 16);
+    auto make = [&](int wpc, int vcb, size_t need, LdsLayout &L) -> bool {
+        const size_t fixed = layout(vcb, wpc);
+        const size_t per_wg = li.lds_cu / wpc;
+        if (per_wg < fixed + need + LDS_RESERVE) return false;
+        L.wpc = wpc;
+        L.n_wg = std::max(1, h->n_cu * wpc);
+        L.vc_bins = w.vc_bins; L.q_off = w.q_off; L.zw_off = w.zw_off; L.bins_off = w.bins_off;
+        L.band_off = (int)fixed;
+        L.band_words = (int)(((per_wg - fixed - LDS_RESERVE) & ~(size_t)15) / 4);
+        L.smem = fixed + (size_t)L.band_words * 4;
+        return true;
+    };
+    const int force = li.force;
+    LdsLayout l0{}, l1{};
+    const int w0 = force == 1 || force == 2 ? force : (make(2, vc_max, li.need_4, l0) ? 2 : 1);
+    if (!make(w0, vc_max, li.need_min, l0)) {
+        char b2[200];
+        const size_t fixed = layout(vc_max, w0), per_wg = li.lds_cu / w0;
+        snprintf(b2, sizeof b2, "a part rotation needs %zu bytes of LDS band, %zu available per workgroup",
+                 li.need_min, per_wg > fixed + LDS_RESERVE ? per_wg - fixed - LDS_RESERVE : (size_t)0);
+        h->err = b2;
+        return BRKGA_EINVAL;
+    }
+    if (force == 0 || force == 3) {
+        for (int v = std::min(vc_max, VC3_MAX); v >= std::min(vc_max, 2) && !l1.n_wg; v--) make(3, v, li.need_4, l1);
+        if (force == 3 && !l1.n_wg)
+            for (int v = vc_max; v >= std::min(vc_max, 2) && !l1.n_wg; v--) make(3, v, li.need_min, l1);
+        if (force == 3 && !l1.n_wg) {
+            h->err = "three workgroups per CU do not fit the LDS for this problem";
+            return BRKGA_EINVAL;
+        }
+    }
+    h->lay[0] = l0;
+    h->lay[1] = l1;
+    h->force3 = force == 3;
+    w.vc_bins = l0.vc_bins; w.q_off = l0.q_off; w.zw_off = l0.zw_off; w.bins_off = l0.bins_off;
+    w.band_off = l0.band_off; w.band_words = l0.band_words;
+    h->smem = l0.smem;
+    h->n_wg = std::max(l0.n_wg, l1.n_wg);   // workspace slots for the larger grid
+    for (const void *kern : {(const void *)decode_kernel<false, false, 2>, (const void *)decode_kernel<true, false, 2>,
+                             (const void *)decode_kernel<false, true, 2>, (const void *)decode_kernel<true, true, 2>,
+                             (const void *)greedy_test_kernel<false>, (const void *)greedy_test_kernel<true>,
+                             (const void *)greedy_insert_kernel})
+        if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l0.smem) != hipSuccess) {
+            h->err = "hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed";
+            return BRKGA_EDEVICE;
+        }
+    if (l1.n_wg)
+        for (const void *kern : {(const void *)decode_kernel<false, false, 3>, (const void *)decode_kernel<false, true, 3>})
+            if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1.smem) != hipSuccess) {
+                h->err = "hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed";
+                return BRKGA_EDEVICE;
+            }
+    return BRKGA_OK;
+}
 
 extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkga_handle **out) {
     if (!out) return fail(nullptr, BRKGA_EINVAL, "out is NULL");
@@ -1990,21 +2081,6 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
     w.hrot = hrot;
     size_t qwords = 1;
     for (int r = 0; r < R; r++) qwords = std::max(qwords, (size_t)p->rot_h[r] * (2 * nc4[r] + 2));
-    // fixed arrays for a vacancy cache of vc bins, offsets into w
-    auto layout = [&](int vc, int wpc) -> size_t {
-        w.vc_bins = vc;
-        // (+64 rows: the vacancy filter reads up to 63 rows past a vacancy vector)
-        size_t f = (size_t)p->n_parts * (8 + 4 + 4) + (size_t)MAX_ROT * hrot * 4 +
-                   (size_t)MAX_ROT * hmax * 2 + (size_t)hmax * 2 + (size_t)vc * hmax * 2 + 64 * 2;
-        f = (f + 15) & ~(size_t)15;
-        w.q_off = (int)f;
-        f += (qwords * 4 + 15) & ~(size_t)15;
-        w.zw_off = (int)f;
-        f += (size_t)(2 * 4 * zw_wave(wpc) + 4) * sizeof(unsigned long long);   // + the screen's tile masks
-        w.bins_off = (int)f;
-        f += ((size_t)p->n_parts * (3 * sizeof(double) + sizeof(int)) + 15) & ~(size_t)15;
-        return f;
-    };
     // band sizes in bytes: the smallest that works (one row group of one tile,
     // and one grid row for the stamp) and one holding a 32-row group of every
     // rotation with at least a 4-tile window (windows shrink at run time when
@@ -2019,73 +2095,16 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
             need_4 = std::max(need_4, ((size_t)(rh + GROUP_ROWS - 1) * band_stride(std::min(Tt, 4), 2 * nc4[r]) +
                                        BAND_SLACK) * 4);
         }
-    size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (size_t)prop.maxSharedMemoryPerMultiProcessor : 65536;
-    // Two LDS layouts.  lay[0] (the WPC = 2 kernels, the full-shape decode and
-    // the greedy): two workgroups per CU when a 32-row group of every
-    // rotation fits a 4-tile band beside the fixed arrays, else one, with a
-    // vacancy cache of up to 16 bins.  lay[1] (the WPC = 3 kernels of the
-    // pruned decode): three per CU, the vacancy cache giving way first (down
-    // to 2 bins: later bins read their free runs from HBM); absent when that
-    // does not fit.  options.wgs_per_cu 1 or 2 forces lay[0] alone, 3 forces
-    // lay[1] for every pruned decode.
     if (opt.wgs_per_cu < 0 || opt.wgs_per_cu > 3) {
         h->err = "wgs_per_cu must be 0 (automatic), 1, 2 or 3";
         return bail(BRKGA_EINVAL);
     }
-    const int vc_max = std::min(mb, 16);
-    auto make = [&](int wpc, int vcb, size_t need, LdsLayout &L) -> bool {
-        const size_t fixed = layout(vcb, wpc);
-        const size_t per_wg = lds_cu / wpc;
-        if (per_wg < fixed + need + LDS_RESERVE) return false;
-        L.wpc = wpc;
-        L.n_wg = std::max(1, h->n_cu * wpc);
-        L.vc_bins = w.vc_bins; L.q_off = w.q_off; L.zw_off = w.zw_off; L.bins_off = w.bins_off;
-        L.band_off = (int)fixed;
-        L.band_words = (int)(((per_wg - fixed - LDS_RESERVE) & ~(size_t)15) / 4);
-        L.smem = fixed + (size_t)L.band_words * 4;
-        return true;
-    };
-    const int force = opt.wgs_per_cu;
-    const int w0 = force == 1 || force == 2 ? force : (make(2, vc_max, need_4, h->lay[0]) ? 2 : 1);
-    if (!make(w0, vc_max, need_min, h->lay[0])) {
-        char b2[200];
-        const size_t fixed = layout(vc_max, w0), per_wg = lds_cu / w0;
-        snprintf(b2, sizeof b2, "a part rotation needs %zu bytes of LDS band, %zu available per workgroup",
-                 need_min, per_wg > fixed + LDS_RESERVE ? per_wg - fixed - LDS_RESERVE : (size_t)0);
-        h->err = b2;
-        return bail(BRKGA_EINVAL);
-    }
-    h->lay[1] = LdsLayout{};
-    if (force == 0 || force == 3) {
-        for (int v = std::min(vc_max, VC3_MAX); v >= std::min(vc_max, 2) && !h->lay[1].n_wg; v--) make(3, v, need_4, h->lay[1]);
-        if (force == 3 && !h->lay[1].n_wg)
-            for (int v = vc_max; v >= std::min(vc_max, 2) && !h->lay[1].n_wg; v--) make(3, v, need_min, h->lay[1]);
-        if (force == 3 && !h->lay[1].n_wg) {
-            h->err = "three workgroups per CU do not fit the LDS for this problem";
-            return bail(BRKGA_EINVAL);
-        }
-    }
-    h->force3 = force == 3;
-    const LdsLayout &L0 = h->lay[0];
-    w.vc_bins = L0.vc_bins; w.q_off = L0.q_off; w.zw_off = L0.zw_off; w.bins_off = L0.bins_off;
-    w.band_off = L0.band_off; w.band_words = L0.band_words;
-    h->smem = L0.smem;
-    h->n_wg = std::max(L0.n_wg, h->lay[1].n_wg);   // workspace slots for the larger grid
+    LayoutInputs &li = h->li;
+    li.n_parts = p->n_parts; li.hrot = hrot; li.hmax = hmax; li.force = opt.wgs_per_cu;
+    li.qwords = qwords; li.need_min = need_min; li.need_4 = need_4;
+    li.lds_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (size_t)prop.maxSharedMemoryPerMultiProcessor : 65536;
+    if ((rc = compute_layouts(h, mb))) return bail(rc);
     if ((rc = device_workspace(h, mb))) return bail(rc);
-    for (const void *kern : {(const void *)decode_kernel<false, false, 2>, (const void *)decode_kernel<true, false, 2>,
-                             (const void *)decode_kernel<false, true, 2>, (const void *)decode_kernel<true, true, 2>,
-                             (const void *)greedy_test_kernel<false>, (const void *)greedy_test_kernel<true>,
-                             (const void *)greedy_insert_kernel})
-        if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L0.smem) != hipSuccess) {
-            h->err = "hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed";
-            return bail(BRKGA_EDEVICE);
-        }
-    if (h->lay[1].n_wg)
-        for (const void *kern : {(const void *)decode_kernel<false, false, 3>, (const void *)decode_kernel<false, true, 3>})
-            if (hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)h->lay[1].smem) != hipSuccess) {
-                h->err = "hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed";
-                return bail(BRKGA_EDEVICE);
-            }
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) { h->err = "hipEventCreate failed"; return bail(BRKGA_EDEVICE); }
     if ((rc = scratch_address(h, h->device, &h->scratch))) return bail(rc);
     *out = h;
@@ -2342,6 +2361,7 @@ extern "C" int brkga_evaluate(brkga_handle *h, const void *keys, int key_dtype, 
         const int nb = std::min(h->prob.n_parts, h->dw.max_bins * 2);
         if (nb == h->dw.max_bins) break;
         HIPCHK(h, hipStreamSynchronize(s));
+        if ((rc = compute_layouts(h, nb))) return rc;   // the bin bookkeeping in LDS grows too
         if ((rc = device_workspace(h, nb))) return rc;
         h->growths++;
     }
