@@ -761,7 +761,7 @@ template <bool DENSE, bool F4, int WPC>
 __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid, int gpw, int W, int h, int w, int nc,
                              int cw, int qs, const unsigned *qbits_g, const short *cand, int ncand, int r0,
                              lds_u32 *band, int band_words, lds_u32 *s_q, lds_u64 *s_zw,
-                             unsigned long long &groups, unsigned long long &mfma, unsigned long long *ph,
+                             unsigned long long &groups, unsigned long long &mfma, lds_u64 *ph,
                              int tpcap, int wcap, int screen) {
     const int tid = threadIdx.x, lane = tid & 63;
     // mask bits of the rotation -> LDS (read by every wave for every group);
@@ -1253,7 +1253,7 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
                                               const short *s_cand, const int *s_ncand, int hmax, lds_u32 *band,
                                               lds_u32 *s_q, lds_u64 *s_zw, unsigned long long &c_tests,
                                               unsigned long long &c_algo, unsigned long long &c_groups,
-                                              unsigned long long &c_mfma, unsigned long long *ph, int &by, int &bx) {
+                                              unsigned long long &c_mfma, lds_u64 *ph, int &by, int &bx) {
     int best = -1, bl = 0, brow = 0;
     for (int k = 0; k < nr; k++) {
         const int r = k == 0 ? rf : (k <= rf ? k - 1 : k);
@@ -1384,11 +1384,17 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
 
     unsigned long long c_ctx = 0, c_place = 0, c_bins = 0, c_tests = 0, c_groups = 0, c_mfma = 0,
                        c_algo = 0, c_over = 0;
-    unsigned long long ph[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ctx_max = 0, ctx_sum = 0;
+    // Time attribution, thread 0 only, in LDS so that it takes no registers
+    // for the kernel's life: [0..7] phase totals, [8] longest context, [9]
+    // context sum, [10] this context's start, [11] this phase's start.
+    __shared__ unsigned long long s_tm[12];
+    lds_u64 *ph = (lds_u64 *)s_tm;
+    if (tid == 0)
+        for (int k = 0; k < 12; k++) ph[k] = 0;
     const long long n_ctx = L.n_chrom * M;
 
     for (bool first = true;; first = false) {
-        unsigned long long tc0 = tid == 0 ? tick() : 0, tp = tc0;
+        if (tid == 0) { ph[10] = tick(); ph[11] = ph[10]; }
         // The first G = gridDim.x positions (G <= n_ctx) go one to each
         // workgroup, later ones come from the queue.  In a full launch the
         // workgroups b, b + G/k, b + 2G/k, ... (k per CU) share a compute unit,
@@ -1435,7 +1441,7 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
             if (!P.fits[(long long)mach * P.n_types + P.slot_type[j]]) s_infeasible = 1;
         }
         __syncthreads();
-        if (tid == 0) { const unsigned long long t = tick(); ph[0] += t - tp; tp = t; }
+        if (tid == 0) { const unsigned long long t = tick(); ph[0] += t - ph[11]; ph[11] = t; }
         const int cnt = uni(s_cnt);
         const int log_off = uni(s_off);
         int *logp = L.log ? L.log + (chrom * n + log_off) * 6 : nullptr;
@@ -1465,7 +1471,7 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
             for (int b = 0; b < nb && !placed; b++) {
                 if (b_area[b] + parea > plate_area) continue;   // placement.py:57-59
                 const unsigned *grid = grids + (long long)b * hmax * Wk.pw;
-                if (tid == 0) tp = tick();
+                if (tid == 0) ph[11] = tick();
                 const short *vrow = s_vcache + b * hmax;
                 if (b >= Wk.vc_bins) {   // vacancy of bins beyond the LDS cache comes from HBM
                     const int *vac = vacs + (long long)b * hmax;
@@ -1476,7 +1482,7 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
                 // ---- vacancy filter, per candidate top row (wave r: rotation r) ----
                 vacancy_filter<DENSE>(P, rb, nr, H, W, b_minrow[b], vrow, s_den, s_cand, hmax, s_ncand);
                 __syncthreads();
-                if (tid == 0) ph[1] += tick() - tp;
+                if (tid == 0) ph[1] += tick() - ph[11];
                 // ---- collision tests, one rotation at a time (whole workgroup) ----
                 int bx = 0, by = 0;
                 const int best = best_placement<DENSE, F4, WPC>(P, Wk, mach, rb, nr, P.best_rot[ty], H, W, H - b_minrow[b], grid, s_cand,
@@ -1488,7 +1494,7 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
                 }
                 __syncthreads();
             }
-            if (tid == 0) tp = tick();
+            if (tid == 0) ph[11] = tick();
             if (!placed) {
                 if (nb >= Wk.max_bins) { overflow = true; break; }
                 pb = nb; prot = P.best_rot[ty];
@@ -1508,7 +1514,7 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
                 }
                 c_bins++;
                 __syncthreads();
-                if (tid == 0) { const unsigned long long t = tick(); ph[6] += t - tp; tp = t; }
+                if (tid == 0) { const unsigned long long t = tick(); ph[6] += t - ph[11]; ph[11] = t; }
             }
             // ---- stamp the mask (binClassNew.py:155-160, cuda_batch_update.py:90-126) ----
             {
@@ -1531,7 +1537,7 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
                 }
                 c_place++;
                 __syncthreads();
-                if (tid == 0) ph[5] += tick() - tp;
+                if (tid == 0) ph[5] += tick() - ph[11];
             }
         }
         if (tid == 0) {
@@ -1548,10 +1554,10 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
             atomicAdd(&Wk.counters[ST_MCTX + mach], 1ull);
             atomicAdd(&Wk.counters[ST_MBINS + mach], (unsigned long long)nbf);
             atomicMax(&Wk.counters[ST_MPEAK + mach], (unsigned long long)nbf);
-            const unsigned long long t = tick();
-            ph[7] += t - tp;
-            ctx_sum += t - tc0;
-            ctx_max = t - tc0 > ctx_max ? t - tc0 : ctx_max;
+            const unsigned long long t = tick(), tc = t - ph[10];
+            ph[7] += t - ph[11];
+            ph[9] += tc;
+            ph[8] = tc > ph[8] ? tc : ph[8];
         }
         __syncthreads();
     }
@@ -1566,9 +1572,9 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
         atomicAdd(&Wk.counters[ST_PLACE], c_place);
         atomicAdd(&Wk.counters[ST_BINS], c_bins);
         atomicAdd(&Wk.counters[ST_OVERFLOW], c_over);
-        for (int k = 0; k < 8; k++) atomicAdd(&Wk.counters[ST_PH0 + k], ph[k]);
-        atomicMax(&Wk.counters[ST_CTXMAX], ctx_max);
-        atomicAdd(&Wk.counters[ST_CTXSUM], ctx_sum);
+        for (int k = 0; k < 8; k++) atomicAdd(&Wk.counters[ST_PH0 + k], (unsigned long long)ph[k]);
+        atomicMax(&Wk.counters[ST_CTXMAX], (unsigned long long)ph[8]);
+        atomicAdd(&Wk.counters[ST_CTXSUM], (unsigned long long)ph[9]);
     }
 }
 
@@ -1612,7 +1618,9 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P
     __syncthreads();
     vacancy_filter<false>(P, rb, nr, H, W, H - enc, s_vac, s_den, s_cand, hmax, s_ncand);
     __syncthreads();
-    unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0, ph[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    __shared__ unsigned long long s_tm[8];   // phase times (unused here; the decode's timers)
+    lds_u64 *ph = (lds_u64 *)s_tm;
     int bx = 0, by = 0;
     const int best = best_placement<false, F4, 2>(P, Wk, m, rb, nr, P.best_rot[ty], H, W, enc, grid, s_cand, s_ncand, hmax, band, s_q,
                                            s_zw, c0, c1, c2, c3, ph, by, bx);
