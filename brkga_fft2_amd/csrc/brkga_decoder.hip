@@ -70,6 +70,7 @@ typedef LDSP unsigned lds_u32;
 typedef LDSP const v4i lds_cv4i;
 typedef LDSP v4i lds_v4i;
 typedef LDSP unsigned long long lds_u64;
+typedef LDSP const unsigned long long lds_cu64;
 typedef LDSP const short lds_cs16;
 typedef LDSP const int lds_ci32;
 
@@ -1003,142 +1004,116 @@ __global__ __launch_bounds__(ORDER_BUCKETS) void order_scatter_kernel(const int 
     }
 }
 
-// Vacancy filter of one rotation over top rows [y0, y0 + 64 NK), run by one
-// wave: bit l of word k says top row y = y0 + 64k + l passes, i.e. every grid
-// row y+a (a < h) has a free run >= the mask row's longest 1-run
-// (numba_utils.py:131-157 applies the same test to decide whether the bin is
-// tried).  den holds (density << 16) | a for the rotation's mask rows a, the
-// densest first, so that a bin with no room fails within the first steps (the
-// result is an AND over the rows, whatever their order).  For mask row a,
-// lane l tests grid row y0 + 64k + a + l, so the ballot is already aligned
-// with the candidate word.  Four mask rows per step
-// keep 4 x NK reads in flight; the NK <= 8 words are all indexed by constants
-// so that they stay in scalar registers and the loop exit stays uniform.
-// ny: valid top rows from y0 on (H - h + 1 - y0).  Lanes whose top row is
-// past H - h read rows up to H + 62 (inside the LDS allocation: the layout
-// pads the vacancy arrays) and are cleared by the initial mask.  The passing
+// Free runs of grid rows packed four to a word: vq[y] holds the free runs of
+// rows y, y+1, y+2, y+3 (16 bits each, row y in the low bits; rows from H on
+// count as 0) for y < H.  get(y) returns row y's free run.  Whole workgroup;
+// the caller synchronises before the vacancy filter reads vq.
+template <typename Get>
+__device__ __forceinline__ void vacancy_quads(lds_u64 *vq, int H, Get get) {
+    for (int y = threadIdx.x; y < H; y += WG_THREADS) {
+        unsigned long long q = 0;
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+            if (y + s < H) q |= (unsigned long long)(unsigned short)get(y + s) << (16 * s);
+        vq[y] = q;
+    }
+}
+
+// Vacancy filter of one rotation over top rows [y0, y0 + 256 NQ), run by one
+// wave.  A top row y passes when every grid row y+a (a < h) has a free run >=
+// the mask row's longest 1-run (numba_utils.py:131-157 applies the same test
+// to decide whether the bin is tried).  Lane l of quad word k holds the four
+// top rows y0 + 256k + 4l + s (s < 4): for mask row a (density d) its one
+// 8-byte read vq[y0 + 256k + 4l + a] gives the free runs v_s of the four grid
+// rows those top rows meet, and two packed 16-bit subtractions and maxima
+// keep acc_s = max over the rows so far of d - v_s; top row s passes when
+// acc_s <= 0 (free runs and densities are below 2^13, so no field
+// overflows).  den holds (density << 16) | a for the rotation's mask rows a,
+// the densest first, so that a bin with no room fails within the first steps
+// (the result is a maximum over the rows, whatever their order); the
+// densities of rows b .. b+63 are held one per lane and a step's thresholds
+// come from registers (readlane).  VAC_STEP mask rows per step keep VAC_STEP
+// x NQ reads in flight, and after each step the wave stops once no top row
+// is left.  ny: valid top rows from y0 on (H - h + 1 - y0); the fields of
+// top rows past them start at 1 (failed), and a lane whose first top row is
+// past them reads from y0 instead, so every read stays below H.  The passing
 // top rows go to cand in descending order; returns their count.
 #ifndef VAC_STEP
-#define VAC_STEP 4   // mask rows per step of the vacancy filter (their reads are in flight together)
+#define VAC_STEP 8   // mask rows per step of the vacancy filter (their reads are in flight together)
 #endif
-// Mask rows [a, aend) (densest first) against the K candidate words widx[0..K)
-// (word k: top rows y0 + 64k .. + 63), cw[j] the passing bits of word widx[j].
-// Returns true once no candidate is left.  The densities of rows b .. b+63
-// (b a multiple of 64) are held one per lane, so a step's thresholds come from
-// registers (readlane) instead of a round trip through LDS.
-template <int K>
-__device__ __forceinline__ bool vacancy_rows(lds_cs16 *vl, lds_ci32 *den, int h, int a, int aend,
-                                             const int (&widx)[K], unsigned long long (&cw)[K]) {
+#define VAC_QUADS 2  // quad words (256 top rows each) per pass of the vacancy filter
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+template <int NQ>
+__device__ __forceinline__ int vacancy_rotation(lds_cu64 *vq, lds_ci32 *den, int h, int ny, short *cand,
+                                                int y0) {
     const int lane = threadIdx.x & 63;
-    for (int a0 = a & ~63; a0 < aend; a0 += 64) {
+    s16x2 acc[NQ][2];
+    int base[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+        const int t = (k << 8) + (lane << 2);   // the lane's first top row, from y0
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            acc[k][j] = (s16x2){(short)(t + 2 * j < ny ? -32768 : 1), (short)(t + 2 * j + 1 < ny ? -32768 : 1)};
+        base[k] = y0 + (t < ny ? t : 0);
+    }
+    bool alive = true;
+    for (int a0 = 0; a0 < h && alive; a0 += 64) {
         const int dl = den[min(a0 + lane, h - 1)];
-        const int a1 = min(aend, a0 + 64);
-        for (int b = max(a, a0); b < a1; b += VAC_STEP) {
-            int aj[VAC_STEP], d[VAC_STEP];
+        const int a1 = min(h, a0 + 64);
+        for (int b = a0; b < a1; b += VAC_STEP) {
+            s16x2 d[VAC_STEP];
+            int aj[VAC_STEP];
 #pragma unroll
             for (int j = 0; j < VAC_STEP; j++) {
-                const int e = __builtin_amdgcn_readlane(dl, min(b + j, aend - 1) - a0);
-                d[j] = e >> 16;
+                const int e = __builtin_amdgcn_readlane(dl, min(b + j, a1 - 1) - a0);
+                const short dv = (short)((unsigned)e >> 16);
+                d[j] = (s16x2){dv, dv};
                 aj[j] = e & 0xffff;
             }
 #pragma unroll
             for (int j = 0; j < VAC_STEP; j++)
 #pragma unroll
-                for (int k = 0; k < K; k++) cw[k] &= __ballot(vl[aj[j] + (widx[k] << 6)] >= d[j]);
-            unsigned long long any = 0;
+                for (int k = 0; k < NQ; k++) {
+                    const unsigned long long q = vq[base[k] + aj[j]];
+                    const s16x2 lo = __builtin_bit_cast(s16x2, (unsigned)q);
+                    const s16x2 hi = __builtin_bit_cast(s16x2, (unsigned)(q >> 32));
+                    acc[k][0] = __builtin_elementwise_max(acc[k][0], d[j] - lo);
+                    acc[k][1] = __builtin_elementwise_max(acc[k][1], d[j] - hi);
+                }
+            s16x2 m = __builtin_elementwise_min(acc[0][0], acc[0][1]);
 #pragma unroll
-            for (int k = 0; k < K; k++) any |= cw[k];
-            if (!any) return true;
+            for (int k = 1; k < NQ; k++)
+                m = __builtin_elementwise_min(m, __builtin_elementwise_min(acc[k][0], acc[k][1]));
+            if (!__ballot(m.x <= 0 || m.y <= 0)) { alive = false; break; }
         }
     }
-    return false;
-}
-
-// The rows after the head over the first KK entries of a compacted word list.
-#define VAC_HEAD 8   // densest rows tested over every candidate word before the list is compacted
-#ifndef VAC_COMPACT
-#define VAC_COMPACT 5   // candidate words (of 64 top rows) from which the list is compacted
-#endif
-template <int NK, int KK>
-__device__ __forceinline__ void vacancy_rows_sub(lds_cs16 *vl, lds_ci32 *den, int h, const int (&widx)[NK],
-                                                 unsigned long long (&cc)[NK]) {
-    constexpr int KE = KK < NK ? KK : NK;   // (KK >= NK is never called; it only keeps the arrays in bounds)
-    int wi[KE];
-    unsigned long long c[KE];
-#pragma unroll
-    for (int j = 0; j < KE; j++) { wi[j] = widx[j]; c[j] = cc[j]; }
-    vacancy_rows<KE>(vl, den, h, VAC_HEAD, h, wi, c);
-#pragma unroll
-    for (int j = 0; j < KE; j++) cc[j] = c[j];
-}
-
-template <int NK>
-__device__ __forceinline__ int vacancy_rotation(lds_cs16 *vrow, lds_ci32 *den, int h, int ny, short *cand,
-                                                int y0) {
-    const int lane = threadIdx.x & 63;
-    unsigned long long cw[NK];
-#pragma unroll
-    for (int k = 0; k < NK; k++) {
-        const int left = ny - (k << 6);   // >= 1 for k < NK
-        cw[k] = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
-    }
-    lds_cs16 *vl = vrow + y0 + lane;
-    // The densest VAC_HEAD rows over every word first: they end most
-    // candidates.  The words with candidates left are then listed (padded to a
-    // power of two with a dead word, whose reads change nothing) and the other
-    // rows read only those.
-    int widx[NK];
-#pragma unroll
-    for (int k = 0; k < NK; k++) widx[k] = k;
-    if (NK < VAC_COMPACT) {   // few words: no list
-        vacancy_rows<NK>(vl, den, h, 0, h, widx, cw);
-    } else if (!vacancy_rows<NK>(vl, den, h, 0, min(h, VAC_HEAD), widx, cw) && h > VAC_HEAD) {
-        int live = 0, dead = -1;
-#pragma unroll
-        for (int k = 0; k < NK; k++) {
-            if (cw[k]) widx[live++] = k;
-            else if (dead < 0) dead = k;
-        }
-        const int kk = live <= 1 ? 1 : (live <= 2 ? 2 : (live <= 4 ? 4 : NK));
-        if (kk < NK) {
-#pragma unroll
-            for (int j = 0; j < NK; j++)
-                if (j >= live) widx[j] = dead;   // dead >= 0: live < NK here
-            unsigned long long cc[NK];
-#pragma unroll
-            for (int j = 0; j < NK; j++) {
-                cc[j] = 0;
-#pragma unroll
-                for (int k = 0; k < NK; k++) if (widx[j] == k) cc[j] = cw[k];
-            }
-            switch (kk) {
-                case 1: vacancy_rows_sub<NK, 1>(vl, den, h, widx, cc); break;
-                case 2: vacancy_rows_sub<NK, 2>(vl, den, h, widx, cc); break;
-                default: vacancy_rows_sub<NK, 4>(vl, den, h, widx, cc); break;
-            }
-#pragma unroll
-            for (int k = 0; k < NK; k++) cw[k] = 0;
-#pragma unroll
-            for (int j = 0; j < NK; j++)
-#pragma unroll
-                for (int k = 0; k < NK; k++) if (j < live && widx[j] == k) cw[k] = cc[j];
-        } else {
-#pragma unroll
-            for (int k = 0; k < NK; k++) widx[k] = k;
-            vacancy_rows<NK>(vl, den, h, VAC_HEAD, h, widx, cw);
-        }
-    }
-    // compaction in scalar form: bit-reversed, word k gives lane l the row
-    // y = 64k + 63 - l (inverse ballot) and its rank among the passing rows
-    // above it (mbcnt)
+    if (!alive) return 0;
+    // the list, descending: lane l of word k writes its passing rows after
+    // those of the lanes above it (popcount - mbcnt - its own bit, per s)
     int nc_ = 0;
 #pragma unroll
-    for (int k = NK - 1; k >= 0; k--) {   // descending y
-        if (cw[k]) {
-            const unsigned long long rv = __builtin_bitreverse64(cw[k]);
-            const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(rv >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)rv, 0));
-            if (__builtin_amdgcn_inverse_ballot_w64(rv)) cand[nc_ + pos] = (short)(y0 + (k << 6) + 63 - lane);
-            nc_ += __popcll(cw[k]);
+    for (int k = NQ - 1; k >= 0; k--) {
+        unsigned long long c[4];
+        c[0] = __ballot(acc[k][0].x <= 0);
+        c[1] = __ballot(acc[k][0].y <= 0);
+        c[2] = __ballot(acc[k][1].x <= 0);
+        c[3] = __ballot(acc[k][1].y <= 0);
+        int tot = 0, above = 0;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int pc = __popcll(c[s]);
+            const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(c[s] >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)c[s], 0));
+            above += pc - below - (int)((c[s] >> lane) & 1ull);
+            tot += pc;
+        }
+        if (tot) {
+            int p = nc_ + above;
+            const int yb = y0 + (k << 8) + (lane << 2);
+#pragma unroll
+            for (int s = 3; s >= 0; s--)
+                if ((c[s] >> lane) & 1ull) cand[p++] = (short)(yb + s);
+            nc_ += tot;
         }
     }
     return nc_;
@@ -1170,7 +1145,7 @@ __device__ __forceinline__ void load_densities(const DevProblem &P, int rb, int 
 // list (they are the highest, so the list stays descending).
 template <bool DENSE>
 __device__ __forceinline__ void vacancy_filter(const DevProblem &P, int rb, int nr, int H, int W, int minrow,
-                                               const short *vrow, const int *s_den, short *s_cand, int hmax,
+                                               lds_cu64 *vq, const int *s_den, short *s_cand, int hmax,
                                                int *s_ncand) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (wave < nr) {
@@ -1184,24 +1159,15 @@ __device__ __forceinline__ void vacancy_filter(const DevProblem &P, int rb, int 
         if (h <= H && w <= W) {
             const int ny = H - h + 1;
             const int yf = DENSE ? 0 : min(ny, max(0, uni(minrow) - h + 1));   // top rows below yf pass
-            const int nk = (ny - yf + 63) >> 6;
-            lds_cs16 *vr = (lds_cs16 *)vrow;
+            const int nq = (ny - yf + 255) >> 8;
             lds_ci32 *dn = (lds_ci32 *)den;
-            // chunks of at most 8 words (512 top rows), the highest chunk first
-            // so that the list stays descending; one chunk when H - h < 512
-            for (int c = (nk - 1) >> 3; c >= 0; c--) {
-                const int y0 = yf + (c << 9), nkc = min(8, nk - (c << 3));
+            // passes of at most VAC_QUADS quad words (512 top rows), the highest
+            // first so that the list stays descending; one pass when H - h < 512
+            for (int c = (nq - 1) / VAC_QUADS; c >= 0; c--) {
+                const int y0 = yf + (c << 8) * VAC_QUADS, nqc = min(VAC_QUADS, nq - c * VAC_QUADS);
                 short *cc = cand + nc_;
-                switch (nkc) {
-                    case 1: nc_ += vacancy_rotation<1>(vr, dn, h, ny - y0, cc, y0); break;
-                    case 2: nc_ += vacancy_rotation<2>(vr, dn, h, ny - y0, cc, y0); break;
-                    case 3: nc_ += vacancy_rotation<3>(vr, dn, h, ny - y0, cc, y0); break;
-                    case 4: nc_ += vacancy_rotation<4>(vr, dn, h, ny - y0, cc, y0); break;
-                    case 5: nc_ += vacancy_rotation<5>(vr, dn, h, ny - y0, cc, y0); break;
-                    case 6: nc_ += vacancy_rotation<6>(vr, dn, h, ny - y0, cc, y0); break;
-                    case 7: nc_ += vacancy_rotation<7>(vr, dn, h, ny - y0, cc, y0); break;
-                    default: nc_ += vacancy_rotation<8>(vr, dn, h, ny - y0, cc, y0); break;
-                }
+                if (nqc == 1) nc_ += vacancy_rotation<1>(vq, dn, h, ny - y0, cc, y0);
+                else nc_ += vacancy_rotation<VAC_QUADS>(vq, dn, h, ny - y0, cc, y0);
             }
             for (int k = lane; k < yf; k += 64) cand[nc_ + k] = (short)(yf - 1 - k);
             nc_ += yf;
@@ -1361,9 +1327,9 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
     int *s_mach = (int *)(s_sv + n);                        // [n]
     int *s_ty = s_mach + n;                                 // [n] part type at sequence position i
     int *s_den = s_ty + n;                                  // [<= MAX_ROT * hrot] one type's densities
-    short *s_cand = (short *)(s_den + MAX_ROT * Wk.hrot);   // [MAX_ROT][hmax]
-    short *s_vac = s_cand + MAX_ROT * hmax;                 // [hmax] vacancy of an uncached bin
-    short *s_vcache = s_vac + hmax;                         // [vc_bins][hmax] vacancy of bins < vc_bins
+    lds_u64 *s_vq = (lds_u64 *)(s_den + MAX_ROT * Wk.hrot); // [hmax] the tested bin's free runs, 4 rows a word
+    short *s_cand = (short *)(s_vq + hmax);                 // [MAX_ROT][hmax]
+    short *s_vcache = s_cand + MAX_ROT * hmax;              // [vc_bins][hmax] vacancy of bins < vc_bins
     lds_u32 *s_q = (lds_u32 *)(smem + Wk.q_off);            // [hrot * (max nc + 1)]
     lds_u32 *band = (lds_u32 *)(smem + Wk.band_off);        // [band_words]
     lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);          // [2][4][zw_wave] zero masks, 4 screen masks
@@ -1472,15 +1438,16 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
                 if (b_area[b] + parea > plate_area) continue;   // placement.py:57-59
                 const unsigned *grid = grids + (long long)b * hmax * Wk.pw;
                 if (tid == 0) ph[11] = tick();
-                const short *vrow = s_vcache + b * hmax;
                 if (b >= Wk.vc_bins) {   // vacancy of bins beyond the LDS cache comes from HBM
                     const int *vac = vacs + (long long)b * hmax;
-                    for (int y = tid; y < H; y += WG_THREADS) s_vac[y] = (short)vac[y];
-                    vrow = s_vac;
-                    __syncthreads();
+                    vacancy_quads(s_vq, H, [&](int y) { return vac[y]; });
+                } else {
+                    lds_cs16 *vr = (lds_cs16 *)(s_vcache + b * hmax);
+                    vacancy_quads(s_vq, H, [&](int y) { return vr[y]; });
                 }
+                __syncthreads();
                 // ---- vacancy filter, per candidate top row (wave r: rotation r) ----
-                vacancy_filter<DENSE>(P, rb, nr, H, W, b_minrow[b], vrow, s_den, s_cand, hmax, s_ncand);
+                vacancy_filter<DENSE>(P, rb, nr, H, W, b_minrow[b], s_vq, s_den, s_cand, hmax, s_ncand);
                 __syncthreads();
                 if (tid == 0) ph[1] += tick() - ph[11];
                 // ---- collision tests, one rotation at a time (whole workgroup) ----
@@ -1601,8 +1568,8 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = P.n_parts, hmax = Wk.hmax, tid = threadIdx.x;
     int *s_den = (int *)((double *)smem + n) + 2 * n;       // same layout as decode_kernel
-    short *s_cand = (short *)(s_den + MAX_ROT * Wk.hrot);
-    short *s_vac = s_cand + MAX_ROT * hmax;
+    lds_u64 *s_vq = (lds_u64 *)(s_den + MAX_ROT * Wk.hrot);
+    short *s_cand = (short *)(s_vq + hmax);
     lds_u32 *s_q = (lds_u32 *)(smem + Wk.q_off);
     lds_u32 *band = (lds_u32 *)(smem + Wk.band_off);
     lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);
@@ -1614,9 +1581,9 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P
     const unsigned *grid = A.grids + slot * hmax * Wk.pw;
     const int *vac = A.vac + slot * hmax;
     load_densities(P, rb, nr, s_den);
-    for (int y = tid; y < H; y += WG_THREADS) s_vac[y] = (short)vac[y];
+    vacancy_quads(s_vq, H, [&](int y) { return vac[y]; });
     __syncthreads();
-    vacancy_filter<false>(P, rb, nr, H, W, H - enc, s_vac, s_den, s_cand, hmax, s_ncand);
+    vacancy_filter<false>(P, rb, nr, H, W, H - enc, s_vq, s_den, s_cand, hmax, s_ncand);
     __syncthreads();
     unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0;
     __shared__ unsigned long long s_tm[8];   // phase times (unused here; the decode's timers)
@@ -1851,9 +1818,9 @@ static int compute_layouts(brkga_handle *h, int mb) {
     // fixed arrays for a vacancy cache of vc bins, offsets into w
     auto layout = [&](int vc, int wpc) -> size_t {
         w.vc_bins = vc;
-        // (+64 rows: the vacancy filter reads up to 63 rows past a vacancy vector)
-        size_t f = (size_t)li.n_parts * (8 + 4 + 4) + (size_t)MAX_ROT * li.hrot * 4 +
-                   (size_t)MAX_ROT * li.hmax * 2 + (size_t)li.hmax * 2 + (size_t)vc * li.hmax * 2 + 64 * 2;
+        // (the packed free runs vq, 8 bytes a row, follow the densities)
+        size_t f = (size_t)li.n_parts * (8 + 4 + 4) + (size_t)MAX_ROT * li.hrot * 4 + (size_t)li.hmax * 8 +
+                   (size_t)MAX_ROT * li.hmax * 2 + (size_t)vc * li.hmax * 2;
         f = (f + 15) & ~(size_t)15;
         w.q_off = (int)f;
         f += (li.qwords * 4 + 15) & ~(size_t)15;

@@ -196,71 +196,70 @@ def test_segments_cover_the_window_without_leaving_it():
             assert covered == set(range(tl))
 
 
-def vacancy_words(vac, den_sorted, ny, nk, head=8, compact=5):
-    """vacancy_rotation: candidate words of 64 top rows; the densest `head`
-    rows over every word, then (from `compact` words on) the words with
-    candidates left listed and padded to 1, 2 or 4 entries with a dead word
-    whose bits stay zero, and the other rows over the list only."""
-    cw = []
-    for k in range(nk):
-        left = ny - 64 * k
-        cw.append((1 << min(64, left)) - 1)
+def vacancy_quads(vac, den_sorted, ny, y0, nq, H):
+    """vacancy_rotation: lane l of quad word k holds top rows y0 + 256k + 4l + s
+    (s < 4) and reads the packed free runs vq[base + a] of rows base + a ..
+    base + a + 3 (base = y0 + 256k + 4l, or y0 for a lane past the valid rows;
+    rows from H on pack as 0); returns the masks c[k][s] and the list of
+    passing top rows in the kernel's order (lanes above first, then s
+    descending)."""
+    def vq(y):
+        assert 0 <= y < H, y                       # every read stays below H
+        return [vac[y + s] if y + s < H else 0 for s in range(4)]
 
-    def rows(a0, a1, idx, c):
-        for a in range(a0, a1):
-            d, row = den_sorted[a]
-            for j, k in enumerate(idx):
-                bits = 0
-                for lane in range(64):
-                    if vac[64 * k + lane + row] >= d:
-                        bits |= 1 << lane
-                c[j] &= bits
-            if not any(c):
-                return True
-        return False
-
-    h = len(den_sorted)
-    idx = list(range(nk))
-    if nk < compact:
-        rows(0, h, idx, cw)
-        return cw
-    if rows(0, min(h, head), idx, cw) or h <= head:
-        return cw
-    live = [k for k in range(nk) if cw[k]]
-    dead = next(k for k in range(nk) if not cw[k]) if len(live) < nk else None
-    kk = 1 if len(live) <= 1 else (2 if len(live) <= 2 else (4 if len(live) <= 4 else nk))
-    if kk < nk:
-        lst = live + [dead] * (kk - len(live))
-        c = [cw[k] for k in lst]
-        rows(head, h, lst, c)
-        out = [0] * nk
-        for j, k in enumerate(lst[:len(live)]):
-            out[k] = c[j]
-        return out
-    rows(head, h, idx, cw)
-    return cw
+    c = [[0] * 4 for _ in range(nq)]
+    base = [[0] * 64 for _ in range(nq)]
+    for k in range(nq):
+        for lane in range(64):
+            t = 256 * k + 4 * lane
+            for s in range(4):
+                if t + s < ny:
+                    c[k][s] |= 1 << lane
+            base[k][lane] = y0 + (t if t < ny else 0)
+    for d, row in den_sorted:
+        for k in range(nq):
+            for lane in range(64):
+                f = vq(base[k][lane] + row)
+                for s in range(4):
+                    if f[s] < d:
+                        c[k][s] &= ~(1 << lane)
+    out = []
+    for k in range(nq - 1, -1, -1):
+        tot = sum(bin(c[k][s]).count("1") for s in range(4))
+        slots = [None] * tot
+        for lane in range(64):
+            above = sum(bin(c[k][s] >> (lane + 1)).count("1") for s in range(4))
+            p = above
+            for s in range(3, -1, -1):
+                if (c[k][s] >> lane) & 1:
+                    slots[p] = y0 + 256 * k + 4 * lane + s
+                    p += 1
+        out += slots
+    return c, out
 
 
-def test_vacancy_word_compaction_keeps_the_candidates():
-    """The compacted vacancy filter lists exactly the top rows y whose rows
-    y..y+h-1 all have a free run at least the mask row's density."""
+def test_vacancy_quads_list_the_candidates_descending():
+    """The packed vacancy filter lists exactly the top rows y whose rows
+    y..y+h-1 all have a free run at least the mask row's density, in
+    descending order, over passes of two quad words (512 top rows)."""
     rng = random.Random(13)
-    for _ in range(300):
+    for _ in range(200):
         h = rng.randint(1, 40)
-        ny = rng.randint(1, 64 * 8)
-        nk = (ny + 63) // 64
-        H = ny + h - 1
+        H = rng.randint(h, 1200)
+        ny = H - h + 1
+        yf = rng.randint(0, ny) if rng.random() < 0.3 else 0
         W = 100
         vac = [rng.choice([W, W, rng.randint(0, W)]) for _ in range(H)]
-        # blocks of full rows so that some words die early and others live
         for _ in range(rng.randint(0, 6)):
             a = rng.randrange(H)
             for r in range(a, min(H, a + rng.randint(1, 80))):
                 vac[r] = rng.randint(0, 20)
-        vac += [0] * (64 * nk + h)     # rows past H read as occupied (masked by cw)
         den = [rng.randint(1, 60) for _ in range(h)]
         den_sorted = sorted(((den[a], a) for a in range(h)), key=lambda t: -t[0])
-        want = [y for y in range(ny) if all(vac[y + a] >= den[a] for a in range(h))]
-        cw = vacancy_words(vac, den_sorted, ny, nk)
-        got = [64 * k + i for k in range(nk) for i in range(64) if (cw[k] >> i) & 1]
+        want = [y for y in range(ny - 1, yf - 1, -1) if all(vac[y + a] >= den[a] for a in range(h))]
+        nq = (ny - yf + 255) // 256
+        got = []
+        for cpass in range((nq - 1) // 2, -1, -1):
+            y0 = yf + 512 * cpass
+            got += vacancy_quads(vac, den_sorted, ny - y0, y0, min(2, nq - 2 * cpass), H)[1]
         assert got == want
