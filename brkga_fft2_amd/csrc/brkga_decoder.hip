@@ -107,7 +107,7 @@ struct DevProblem {
     const int *rot_nc, *rot_nc4;   // Toeplitz chunks of 32 / 64 columns: ceil((w + 31) / 32), ceil((w + 31) / 64)
     const long long *mask_off, *dens_off, *qbits_off;
     const unsigned char *masks;
-    const int *dens;         // per rotation: (longest 1-run << 16) | mask row, densest rows first
+    const int *dens;         // per rotation: (longest 1-run << 16) | 8 mask row, densest rows first
     const unsigned *qbits;   // per rotation: h rows of 2 nc4 + 2 dwords, mask bits after 32 zero bits, zero padded
     const int *plate_h, *plate_w;
     const double *setup, *proc_time, *pth;
@@ -541,7 +541,7 @@ __host__ __device__ constexpr int screen_tiles(int nc, int wpc) {
 template <int NC, bool F4, int WPC>
 __device__ __attribute__((always_inline)) unsigned long long screen_window(lds_cu32 *band, int stride, const Fold &f,
                                                                            int h, int S, lds_cu32 *qbits, int qs, int tl,
-                                                                           lds_u64 *s_bits, unsigned long long &mfma) {
+                                                                           lds_u64 *s_bits, lds_u64 &mfma) {
     constexpr int TW = screen_tiles(NC, WPC);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int q = (tl + 3) >> 2;
@@ -580,7 +580,7 @@ __device__ __attribute__((always_inline)) unsigned long long screen_window(lds_c
 template <bool F4, int WPC>
 __device__ __forceinline__ unsigned long long screen_dispatch(int nc, lds_cu32 *band, int stride, const Fold &f, int h,
                                                               int S, lds_cu32 *qbits, int qs, int tl, lds_u64 *s_bits,
-                                                              unsigned long long &mfma) {
+                                                              lds_u64 &mfma) {
 #define SCASE(K) case K: return screen_window<K, F4, WPC>(band, stride, f, h, S, qbits, qs, tl, s_bits, mfma);
     switch (nc) {
         SCASE(1) SCASE(2) SCASE(3) SCASE(4) SCASE(5) SCASE(6) SCASE(7) SCASE(8) SCASE(9) SCASE(10) SCASE(11) SCASE(12)
@@ -608,7 +608,7 @@ __device__ __forceinline__ unsigned long long screen_dispatch(int nc, lds_cu32 *
 template <int NC, bool F4, int WPC>
 __device__ __attribute__((always_inline)) int window_passes(lds_cu32 *band, int stride, const Fold &f, int h,
                              lds_cu32 *qbits, int qs, unsigned long long need, int xbase, int xmax, int nrows,
-                             lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
+                             lds_u64 *s_zw, lds_u64 &mfma, int tpcap, bool early) {
     constexpr int TPM = pass_tiles(NC, WPC);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int a0 = f.skip + (((h - f.skip) * wave) >> 2), a1 = f.skip + (((h - f.skip) * (wave + 1)) >> 2);
@@ -645,7 +645,7 @@ __device__ __attribute__((always_inline)) int window_passes(lds_cu32 *band, int 
 template <bool F4, int WPC>
 __device__ int window_passes_generic(lds_cu32 *band, int stride, const Fold &f, int h, int nc,
                                      lds_cu32 *qbits, int qs, int tl, int xbase, int xmax, int nrows,
-                                     lds_u64 *s_zw, unsigned long long &mfma, bool early) {
+                                     lds_u64 *s_zw, lds_u64 &mfma, bool early) {
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int g = lane >> 5, ks = g << 2, bs = F4 ? 32 - (lane & 31) : 32 - (lane & 31) + ks;
     const int a0 = f.skip + (((h - f.skip) * wave) >> 2), a1 = f.skip + (((h - f.skip) * (wave + 1)) >> 2);
@@ -683,7 +683,7 @@ __device__ int window_passes_generic(lds_cu32 *band, int stride, const Fold &f, 
 template <bool F4, int WPC>
 __device__ __forceinline__ int window_dispatch(int nc, lds_cu32 *band, int stride, const Fold &f, int h,
                                lds_cu32 *qbits, int qs, int tl, unsigned long long need, int xbase, int xmax,
-                               int nrows, lds_u64 *s_zw, unsigned long long &mfma, int tpcap, bool early) {
+                               int nrows, lds_u64 *s_zw, lds_u64 &mfma, int tpcap, bool early) {
 #define CASE(K) case K: return window_passes<K, F4, WPC>(band, stride, f, h, qbits, qs, need, xbase, xmax, nrows, s_zw, mfma, tpcap, early);
     switch (nc) {
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
@@ -762,7 +762,7 @@ template <bool DENSE, bool F4, int WPC>
 __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid, int gpw, int W, int h, int w, int nc,
                              int cw, int qs, const unsigned *qbits_g, const short *cand, int ncand, int r0,
                              lds_u32 *band, int band_words, lds_u32 *s_q, lds_u64 *s_zw,
-                             unsigned long long &groups, unsigned long long &mfma, lds_u64 *ph,
+                             lds_u64 &groups, lds_u64 &mfma, lds_u64 *ph,
                              int tpcap, int wcap, int screen) {
     const int tid = threadIdx.x, lane = tid & 63;
     // mask bits of the rotation -> LDS (read by every wave for every group);
@@ -793,7 +793,7 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid
         // mask rows a < r0 - ytop meet empty rows for every candidate
         const int skip = DENSE ? 0 : min(h, max(0, r0 - ytop));
         int gkey = skip == h ? 0 : KEY_NONE;   // the part lies in empty rows: candidate 0 at x = 0
-        groups++;
+        if (tid == 0) groups += 1;
         for (int tw0 = 0; tw0 < T && skip < h; tw0 += TW) {
             unsigned long long t0 = tid == 0 ? tick() : 0;
             const int tl = min(TW, T - tw0);
@@ -1005,16 +1005,16 @@ __global__ __launch_bounds__(ORDER_BUCKETS) void order_scatter_kernel(const int 
 }
 
 // Free runs of grid rows packed four to a word: vq[y] holds the free runs of
-// rows y, y+1, y+2, y+3 (16 bits each, row y in the low bits; rows from H on
-// count as 0) for y < H.  get(y) returns row y's free run.  Whole workgroup;
-// the caller synchronises before the vacancy filter reads vq.
+// rows y, y+64, y+128 and y+192 (16 bits each, row y in the low bits; rows
+// from H on count as 0) for y < H.  get(y) returns row y's free run.  Whole
+// workgroup; the caller synchronises before the vacancy filter reads vq.
 template <typename Get>
 __device__ __forceinline__ void vacancy_quads(lds_u64 *vq, int H, Get get) {
     for (int y = threadIdx.x; y < H; y += WG_THREADS) {
         unsigned long long q = 0;
 #pragma unroll
         for (int s = 0; s < 4; s++)
-            if (y + s < H) q |= (unsigned long long)(unsigned short)get(y + s) << (16 * s);
+            if (y + 64 * s < H) q |= (unsigned long long)(unsigned short)get(y + 64 * s) << (16 * s);
         vq[y] = q;
     }
 }
@@ -1022,40 +1022,40 @@ __device__ __forceinline__ void vacancy_quads(lds_u64 *vq, int H, Get get) {
 // Vacancy filter of one rotation over top rows [y0, y0 + 256 NQ), run by one
 // wave.  A top row y passes when every grid row y+a (a < h) has a free run >=
 // the mask row's longest 1-run (numba_utils.py:131-157 applies the same test
-// to decide whether the bin is tried).  Lane l of quad word k holds the four
-// top rows y0 + 256k + 4l + s (s < 4): for mask row a (density d) its one
-// 8-byte read vq[y0 + 256k + 4l + a] gives the free runs v_s of the four grid
-// rows those top rows meet, and two packed 16-bit subtractions and maxima
-// keep acc_s = max over the rows so far of d - v_s; top row s passes when
-// acc_s <= 0 (free runs and densities are below 2^13, so no field
-// overflows).  den holds (density << 16) | a for the rotation's mask rows a,
-// the densest first, so that a bin with no room fails within the first steps
-// (the result is a maximum over the rows, whatever their order); the
-// densities of rows b .. b+63 are held one per lane and a step's thresholds
-// come from registers (readlane).  VAC_STEP mask rows per step keep VAC_STEP
-// x NQ reads in flight, and after each step the wave stops once no top row
-// is left.  ny: valid top rows from y0 on (H - h + 1 - y0); the fields of
-// top rows past them start at 1 (failed), and a lane whose first top row is
-// past them reads from y0 instead, so every read stays below H.  The passing
-// top rows go to cand in descending order; returns their count.
+// to decide whether the bin is tried).  The top rows form words of 64, word
+// K = 4k + s holding y0 + 64K + l in lane l: for mask row a (density d), lane
+// l's one 8-byte read vq[y0 + 256k + l + a] gives the free runs v_s of the
+// grid rows its four top rows of quad k meet (consecutive lanes read
+// consecutive words: no bank conflicts), and packed 16-bit subtractions and
+// maxima keep acc_s = max over the rows so far of d - v_s; top row s passes
+// when acc_s <= 0 (free runs and densities are at most 4096, so no field
+// overflows).  den holds (density << 16) | 8a for the rotation's mask rows a
+// (8a: the row's byte offset in vq), the densest first, so that a bin with
+// no room fails within the first steps (the result is a maximum over the
+// rows, whatever their order); the entries of rows b .. b+63 are held one per
+// lane and a step's come from registers (readlane).  VAC_STEP mask rows per
+// step keep their reads in flight together, and after each step the wave
+// stops once no top row is left.  ny: valid top rows from y0 on (H - h + 1 -
+// y0); the fields of top rows past them start at 1 (failed), and a lane
+// whose quad has none reads from y0 instead, so every read stays below H.
+// The passing top rows go to cand in descending order; returns their count.
 #ifndef VAC_STEP
 #define VAC_STEP 8   // mask rows per step of the vacancy filter (their reads are in flight together)
 #endif
-#define VAC_QUADS 2  // quad words (256 top rows each) per pass of the vacancy filter
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 template <int NQ>
 __device__ __forceinline__ int vacancy_rotation(lds_cu64 *vq, lds_ci32 *den, int h, int ny, short *cand,
                                                 int y0) {
     const int lane = threadIdx.x & 63;
     s16x2 acc[NQ][2];
-    int base[NQ];
+    LDSP const unsigned char *p[NQ];
 #pragma unroll
     for (int k = 0; k < NQ; k++) {
-        const int t = (k << 8) + (lane << 2);   // the lane's first top row, from y0
+        const int t = (k << 8) + lane;   // the lane's top row of word 4k, from y0
 #pragma unroll
         for (int j = 0; j < 2; j++)
-            acc[k][j] = (s16x2){(short)(t + 2 * j < ny ? -32768 : 1), (short)(t + 2 * j + 1 < ny ? -32768 : 1)};
-        base[k] = y0 + (t < ny ? t : 0);
+            acc[k][j] = (s16x2){(short)(t + 128 * j < ny ? -32768 : 1), (short)(t + 128 * j + 64 < ny ? -32768 : 1)};
+        p[k] = (LDSP const unsigned char *)(vq + y0 + (t < ny ? t : 0));
     }
     bool alive = true;
     for (int a0 = 0; a0 < h && alive; a0 += 64) {
@@ -1063,24 +1063,28 @@ __device__ __forceinline__ int vacancy_rotation(lds_cu64 *vq, lds_ci32 *den, int
         const int a1 = min(h, a0 + 64);
         for (int b = a0; b < a1; b += VAC_STEP) {
             s16x2 d[VAC_STEP];
-            int aj[VAC_STEP];
+            int ab[VAC_STEP];
 #pragma unroll
             for (int j = 0; j < VAC_STEP; j++) {
                 const int e = __builtin_amdgcn_readlane(dl, min(b + j, a1 - 1) - a0);
                 const short dv = (short)((unsigned)e >> 16);
                 d[j] = (s16x2){dv, dv};
-                aj[j] = e & 0xffff;
+                ab[j] = e & 0xffff;
             }
 #pragma unroll
             for (int j = 0; j < VAC_STEP; j++)
 #pragma unroll
                 for (int k = 0; k < NQ; k++) {
-                    const unsigned long long q = vq[base[k] + aj[j]];
+                    const unsigned long long q = *(lds_cu64 *)(p[k] + ab[j]);
                     const s16x2 lo = __builtin_bit_cast(s16x2, (unsigned)q);
                     const s16x2 hi = __builtin_bit_cast(s16x2, (unsigned)(q >> 32));
                     acc[k][0] = __builtin_elementwise_max(acc[k][0], d[j] - lo);
                     acc[k][1] = __builtin_elementwise_max(acc[k][1], d[j] - hi);
                 }
+            // the step's reads first, then its arithmetic: the reads are in
+            // flight together rather than each waited for before the next
+            __builtin_amdgcn_sched_group_barrier(0x0100, VAC_STEP * NQ, 0);
+            __builtin_amdgcn_sched_group_barrier(0x0002, 4 * VAC_STEP * NQ, 0);
             s16x2 m = __builtin_elementwise_min(acc[0][0], acc[0][1]);
 #pragma unroll
             for (int k = 1; k < NQ; k++)
@@ -1089,38 +1093,26 @@ __device__ __forceinline__ int vacancy_rotation(lds_cu64 *vq, lds_ci32 *den, int
         }
     }
     if (!alive) return 0;
-    // the list, descending: lane l of word k writes its passing rows after
-    // those of the lanes above it (popcount - mbcnt - its own bit, per s)
+    // the list in scalar form, descending: bit-reversed, word K gives lane l
+    // the row y0 + 64K + 63 - l (inverse ballot) and its rank among the
+    // passing rows above it (mbcnt)
     int nc_ = 0;
 #pragma unroll
-    for (int k = NQ - 1; k >= 0; k--) {
-        unsigned long long c[4];
-        c[0] = __ballot(acc[k][0].x <= 0);
-        c[1] = __ballot(acc[k][0].y <= 0);
-        c[2] = __ballot(acc[k][1].x <= 0);
-        c[3] = __ballot(acc[k][1].y <= 0);
-        int tot = 0, above = 0;
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            const int pc = __popcll(c[s]);
-            const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(c[s] >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)c[s], 0));
-            above += pc - below - (int)((c[s] >> lane) & 1ull);
-            tot += pc;
-        }
-        if (tot) {
-            int p = nc_ + above;
-            const int yb = y0 + (k << 8) + (lane << 2);
-#pragma unroll
-            for (int s = 3; s >= 0; s--)
-                if ((c[s] >> lane) & 1ull) cand[p++] = (short)(yb + s);
-            nc_ += tot;
+    for (int K = 4 * NQ - 1; K >= 0; K--) {
+        const s16x2 v = acc[K >> 2][(K >> 1) & 1];
+        const unsigned long long c = __ballot(((K & 1) ? v.y : v.x) <= 0);
+        if (c) {
+            const unsigned long long rv = __builtin_bitreverse64(c);
+            const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(rv >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)rv, 0));
+            if (__builtin_amdgcn_inverse_ballot_w64(rv)) cand[nc_ + pos] = (short)(y0 + (K << 6) + 63 - lane);
+            nc_ += __popcll(c);
         }
     }
     return nc_;
 }
 
 // Densities (longest 1-run per mask row, packed with the row as
-// (density << 16) | row, densest rows first) of rotations rb..rb+nr-1 into
+// (density << 16) | 8 row, densest rows first) of rotations rb..rb+nr-1 into
 // s_den, rotation after rotation (rotation r at the sum of the heights of
 // rotations rb..rb+r-1).  brkga_create appends the densities in rotation
 // order and a type's rotations are consecutive, so this is one contiguous
@@ -1159,15 +1151,15 @@ __device__ __forceinline__ void vacancy_filter(const DevProblem &P, int rb, int 
         if (h <= H && w <= W) {
             const int ny = H - h + 1;
             const int yf = DENSE ? 0 : min(ny, max(0, uni(minrow) - h + 1));   // top rows below yf pass
-            const int nq = (ny - yf + 255) >> 8;
             lds_ci32 *dn = (lds_ci32 *)den;
-            // passes of at most VAC_QUADS quad words (512 top rows), the highest
-            // first so that the list stays descending; one pass when H - h < 512
-            for (int c = (nq - 1) / VAC_QUADS; c >= 0; c--) {
-                const int y0 = yf + (c << 8) * VAC_QUADS, nqc = min(VAC_QUADS, nq - c * VAC_QUADS);
+            // passes of at most 512 top rows (one quad when 256 cover the
+            // rest), the highest first so that the list stays descending; one
+            // pass when H - h < 512
+            for (int c = (ny - yf - 1) >> 9; c >= 0; c--) {
+                const int y0 = yf + (c << 9);
                 short *cc = cand + nc_;
-                if (nqc == 1) nc_ += vacancy_rotation<1>(vq, dn, h, ny - y0, cc, y0);
-                else nc_ += vacancy_rotation<VAC_QUADS>(vq, dn, h, ny - y0, cc, y0);
+                if (ny - y0 <= 256) nc_ += vacancy_rotation<1>(vq, dn, h, ny - y0, cc, y0);
+                else nc_ += vacancy_rotation<2>(vq, dn, h, ny - y0, cc, y0);
             }
             for (int k = lane; k < yf; k += 64) cand[nc_ + k] = (short)(yf - 1 - k);
             nc_ += yf;
@@ -1217,9 +1209,9 @@ template <bool DENSE, bool F4, int WPC>
 __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork &Wk, int mach, int rb, int nr, int rf,
                                               int H, int W, int enc, const unsigned *grid,
                                               const short *s_cand, const int *s_ncand, int hmax, lds_u32 *band,
-                                              lds_u32 *s_q, lds_u64 *s_zw, unsigned long long &c_tests,
-                                              unsigned long long &c_algo, unsigned long long &c_groups,
-                                              unsigned long long &c_mfma, lds_u64 *ph, int &by, int &bx) {
+                                              lds_u32 *s_q, lds_u64 *s_zw, lds_u64 &c_tests,
+                                              lds_u64 &c_algo, lds_u64 &c_groups,
+                                              lds_u64 &c_mfma, lds_u64 *ph, int &by, int &bx) {
     int best = -1, bl = 0, brow = 0;
     for (int k = 0; k < nr; k++) {
         const int r = k == 0 ? rf : (k <= rf ? k - 1 : k);
@@ -1232,8 +1224,10 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
             ncand = uni(winning_prefix(cand, ncand, enc, H, h, bl, brow));
             if (ncand == 0) continue;
         }
-        c_tests++;
-        c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
+        if (threadIdx.x == 0) {
+            c_tests += 1;
+            c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
+        }
         const int ncr = P.rot_nc[rg], nc4 = P.rot_nc4[rg];
         const unsigned char *tc = P.tiles + ((long long)mach * TUNE_NC + min(ncr, TUNE_NC - 1)) * TILE_ENTRY;
         const int key = test_rotation<DENSE, F4, WPC>(grid, Wk.pw, W, h, w, F4 ? nc4 : ncr, F4 ? 2 * nc4 : ncr,
@@ -1348,8 +1342,14 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
     LDSP double *b_pth = b_pt + Wk.max_bins;
     LDSP int *b_minrow = (LDSP int *)(b_pth + Wk.max_bins);
 
-    unsigned long long c_ctx = 0, c_place = 0, c_bins = 0, c_tests = 0, c_groups = 0, c_mfma = 0,
-                       c_algo = 0, c_over = 0;
+    // Statistics, in LDS so that they take no registers for the kernel's
+    // life: the workgroup's counts (thread 0 updates them) and each wave's
+    // MFMA instructions (the wave's lanes update its own entry together).
+    __shared__ unsigned long long s_st[8 + 4];
+    lds_u64 &c_ctx = ((lds_u64 *)s_st)[0], &c_place = ((lds_u64 *)s_st)[1], &c_bins = ((lds_u64 *)s_st)[2],
+            &c_tests = ((lds_u64 *)s_st)[3], &c_groups = ((lds_u64 *)s_st)[4], &c_algo = ((lds_u64 *)s_st)[5],
+            &c_over = ((lds_u64 *)s_st)[6], &c_mfma = ((lds_u64 *)s_st)[8 + wave];
+    if (tid < 12) s_st[tid] = 0;
     // Time attribution, thread 0 only, in LDS so that it takes no registers
     // for the kernel's life: [0..7] phase totals, [8] longest context, [9]
     // context sum, [10] this context's start, [11] this phase's start.
@@ -1421,7 +1421,7 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
             __syncthreads();
             continue;
         }
-        c_ctx++;
+        if (tid == 0) c_ctx += 1;
         bool overflow = false;
         for (int i = 0; i < cnt; i++) {
             const int ty = uni(s_ty[i]);
@@ -1479,7 +1479,7 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
                     b_area[pb] = 0.0; b_pt[pb] = 0.0; b_pth[pb] = 0.0; b_minrow[pb] = H;
                     s_nb = nb + 1;
                 }
-                c_bins++;
+                if (tid == 0) c_bins += 1;
                 __syncthreads();
                 if (tid == 0) { const unsigned long long t = tick(); ph[6] += t - ph[11]; ph[11] = t; }
             }
@@ -1502,7 +1502,7 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
                         e[4] = py + h - 1; e[5] = prot;
                     }
                 }
-                c_place++;
+                if (tid == 0) c_place += 1;
                 __syncthreads();
                 if (tid == 0) ph[5] += tick() - ph[11];
             }
@@ -1511,7 +1511,7 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
             const int nbf = uni(s_nb);
             if (overflow) {
                 L.ctx_ms[ctx] = -2.0;
-                c_over++;
+                c_over += 1;
             } else {
                 double total = 0.0;   // placement.py:86-89
                 for (int b = 0; b < nbf; b++) total = total + ((b_pt[b] + b_pth[b]) + P.setup[mach]);
@@ -1585,12 +1585,12 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P
     __syncthreads();
     vacancy_filter<false>(P, rb, nr, H, W, H - enc, s_vq, s_den, s_cand, hmax, s_ncand);
     __syncthreads();
-    unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-    __shared__ unsigned long long s_tm[8];   // phase times (unused here; the decode's timers)
-    lds_u64 *ph = (lds_u64 *)s_tm;
+    // statistics and phase times (unused here; the decode's)
+    __shared__ unsigned long long s_st[4 + 4], s_tm[8];
+    lds_u64 *st = (lds_u64 *)s_st, *ph = (lds_u64 *)s_tm;
     int bx = 0, by = 0;
     const int best = best_placement<false, F4, 2>(P, Wk, m, rb, nr, P.best_rot[ty], H, W, enc, grid, s_cand, s_ncand, hmax, band, s_q,
-                                           s_zw, c0, c1, c2, c3, ph, by, bx);
+                                           s_zw, st[0], st[1], st[2], st[4 + (tid >> 6)], ph, by, bx);
     if (tid == 0) {
         int *r = A.res + 4 * blockIdx.x;
         r[0] = best >= 0;
@@ -1940,12 +1940,13 @@ extern "C" int brkga_create(const brkga_problem *p, const brkga_options *o, brkg
         const int rh = p->rot_h[r], rw = p->rot_w[r];
         const uint8_t *mk = p->masks + p->mask_off[r];
         dens_off[r] = (long long)dens.size();
-        {   // (density << 16) | row, densest rows first: the filter's early exit comes sooner
+        {   // (density << 16) | 8 row (the row's byte offset in the packed free runs), densest rows
+            // first: the filter's early exit comes sooner
             std::vector<int> dr(rh);
             for (int a = 0; a < rh; a++) {
                 int best = 0, cur = 0;
                 for (int b = 0; b < rw; b++) { cur = mk[a * rw + b] ? cur + 1 : 0; best = std::max(best, cur); }
-                dr[a] = (best << 16) | a;   // best <= rw <= 4096 and a < rh <= 4096
+                dr[a] = (best << 16) | (a << 3);   // best <= rw <= 4096 and 8a < 8 rh <= 32768 (16 bits)
             }
             std::stable_sort(dr.begin(), dr.end(), [](int x, int y) { return (x >> 16) > (y >> 16); });
             dens.insert(dens.end(), dr.begin(), dr.end());

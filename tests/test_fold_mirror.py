@@ -196,54 +196,42 @@ def test_segments_cover_the_window_without_leaving_it():
             assert covered == set(range(tl))
 
 
-def vacancy_quads(vac, den_sorted, ny, y0, nq, H):
-    """vacancy_rotation: lane l of quad word k holds top rows y0 + 256k + 4l + s
-    (s < 4) and reads the packed free runs vq[base + a] of rows base + a ..
-    base + a + 3 (base = y0 + 256k + 4l, or y0 for a lane past the valid rows;
-    rows from H on pack as 0); returns the masks c[k][s] and the list of
-    passing top rows in the kernel's order (lanes above first, then s
-    descending)."""
+def vacancy_words(vac, den_sorted, ny, y0, nq, H):
+    """vacancy_rotation: word K = 4k + s of 64 top rows holds y0 + 64K + l in
+    lane l; lane l's read vq[base + a] (base = y0 + 256k + l, or y0 for a lane
+    whose quad has no valid top row) packs the free runs of rows base + a +
+    64s (rows past H pack as 0); returns the passing top rows in the kernel's
+    order (words descending, lanes by descending row within a word)."""
     def vq(y):
         assert 0 <= y < H, y                       # every read stays below H
-        return [vac[y + s] if y + s < H else 0 for s in range(4)]
+        return [vac[y + 64 * s] if y + 64 * s < H else 0 for s in range(4)]
 
-    c = [[0] * 4 for _ in range(nq)]
-    base = [[0] * 64 for _ in range(nq)]
+    ok = {}
     for k in range(nq):
         for lane in range(64):
-            t = 256 * k + 4 * lane
-            for s in range(4):
-                if t + s < ny:
-                    c[k][s] |= 1 << lane
-            base[k][lane] = y0 + (t if t < ny else 0)
-    for d, row in den_sorted:
-        for k in range(nq):
-            for lane in range(64):
-                f = vq(base[k][lane] + row)
+            t = 256 * k + lane
+            base = y0 + (t if t < ny else 0)
+            fields = [t + 64 * s < ny for s in range(4)]
+            for d, row in den_sorted:
+                f = vq(base + row)
                 for s in range(4):
                     if f[s] < d:
-                        c[k][s] &= ~(1 << lane)
+                        fields[s] = False
+            for s in range(4):
+                ok[(4 * k + s, lane)] = fields[s]
     out = []
-    for k in range(nq - 1, -1, -1):
-        tot = sum(bin(c[k][s]).count("1") for s in range(4))
-        slots = [None] * tot
-        for lane in range(64):
-            above = sum(bin(c[k][s] >> (lane + 1)).count("1") for s in range(4))
-            p = above
-            for s in range(3, -1, -1):
-                if (c[k][s] >> lane) & 1:
-                    slots[p] = y0 + 256 * k + 4 * lane + s
-                    p += 1
-        out += slots
-    return c, out
+    for K in range(4 * nq - 1, -1, -1):
+        out += [y0 + 64 * K + lane for lane in range(63, -1, -1) if ok[(K, lane)]]
+    return out
 
 
-def test_vacancy_quads_list_the_candidates_descending():
+def test_vacancy_words_list_the_candidates_descending():
     """The packed vacancy filter lists exactly the top rows y whose rows
     y..y+h-1 all have a free run at least the mask row's density, in
-    descending order, over passes of two quad words (512 top rows)."""
+    descending order, over passes of 512 top rows (one quad of 256 when that
+    covers the rest)."""
     rng = random.Random(13)
-    for _ in range(200):
+    for _ in range(150):
         h = rng.randint(1, 40)
         H = rng.randint(h, 1200)
         ny = H - h + 1
@@ -257,9 +245,9 @@ def test_vacancy_quads_list_the_candidates_descending():
         den = [rng.randint(1, 60) for _ in range(h)]
         den_sorted = sorted(((den[a], a) for a in range(h)), key=lambda t: -t[0])
         want = [y for y in range(ny - 1, yf - 1, -1) if all(vac[y + a] >= den[a] for a in range(h))]
-        nq = (ny - yf + 255) // 256
         got = []
-        for cpass in range((nq - 1) // 2, -1, -1):
-            y0 = yf + 512 * cpass
-            got += vacancy_quads(vac, den_sorted, ny - y0, y0, min(2, nq - 2 * cpass), H)[1]
+        if ny > yf:
+            for cpass in range((ny - yf - 1) // 512, -1, -1):
+                y0 = yf + 512 * cpass
+                got += vacancy_words(vac, den_sorted, ny - y0, y0, 1 if ny - y0 <= 256 else 2, H)
         assert got == want
