@@ -1321,11 +1321,13 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
     int *s_mach = (int *)(s_sv + n);                        // [n]
     int *s_ty = s_mach + n;                                 // [n] part type at sequence position i
     int *s_den = s_ty + n;                                  // [<= MAX_ROT * hrot] one type's densities
-    lds_u64 *s_vq = (lds_u64 *)(s_den + MAX_ROT * Wk.hrot); // [hmax] the tested bin's free runs, 4 rows a word
-    short *s_cand = (short *)(s_vq + hmax);                 // [MAX_ROT][hmax]
+    short *s_cand = (short *)(s_den + MAX_ROT * Wk.hrot);   // [MAX_ROT][hmax]
     short *s_vcache = s_cand + MAX_ROT * hmax;              // [vc_bins][hmax] vacancy of bins < vc_bins
     lds_u32 *s_q = (lds_u32 *)(smem + Wk.q_off);            // [hrot * (max nc + 1)]
     lds_u32 *band = (lds_u32 *)(smem + Wk.band_off);        // [band_words]
+    // [hmax] the tested bin's free runs, 4 rows a word: they live in the band,
+    // which the vacancy filter does not use (it is staged after the filter)
+    lds_u64 *s_vq = (lds_u64 *)band;
     lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);          // [2][4][zw_wave] zero masks, 4 screen masks
     __shared__ int s_ctx, s_cnt, s_off, s_infeasible, s_nb;
     __shared__ int s_ncand[MAX_ROT];
@@ -1568,10 +1570,10 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = P.n_parts, hmax = Wk.hmax, tid = threadIdx.x;
     int *s_den = (int *)((double *)smem + n) + 2 * n;       // same layout as decode_kernel
-    lds_u64 *s_vq = (lds_u64 *)(s_den + MAX_ROT * Wk.hrot);
-    short *s_cand = (short *)(s_vq + hmax);
+    short *s_cand = (short *)(s_den + MAX_ROT * Wk.hrot);
     lds_u32 *s_q = (lds_u32 *)(smem + Wk.q_off);
     lds_u32 *band = (lds_u32 *)(smem + Wk.band_off);
+    lds_u64 *s_vq = (lds_u64 *)band;   // the free runs, in the band (staged after the filter)
     lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);
     __shared__ int s_ncand[MAX_ROT];
     const int *job = A.jobs + 3 * blockIdx.x;
@@ -1818,8 +1820,7 @@ static int compute_layouts(brkga_handle *h, int mb) {
     // fixed arrays for a vacancy cache of vc bins, offsets into w
     auto layout = [&](int vc, int wpc) -> size_t {
         w.vc_bins = vc;
-        // (the packed free runs vq, 8 bytes a row, follow the densities)
-        size_t f = (size_t)li.n_parts * (8 + 4 + 4) + (size_t)MAX_ROT * li.hrot * 4 + (size_t)li.hmax * 8 +
+        size_t f = (size_t)li.n_parts * (8 + 4 + 4) + (size_t)MAX_ROT * li.hrot * 4 +
                    (size_t)MAX_ROT * li.hmax * 2 + (size_t)vc * li.hmax * 2;
         f = (f + 15) & ~(size_t)15;
         w.q_off = (int)f;
@@ -1833,6 +1834,7 @@ static int compute_layouts(brkga_handle *h, int mb) {
     const int vc_max = std::min(mb, 16);
     auto make = [&](int wpc, int vcb, size_t need, LdsLayout &L) -> bool {
         const size_t fixed = layout(vcb, wpc);
+        need = std::max(need, (size_t)li.hmax * 8);   // the band also holds the packed free runs
         const size_t per_wg = li.lds_cu / wpc;
         if (per_wg < fixed + need + LDS_RESERVE) return false;
         L.wpc = wpc;
