@@ -1111,18 +1111,49 @@ __device__ __forceinline__ int vacancy_rotation(lds_cu64 *vq, lds_ci32 *den, int
     return nc_;
 }
 
-// Densities (longest 1-run per mask row, packed with the row as
-// (density << 16) | 8 row, densest rows first) of rotations rb..rb+nr-1 into
-// s_den, rotation after rotation (rotation r at the sum of the heights of
-// rotations rb..rb+r-1).  brkga_create appends the densities in rotation
-// order and a type's rotations are consecutive, so this is one contiguous
-// copy by the whole workgroup (the caller synchronises before the vacancy
-// filter reads them).
-__device__ __forceinline__ void load_densities(const DevProblem &P, int rb, int nr, int *s_den) {
-    int tot = 0;
-    for (int r = 0; r < nr; r++) tot += P.rot_h[rb + r];
-    const int *src = P.dens + P.dens_off[rb];
+// A part type's rotations, staged in LDS once per part (load_part): every
+// vacancy filter, collision test and stamp of the part reads them there
+// instead of following the problem tables in HBM (a chain of dependent loads
+// per test otherwise).
+struct RotInfo {
+    int h, w, nc, nc4;                 // mask rows and columns; Toeplitz chunks (int8, fp4)
+    int tiles, window, screen, doff;   // the machine's tile-table entry; densities' offset in s_den
+    long long qoff;                    // mask bits of the rotation in P.qbits
+};
+typedef LDSP RotInfo lds_rot;
+
+// Rotations rb..rb+nr-1 of a part's type into ri (thread r fills rotation r;
+// tiles: the context machine's tile table, TUNE_NC entries of TILE_ENTRY
+// bytes) and their densities ((density << 16) | 8 row, densest rows first)
+// into s_den, rotation after rotation (rotation r at ri[r].doff).
+// brkga_create appends the densities in rotation order and a type's rotations
+// are consecutive, so they arrive in one contiguous copy by the whole
+// workgroup.  Ends synchronised.
+__device__ __forceinline__ void load_part(const DevProblem &P, const unsigned char *tiles, int rb, int nr,
+                                          int *s_den, lds_rot *ri) {
+    const int tid = threadIdx.x;
+    const long long dens0 = P.dens_off[rb];
+    if (tid < nr) {
+        const int rg = rb + tid;
+        const int nc = P.rot_nc[rg];
+        const unsigned char *tc = tiles + min(nc, TUNE_NC - 1) * TILE_ENTRY;
+        int doff = 0;
+        for (int q = 0; q < tid; q++) doff += P.rot_h[rb + q];
+        ri[tid].h = P.rot_h[rg];
+        ri[tid].w = P.rot_w[rg];
+        ri[tid].nc = nc;
+        ri[tid].nc4 = P.rot_nc4[rg];
+        ri[tid].qoff = P.qbits_off[rg];
+        ri[tid].tiles = tc[0];
+        ri[tid].window = tc[1];
+        ri[tid].screen = tc[2];
+        ri[tid].doff = doff;
+    }
+    __syncthreads();
+    const int tot = ri[nr - 1].doff + ri[nr - 1].h;
+    const int *src = P.dens + dens0;
     wg_copy<int>(tot, [&](int k) { return src[k]; }, [&](int k, int v) { s_den[k] = v; });
+    __syncthreads();
 }
 
 // Vacancy filter of a part's rotations against one bin, wave r: rotation r.
@@ -1136,17 +1167,14 @@ __device__ __forceinline__ void load_densities(const DevProblem &P, int rb, int 
 // the top rows from there on are tested, and the others follow them in the
 // list (they are the highest, so the list stays descending).
 template <bool DENSE>
-__device__ __forceinline__ void vacancy_filter(const DevProblem &P, int rb, int nr, int H, int W, int minrow,
+__device__ __forceinline__ void vacancy_filter(lds_rot *ri, int nr, int H, int W, int minrow,
                                                lds_cu64 *vq, const int *s_den, short *s_cand, int hmax,
                                                int *s_ncand) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (wave < nr) {
-        const int rg = rb + wave;
-        const int h = P.rot_h[rg], w = P.rot_w[rg];
+        const int h = uni(ri[wave].h), w = uni(ri[wave].w);
         short *cand = s_cand + wave * hmax;
-        int doff = 0;
-        for (int q = 0; q < wave; q++) doff += P.rot_h[rb + q];
-        const int *den = s_den + doff;
+        const int *den = s_den + uni(ri[wave].doff);
         int nc_ = 0;
         if (h <= H && w <= W) {
             const int ny = H - h + 1;
@@ -1206,7 +1234,7 @@ __device__ __forceinline__ int winning_prefix(const short *cand, int ncand, int 
 // index, so the order changes no choice.  Returns the rotation, or -1, with
 // its top-left (by, bx); uniform across the workgroup.
 template <bool DENSE, bool F4, int WPC>
-__device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork &Wk, int mach, int rb, int nr, int rf,
+__device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork &Wk, lds_rot *ri, int nr, int rf,
                                               int H, int W, int enc, const unsigned *grid,
                                               const short *s_cand, const int *s_ncand, int hmax, lds_u32 *band,
                                               lds_u32 *s_q, lds_u64 *s_zw, lds_u64 &c_tests,
@@ -1217,8 +1245,7 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
         const int r = k == 0 ? rf : (k <= rf ? k - 1 : k);
         int ncand = uni(s_ncand[r]);
         if (ncand == 0) continue;
-        const int rg = rb + r;
-        const int h = P.rot_h[rg], w = P.rot_w[rg];
+        const int h = uni(ri[r].h), w = uni(ri[r].w);
         const short *cand = s_cand + r * hmax;
         if (!DENSE && best >= 0) {
             ncand = uni(winning_prefix(cand, ncand, enc, H, h, bl, brow));
@@ -1228,12 +1255,12 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
             c_tests += 1;
             c_algo += 2ull * h * w * (unsigned long long)(H - h + 1) * (W - w + 1);
         }
-        const int ncr = P.rot_nc[rg], nc4 = P.rot_nc4[rg];
-        const unsigned char *tc = P.tiles + ((long long)mach * TUNE_NC + min(ncr, TUNE_NC - 1)) * TILE_ENTRY;
+        const int ncr = uni(ri[r].nc), nc4 = uni(ri[r].nc4);
+        const long long qoff = ri[r].qoff;
         const int key = test_rotation<DENSE, F4, WPC>(grid, Wk.pw, W, h, w, F4 ? nc4 : ncr, F4 ? 2 * nc4 : ncr,
-                                                 2 * nc4 + 2, P.qbits + P.qbits_off[rg], cand, ncand, H - enc,
+                                                 2 * nc4 + 2, P.qbits + qoff, cand, ncand, H - enc,
                                              band, Wk.band_words, s_q, s_zw, c_groups, c_mfma, ph,
-                                             uni(tc[0]), uni(tc[1]), uni(tc[2]));
+                                             uni(ri[r].tiles), uni(ri[r].window), uni(ri[r].screen));
         if (key == KEY_NONE) continue;
         // density (new enclosure length) -> row -> col (binClassNew.py:101-117)
         const int y = uni(cand[key / XKEY]), x = key % XKEY, row = y + h - 1;
@@ -1246,27 +1273,27 @@ __device__ __forceinline__ int best_placement(const DevProblem &P, const DevWork
     return best;
 }
 
-// Stamp rotation rg at top-left (py, px) into a bin (binClassNew.py:155-160,
+// Stamp a rotation (h x w mask, nc4 fp4 chunks, mask bits at qbits) at
+// top-left (py, px) into a bin (binClassNew.py:155-160,
 // cuda_batch_update.py:90-126), whole workgroup: rows [py, py+h) pass through
 // the LDS band in chunks -- the grid's bit rows (a fresh bin's rows are zero)
 // and the mask bits come in one batched copy, every grid word the part covers
 // takes the 32 mask bits that fall on it, those words go back to HBM, and the
 // rows' free runs go to vv (HBM) and to vcache when it is not null.
-__device__ __forceinline__ void stamp_part(const DevProblem &P, const DevWork &Wk, int rg, int W, unsigned *grid,
-                                           int *vv, short *vcache, bool fresh, int py, int px, lds_u32 *band,
-                                           lds_u32 *s_q) {
+__device__ __forceinline__ void stamp_part(const DevWork &Wk, int h, int w, int nc4, const unsigned *qbits, int W,
+                                           unsigned *grid, int *vv, short *vcache, bool fresh, int py, int px,
+                                           lds_u32 *band, lds_u32 *s_q) {
     const int tid = threadIdx.x;
-    const int h = P.rot_h[rg], w = P.rot_w[rg];
     const int pw = Wk.pw, q4 = pw >> 2;         // words and 16-byte vectors per row
     const DivBy byq4(q4);
     const int chunk = min(h, Wk.band_words / pw);
-    const int nq = 2 * P.rot_nc4[rg] + 2;       // mask bit words per row
+    const int nq = 2 * nc4 + 2;                 // mask bit words per row
     const int q0 = px >> 5, nw = ((px + w - 1) >> 5) - q0 + 1;   // grid words per row the part covers
     const DivBy bynw(nw);
     for (int a0 = 0; a0 < h; a0 += chunk) {
         const int rows = min(chunk, h - a0);
         const unsigned *g0 = grid + (long long)(py + a0) * pw;
-        const unsigned *qg = P.qbits + P.qbits_off[rg] + (long long)a0 * nq;
+        const unsigned *qg = qbits + (long long)a0 * nq;
         if (fresh)
             for (int k = tid; k < rows * q4; k += WG_THREADS) *(lds_v4i *)(band + (k << 2)) = (v4i){0, 0, 0, 0};
         const int ng = fresh ? 0 : rows * q4;
@@ -1331,6 +1358,8 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
     lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);          // [2][4][zw_wave] zero masks, 4 screen masks
     __shared__ int s_ctx, s_cnt, s_off, s_infeasible, s_nb;
     __shared__ int s_ncand[MAX_ROT];
+    __shared__ RotInfo s_ri_[MAX_ROT];   // the current part's rotations
+    lds_rot *s_ri = (lds_rot *)s_ri_;
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
@@ -1387,6 +1416,7 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
         const long long chrom = ctx / M;
         const int mach = (int)(ctx % M);
         const int H = P.plate_h[mach], W = P.plate_w[mach];
+        const unsigned char *tiles = P.tiles + (long long)mach * TUNE_NC * TILE_ENTRY;   // the machine's tile table
         const double plate_area = (double)H * (double)W;
         if (tid == 0) { s_cnt = 0; s_off = 0; s_infeasible = 0; s_nb = 0; }
         // ---- sequence decode (placement.py:109-133) ----
@@ -1430,9 +1460,9 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
             const double parea = P.area[ty];
             const int nr = P.nrot[ty];
             const int rb = P.rot_base[ty];
+            const int best_rot = P.best_rot[ty];
             const int nb = uni(s_nb);
-            load_densities(P, rb, nr, s_den);
-            __syncthreads();   // s_den is read by every wave's vacancy filter
+            load_part(P, tiles, rb, nr, s_den, s_ri);   // s_ri and s_den are read by every wave
             bool placed = false;
             int pb = 0, py = 0, px = 0, prot = 0;
             bool fresh = false;
@@ -1449,12 +1479,12 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
                 }
                 __syncthreads();
                 // ---- vacancy filter, per candidate top row (wave r: rotation r) ----
-                vacancy_filter<DENSE>(P, rb, nr, H, W, b_minrow[b], s_vq, s_den, s_cand, hmax, s_ncand);
+                vacancy_filter<DENSE>(s_ri, nr, H, W, b_minrow[b], s_vq, s_den, s_cand, hmax, s_ncand);
                 __syncthreads();
                 if (tid == 0) ph[1] += tick() - ph[11];
                 // ---- collision tests, one rotation at a time (whole workgroup) ----
                 int bx = 0, by = 0;
-                const int best = best_placement<DENSE, F4, WPC>(P, Wk, mach, rb, nr, P.best_rot[ty], H, W, H - b_minrow[b], grid, s_cand,
+                const int best = best_placement<DENSE, F4, WPC>(P, Wk, s_ri, nr, best_rot, H, W, H - b_minrow[b], grid, s_cand,
                                                        s_ncand, hmax, band, s_q, s_zw, c_tests, c_algo, c_groups,
                                                        c_mfma, ph, by, bx);
                 if (best >= 0) {
@@ -1466,8 +1496,8 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
             if (tid == 0) ph[11] = tick();
             if (!placed) {
                 if (nb >= Wk.max_bins) { overflow = true; break; }
-                pb = nb; prot = P.best_rot[ty];
-                py = H - P.rot_h[rb + prot]; px = 0;
+                pb = nb; prot = best_rot;
+                py = H - uni(s_ri[prot].h); px = 0;
                 fresh = true;
                 // fresh bin: zero grid, full vacancy (full_native_decoder.py:2087-2089)
                 v4i *g4 = (v4i *)(grids + (long long)pb * hmax * Wk.pw);
@@ -1487,9 +1517,9 @@ __global__ __launch_bounds__(WG_THREADS, WPC) void decode_kernel(DevProblem P, D
             }
             // ---- stamp the mask (binClassNew.py:155-160, cuda_batch_update.py:90-126) ----
             {
-                const int rg = rb + prot;
-                const int h = P.rot_h[rg];
-                stamp_part(P, Wk, rg, W, grids + (long long)pb * hmax * Wk.pw, vacs + (long long)pb * hmax,
+                const int h = uni(s_ri[prot].h);
+                stamp_part(Wk, h, uni(s_ri[prot].w), uni(s_ri[prot].nc4), P.qbits + s_ri[prot].qoff, W,
+                           grids + (long long)pb * hmax * Wk.pw, vacs + (long long)pb * hmax,
                            pb < Wk.vc_bins ? s_vcache + pb * hmax : nullptr, fresh, py, px, band, s_q);
                 if (tid == 0) {   // binClassNew.py:147-153,172-174 ; placement.py:81-82
                     const long long mp = (long long)mach * P.n_types + ty;
@@ -1576,28 +1606,30 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_test_kernel(DevProblem P
     lds_u64 *s_vq = (lds_u64 *)band;   // the free runs, in the band (staged after the filter)
     lds_u64 *s_zw = (lds_u64 *)(smem + Wk.zw_off);
     __shared__ int s_ncand[MAX_ROT];
+    __shared__ RotInfo s_ri_[MAX_ROT];
+    lds_rot *s_ri = (lds_rot *)s_ri_;
     const int *job = A.jobs + 3 * blockIdx.x;
     const int m = job[0], b = job[1], enc = job[2], ty = A.ty;
     const int H = P.plate_h[m], W = P.plate_w[m], nr = P.nrot[ty], rb = P.rot_base[ty];
     const long long slot = (long long)m * A.max_batches + b;
     const unsigned *grid = A.grids + slot * hmax * Wk.pw;
     const int *vac = A.vac + slot * hmax;
-    load_densities(P, rb, nr, s_den);
+    load_part(P, P.tiles + (long long)m * TUNE_NC * TILE_ENTRY, rb, nr, s_den, s_ri);
     vacancy_quads(s_vq, H, [&](int y) { return vac[y]; });
     __syncthreads();
-    vacancy_filter<false>(P, rb, nr, H, W, H - enc, s_vq, s_den, s_cand, hmax, s_ncand);
+    vacancy_filter<false>(s_ri, nr, H, W, H - enc, s_vq, s_den, s_cand, hmax, s_ncand);
     __syncthreads();
     // statistics and phase times (unused here; the decode's)
     __shared__ unsigned long long s_st[4 + 4], s_tm[8];
     lds_u64 *st = (lds_u64 *)s_st, *ph = (lds_u64 *)s_tm;
     int bx = 0, by = 0;
-    const int best = best_placement<false, F4, 2>(P, Wk, m, rb, nr, P.best_rot[ty], H, W, enc, grid, s_cand, s_ncand, hmax, band, s_q,
+    const int best = best_placement<false, F4, 2>(P, Wk, s_ri, nr, P.best_rot[ty], H, W, enc, grid, s_cand, s_ncand, hmax, band, s_q,
                                            s_zw, st[0], st[1], st[2], st[4 + (tid >> 6)], ph, by, bx);
     if (tid == 0) {
         int *r = A.res + 4 * blockIdx.x;
         r[0] = best >= 0;
         r[1] = bx;
-        r[2] = best >= 0 ? by + P.rot_h[rb + best] - 1 : 0;
+        r[2] = best >= 0 ? by + s_ri[best].h - 1 : 0;
         r[3] = best;
     }
 }
@@ -1621,7 +1653,8 @@ __global__ __launch_bounds__(WG_THREADS, 2) void greedy_insert_kernel(DevProblem
         for (int k = threadIdx.x; k < I.H; k += WG_THREADS) I.vac[k] = I.W;
         __syncthreads();
     }
-    stamp_part(P, Wk, I.rg, I.W, I.grid, I.vac, nullptr, I.fresh != 0, I.py, I.px, band, s_q);
+    stamp_part(Wk, P.rot_h[I.rg], P.rot_w[I.rg], P.rot_nc4[I.rg], P.qbits + P.qbits_off[I.rg], I.W, I.grid, I.vac,
+               nullptr, I.fresh != 0, I.py, I.px, band, s_q);
 }
 
 __global__ void finalize_kernel(const double *ctx_ms, long long n_chrom, int M, double *out,
