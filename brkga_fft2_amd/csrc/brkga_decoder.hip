@@ -765,12 +765,16 @@ __device__ __attribute__((always_inline)) int test_rotation(const unsigned *grid
                              lds_u64 &groups, lds_u64 &mfma, lds_u64 *ph,
                              int tpcap, int wcap, int screen) {
     const int tid = threadIdx.x, lane = tid & 63;
-    // mask bits of the rotation -> LDS (read by every wave for every group);
-    // timed with the band staging, whose first barrier they share
-    unsigned long long tq = tid == 0 ? tick() : 0;
-    wg_copy<unsigned>(h * qs, [&](int k) { return qbits_g[k]; },
-                      [&](int k, unsigned v) { s_q[k] = v; });
-    if (tid == 0) ph[2] += tick() - tq;
+    // mask bits of the rotation -> LDS (read by every wave for every group)
+    // with direct-to-LDS loads: they take no registers and are not waited
+    // for here; the first band staging's wait for its own loads (issued
+    // later, completed in order) and its barrier cover them
+    {
+        const int nq = h * qs, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+        for (int k0 = wave << 6; k0 < nq; k0 += WG_THREADS)
+            if (k0 + lane < nq)
+                __builtin_amdgcn_global_load_lds((const void *)(qbits_g + k0 + lane), (LDSP void *)(s_q + k0), 4, 0, 0);
+    }
     const int T = (W - w + 1 + 31) >> 5;
     int stride, rcap;
     const int TW = band_window(T, cw, h, band_words, wcap, &stride, &rcap);
