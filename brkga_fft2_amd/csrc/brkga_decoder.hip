@@ -1030,10 +1030,10 @@ __device__ __forceinline__ void vacancy_quads(lds_u64 *vq, int H, Get get) {
 // K = 4k + s holding y0 + 64K + l in lane l: for mask row a (density d), lane
 // l's one 8-byte read vq[y0 + 256k + l + a] gives the free runs v_s of the
 // grid rows its four top rows of quad k meet (consecutive lanes read
-// consecutive words: no bank conflicts), and packed 16-bit subtractions and
-// maxima keep acc_s = max over the rows so far of d - v_s; top row s passes
-// when acc_s <= 0 (free runs and densities are at most 4096, so no field
-// overflows).  den holds (density << 16) | 8a for the rotation's mask rows a
+// consecutive words: no bank conflicts), and packed 16-bit saturating
+// subtractions and three-input ORs keep acc_s = OR over the rows so far of
+// max(d - v_s, 0); top row s passes while acc_s == 0 (free runs and densities
+// are at most 4096: unsigned 16-bit fields).  den holds (density << 16) | 8a for the rotation's mask rows a
 // (8a: the row's byte offset in vq), the densest first, so that a bin with
 // no room fails within the first steps (the result is a maximum over the
 // rows, whatever their order); the entries of rows b .. b+63 are held one per
@@ -1046,19 +1046,21 @@ __device__ __forceinline__ void vacancy_quads(lds_u64 *vq, int H, Get get) {
 #ifndef VAC_STEP
 #define VAC_STEP 8   // mask rows per step of the vacancy filter (their reads are in flight together)
 #endif
-typedef short s16x2 __attribute__((ext_vector_type(2)));
+static_assert(64 % VAC_STEP == 0 && VAC_STEP % 2 == 0, "vacancy steps tile a 64-row block in row pairs");
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 template <int NQ>
 __device__ __forceinline__ int vacancy_rotation(lds_cu64 *vq, lds_ci32 *den, int h, int ny, short *cand,
                                                 int y0) {
     const int lane = threadIdx.x & 63;
-    s16x2 acc[NQ][2];
+    u16x2 acc[NQ][2];
     LDSP const unsigned char *p[NQ];
 #pragma unroll
     for (int k = 0; k < NQ; k++) {
         const int t = (k << 8) + lane;   // the lane's top row of word 4k, from y0
 #pragma unroll
         for (int j = 0; j < 2; j++)
-            acc[k][j] = (s16x2){(short)(t + 128 * j < ny ? -32768 : 1), (short)(t + 128 * j + 64 < ny ? -32768 : 1)};
+            acc[k][j] = (u16x2){(unsigned short)(t + 128 * j < ny ? 0 : 1),
+                                (unsigned short)(t + 128 * j + 64 < ny ? 0 : 1)};
         p[k] = (LDSP const unsigned char *)(vq + y0 + (t < ny ? t : 0));
     }
     bool alive = true;
@@ -1066,34 +1068,37 @@ __device__ __forceinline__ int vacancy_rotation(lds_cu64 *vq, lds_ci32 *den, int
         const int dl = den[min(a0 + lane, h - 1)];
         const int a1 = min(h, a0 + 64);
         for (int b = a0; b < a1; b += VAC_STEP) {
-            s16x2 d[VAC_STEP];
+            u16x2 d[VAC_STEP];
             int ab[VAC_STEP];
 #pragma unroll
             for (int j = 0; j < VAC_STEP; j++) {
-                const int e = __builtin_amdgcn_readlane(dl, min(b + j, a1 - 1) - a0);
-                const short dv = (short)((unsigned)e >> 16);
-                d[j] = (s16x2){dv, dv};
+                // (b - a0 + j < 64: a block is 64 rows, VAC_STEP divides 64;
+                // lanes past the rotation's last row hold that row again)
+                const int e = __builtin_amdgcn_readlane(dl, b - a0 + j);
+                const unsigned short dv = (unsigned short)((unsigned)e >> 16);
+                d[j] = (u16x2){dv, dv};
                 ab[j] = e & 0xffff;
             }
 #pragma unroll
-            for (int j = 0; j < VAC_STEP; j++)
+            for (int j = 0; j < VAC_STEP; j += 2)   // rows in pairs: one three-input OR per pair
 #pragma unroll
                 for (int k = 0; k < NQ; k++) {
-                    const unsigned long long q = *(lds_cu64 *)(p[k] + ab[j]);
-                    const s16x2 lo = __builtin_bit_cast(s16x2, (unsigned)q);
-                    const s16x2 hi = __builtin_bit_cast(s16x2, (unsigned)(q >> 32));
-                    acc[k][0] = __builtin_elementwise_max(acc[k][0], d[j] - lo);
-                    acc[k][1] = __builtin_elementwise_max(acc[k][1], d[j] - hi);
+                    const unsigned long long q0 = *(lds_cu64 *)(p[k] + ab[j]);
+                    const unsigned long long q1 = *(lds_cu64 *)(p[k] + ab[j + 1]);
+                    const u16x2 l0 = __builtin_bit_cast(u16x2, (unsigned)q0), h0 = __builtin_bit_cast(u16x2, (unsigned)(q0 >> 32));
+                    const u16x2 l1 = __builtin_bit_cast(u16x2, (unsigned)q1), h1 = __builtin_bit_cast(u16x2, (unsigned)(q1 >> 32));
+                    acc[k][0] = acc[k][0] | __builtin_elementwise_sub_sat(d[j], l0) | __builtin_elementwise_sub_sat(d[j + 1], l1);
+                    acc[k][1] = acc[k][1] | __builtin_elementwise_sub_sat(d[j], h0) | __builtin_elementwise_sub_sat(d[j + 1], h1);
                 }
             // the step's reads first, then its arithmetic: the reads are in
             // flight together rather than each waited for before the next
             __builtin_amdgcn_sched_group_barrier(0x0100, VAC_STEP * NQ, 0);
             __builtin_amdgcn_sched_group_barrier(0x0002, 4 * VAC_STEP * NQ, 0);
-            s16x2 m = __builtin_elementwise_min(acc[0][0], acc[0][1]);
+            u16x2 m = __builtin_elementwise_min(acc[0][0], acc[0][1]);
 #pragma unroll
             for (int k = 1; k < NQ; k++)
                 m = __builtin_elementwise_min(m, __builtin_elementwise_min(acc[k][0], acc[k][1]));
-            if (!__ballot(m.x <= 0 || m.y <= 0)) { alive = false; break; }
+            if (!__ballot(m.x == 0 || m.y == 0)) { alive = false; break; }
         }
     }
     if (!alive) return 0;
@@ -1103,8 +1108,8 @@ __device__ __forceinline__ int vacancy_rotation(lds_cu64 *vq, lds_ci32 *den, int
     int nc_ = 0;
 #pragma unroll
     for (int K = 4 * NQ - 1; K >= 0; K--) {
-        const s16x2 v = acc[K >> 2][(K >> 1) & 1];
-        const unsigned long long c = __ballot(((K & 1) ? v.y : v.x) <= 0);
+        const u16x2 v = acc[K >> 2][(K >> 1) & 1];
+        const unsigned long long c = __ballot(((K & 1) ? v.y : v.x) == 0);
         if (c) {
             const unsigned long long rv = __builtin_bitreverse64(c);
             const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(rv >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)rv, 0));
