@@ -1033,11 +1033,12 @@ __device__ __forceinline__ void vacancy_quads(lds_u64 *vq, int H, Get get) {
 // consecutive words: no bank conflicts), and packed 16-bit saturating
 // subtractions and three-input ORs keep acc_s = OR over the rows so far of
 // max(d - v_s, 0); top row s passes while acc_s == 0 (free runs and densities
-// are at most 4096: unsigned 16-bit fields).  den holds (density << 16) | 8a for the rotation's mask rows a
-// (8a: the row's byte offset in vq), the densest first, so that a bin with
-// no room fails within the first steps (the result is a maximum over the
-// rows, whatever their order); the entries of rows b .. b+63 are held one per
-// lane and a step's come from registers (readlane).  VAC_STEP mask rows per
+// are at most 4096: unsigned 16-bit fields).  den holds (density << 16) | 8a
+// for the rotation's mask rows a (8a: the row's byte offset in vq), the
+// densest first, so that a bin with no room fails within the first steps
+// (the result is an OR over the rows, whatever their order); the entries of
+// rows b .. b+63 are held one per lane and a step's come from registers
+// (readlane).  VAC_STEP mask rows per
 // step keep their reads in flight together, and after each step the wave
 // stops once no top row is left.  ny: valid top rows from y0 on (H - h + 1 -
 // y0); the fields of top rows past them start at 1 (failed), and a lane
