@@ -1142,13 +1142,14 @@ typedef LDSP RotInfo lds_rot;
 __device__ __forceinline__ void load_part(const DevProblem &P, const unsigned char *tiles, int rb, int nr,
                                           int *s_den, lds_rot *ri) {
     const int tid = threadIdx.x;
+    // offsets from dens_off: the copy need not wait for ri
     const long long dens0 = P.dens_off[rb];
+    const int tot = (int)(P.dens_off[rb + nr - 1] - dens0) + P.rot_h[rb + nr - 1];
     if (tid < nr) {
         const int rg = rb + tid;
         const int nc = P.rot_nc[rg];
         const unsigned char *tc = tiles + min(nc, TUNE_NC - 1) * TILE_ENTRY;
-        int doff = 0;
-        for (int q = 0; q < tid; q++) doff += P.rot_h[rb + q];
+        const int doff = (int)(P.dens_off[rg] - dens0);
         ri[tid].h = P.rot_h[rg];
         ri[tid].w = P.rot_w[rg];
         ri[tid].nc = nc;
@@ -1159,8 +1160,6 @@ __device__ __forceinline__ void load_part(const DevProblem &P, const unsigned ch
         ri[tid].screen = tc[2];
         ri[tid].doff = doff;
     }
-    __syncthreads();
-    const int tot = ri[nr - 1].doff + ri[nr - 1].h;
     const int *src = P.dens + dens0;
     wg_copy<int>(tot, [&](int k) { return src[k]; }, [&](int k, int v) { s_den[k] = v; });
     __syncthreads();
