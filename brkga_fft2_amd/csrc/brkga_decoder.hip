@@ -466,28 +466,17 @@ __device__ __forceinline__ unsigned screen_pass(lds_cu32 *band, int stride, int 
     for (int t = 0; t < TW; t++) acc[t] = (AT){};
     lds_cu32 *base = band + loff + (F4 ? g : 0);
     lds_cu32 *qb = qbits + (F4 ? g : 0);
-    // row i's mask and band words are loaded while row i - 1 correlates
-    unsigned qn[NQ], An[NA];
-    {
-        const int a = skip;
+    // as in corr_pass: the next row's mask words are loaded over the current
+    // ones once its B fragments are built, and band word j + D while word j
+    // correlates (the last D of a row load the next row's first D)
+    constexpr int D = NA < 2 ? NA : 2;
+    unsigned N[D], qv[NQ];
+    int a = skip;
 #pragma unroll
-        for (int c = 0; c < NQ; c++) qn[c] = qb[a * qs + c];
+    for (int j = 0; j < D; j++) N[j] = base[a * stride + min(t0 + j, wlast)];
 #pragma unroll
-        for (int j = 0; j < NA; j++) An[j] = base[a * stride + min(t0 + j, wlast)];
-    }
+    for (int c = 0; c < NQ; c++) qv[c] = qb[a * qs + c];
     for (int i = 0; i < S; i++) {
-        unsigned qv[NQ], Aw[NA];
-#pragma unroll
-        for (int c = 0; c < NQ; c++) qv[c] = qn[c];
-#pragma unroll
-        for (int j = 0; j < NA; j++) Aw[j] = An[j];
-        if (i + 1 < S) {
-            const int a = skip + (int)((step * (unsigned)(i + 1)) >> 16);
-#pragma unroll
-            for (int c = 0; c < NQ; c++) qn[c] = qb[a * qs + c];
-#pragma unroll
-            for (int j = 0; j < NA; j++) An[j] = base[a * stride + min(t0 + j, wlast)];
-        }
         typedef typename std::conditional<F4, v8i, v4i>::type FT;
         FT B[NC];
 #pragma unroll
@@ -495,8 +484,19 @@ __device__ __forceinline__ unsigned screen_pass(lds_cu32 *band, int stride, int 
             if constexpr (F4) B[c] = nib_frag((unsigned)((((unsigned long long)qv[2 * c + 1] << 32) | qv[2 * c]) >> bs));
             else B[c] = bfrag_from_bits(qv[c], qv[c + 1], bs);
         }
+        const int an = i + 1 < S ? skip + (int)((step * (unsigned)(i + 1)) >> 16) : a;
+        lds_cu32 *rc = base + a * stride;
+        lds_cu32 *rn = base + an * stride;
+#pragma unroll
+        for (int c = 0; c < NQ; c++) qv[c] = qb[an * qs + c];
+        unsigned Aw[NA];
+#pragma unroll
+        for (int j = 0; j < D; j++) Aw[j] = N[j];
 #pragma unroll
         for (int j = 0; j < NA; j++) {
+            const int p = j + D;
+            if (p < NA) Aw[p] = rc[min(t0 + p, wlast)];
+            else N[p - NA] = rn[min(t0 + p - NA, wlast)];
             if constexpr (F4) {
                 const v8i A = nib_frag(Aw[j]);
 #pragma unroll
@@ -513,6 +513,7 @@ __device__ __forceinline__ unsigned screen_pass(lds_cu32 *band, int stride, int 
                 }
             }
         }
+        a = an;
     }
     unsigned bits = 0;
 #pragma unroll
