@@ -528,10 +528,11 @@ __device__ __forceinline__ unsigned screen_pass(lds_cu32 *band, int stride, int 
     return bits;
 }
 
-// Tiles per screening step of one wave at most: the accumulators of a step and
-// the B fragments of a row stay within half the full pass's register budget.
+// Tiles per screening step of one wave at most: the step's loop keeps the
+// registers of a full pass of as many tiles, up to 4 (a wave's share of a
+// window's tiles is short).
 __host__ __device__ constexpr int screen_tiles(int nc, int wpc) {
-    return wpc >= 3 ? (nc <= 3 ? 4 : (nc <= 6 ? 2 : 1)) : (nc <= 5 ? 4 : (nc <= 8 ? 2 : 1));
+    return pass_tiles(nc, wpc) < 4 ? pass_tiles(nc, wpc) : 4;
 }
 
 // Screen of a staged window (tiles [0, tl) of each segment) with S mask rows,
