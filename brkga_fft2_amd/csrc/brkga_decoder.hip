@@ -230,9 +230,13 @@ __device__ __forceinline__ v8i nib_frag(unsigned w) {
     return r;
 }
 
-// One fp4 MFMA, both operands e2m1, block scales 2^0 (E8M0 127).
+// One fp4 MFMA, both operands e2m1, unscaled: constant zero scale operands
+// make the compiler emit V_MFMA_F32_32X32X64_F8F6F4 itself (one instruction,
+// no scale load ahead of it, no register holding the scales).  The counts are
+// sums of products of positive e2m1 values, exact in float32, and only their
+// zero test is read.
 __device__ __forceinline__ v16f mfma_f4(const v8i &a, const v8i &b, const v16f &c) {
-    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 4, 4, 0, 127, 0, 127);
+    return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 4, 4, 0, 0, 0, 0);
 }
 
 // Accumulator of a 32x32 tile: int32 counts (int8 operands) or float32 (fp4).
