@@ -523,10 +523,20 @@ __device__ __forceinline__ unsigned screen_pass(lds_cu32 *band, int stride, int 
 #pragma unroll
     for (int t = 0; t < TW; t++) {
         if (t < tcnt) {
-            unsigned long long z = 0;
+            // the counts are non-negative: one is zero iff their minimum is
+            bool zero;
+            if constexpr (F4) {
+                float m = acc[t][0];
 #pragma unroll
-            for (int i = 0; i < 16; i++) z |= __ballot(acc[t][i] == 0);
-            if (z) bits |= 1u << t;
+                for (int i = 1; i < 16; i++) m = __builtin_fminf(m, acc[t][i]);
+                zero = m == 0.0f;
+            } else {
+                int m = acc[t][0];
+#pragma unroll
+                for (int i = 1; i < 16; i++) m = min(m, acc[t][i]);
+                zero = m == 0;
+            }
+            if (__ballot(zero)) bits |= 1u << t;
         }
     }
     return bits;
